@@ -1,0 +1,195 @@
+/*
+ * sp_hip.h -- C ABI of the MI355X (gfx950) Parker pseudo-particle pusher.
+ *
+ * The reference keeps this path behind Fortran module procedures operating on
+ * module-global state (particle_module, mhd_data_parallel, diagnostics).  Each
+ * entry point below replaces one of those procedures; the reference interface
+ * it stands in for is cited as path:line in the xiaocanli/stochastic-parker
+ * repository.  A Fortran caller binds these with iso_c_binding (interfaces in
+ * stochastic-parker_amd/fortran/sp_hip_binding.f90).
+ *
+ * Conventions
+ *  - Every function returns SP_OK (0) or a negative sp_status; the message of
+ *    the last failure on the calling host thread is sp_last_error().
+ *  - Host buffers are borrowed for the duration of the call only.
+ *  - Fields and particles live in device memory owned by the context.
+ *  - One context per host thread; calls on one context are serialised on the
+ *    context's own HIP stream and are synchronous on return.
+ *  - Integer/byte results (counts, histograms, tags, flags) are exact; the
+ *    FP64 precision mode evaluates the reference's FP64 arithmetic in the
+ *    reference's operation order.
+ */
+#ifndef SP_HIP_H
+#define SP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_HIP_ABI_VERSION 1
+
+typedef enum sp_status {
+    SP_OK = 0,
+    SP_ERR_ARG = -1,         /* invalid argument / configuration */
+    SP_ERR_HIP = -2,         /* HIP runtime error */
+    SP_ERR_STATE = -3,       /* call out of order (e.g. mover before fields) */
+    SP_ERR_CAPACITY = -4,    /* request exceeds nptl_max or a host buffer */
+    SP_ERR_UNSUPPORTED = -5, /* flag combination outside this path */
+    SP_ERR_NOMEM = -6        /* device allocation failed */
+} sp_status;
+
+typedef enum sp_precision {
+    SP_PREC_F64 = 0, /* reference arithmetic: FP32 fields, FP64 gather/coefficients/step */
+    SP_PREC_F32 = 1  /* FP32 gather and coefficients; FP64 position, momentum and time */
+} sp_precision;
+
+/* count_flag values, particle_module.f90:55-62 */
+#define SP_COUNT_FLAG_INBOX 1
+#define SP_COUNT_FLAG_OTHERS 0
+#define SP_COUNT_FLAG_ESCAPE_LX (-1)
+#define SP_COUNT_FLAG_ESCAPE_HX (-2)
+#define SP_COUNT_FLAG_ESCAPE_LY (-3)
+#define SP_COUNT_FLAG_ESCAPE_HY (-4)
+#define SP_COUNT_FLAG_ESCAPE_LZ (-5)
+#define SP_COUNT_FLAG_ESCAPE_HZ (-6)
+
+/* Everything the pusher reads from mhd_config.dat (mhd_config.f90:16-29),
+ * conf.dat (particle_module.f90:2778-2878, simulation_setup.f90:104-150) and
+ * the command line (stochastic-mhd.f90:572-946).  pindex is derived as
+ * 3 - gamma_turb (particle_module.f90:2798). */
+typedef struct sp_config {
+    int32_t ndim;               /* ndim_field (-nd): 2 or 3 */
+    int32_t nx, ny, nz;         /* interior grid nodes (mhd_config nx, ny, nz) */
+    double dx, dy, dz;
+    double xmin, ymin, zmin;
+    double xmax, ymax, zmax;
+    double lx, ly, lz;
+    int32_t pbcx, pbcy, pbcz;   /* particle BCs: 0 periodic, 1 open */
+    int32_t momentum_dependency;
+    int32_t mag_dependency;
+    int32_t charge;             /* -ch */
+    double p0, pmin, pmax, b0;
+    double gamma_turb;
+    double kpara0, kret;
+    double dt_min_rel, dt_max_rel;
+    double drift1, drift2;      /* -dp1, -dp2 */
+    int32_t check_drift_2d;     /* -cd */
+    int32_t acc_region_flag;
+    double acc_region[6];       /* acc_xmin, acc_xmax, acc_ymin, ... (0-1 normalised) */
+    int32_t time_interp;        /* -ti: two-frame linear interpolation */
+    int32_t precision;          /* sp_precision */
+    uint64_t seed;              /* Philox key */
+    uint32_t stream;            /* Philox stream word (run / variant id) */
+    int32_t reserved0;
+    int64_t nptl_max;           /* -nm */
+} sp_config;
+
+/* Structure-of-arrays view of particle_type (particle_module.f90:38-50) plus
+ * the counter-based RNG step index.  Any pointer may be NULL on download to
+ * skip that member; all must be non-NULL on upload. */
+typedef struct sp_particles {
+    double *x, *y, *z, *p, *v, *mu, *weight, *t, *dt;
+    int8_t *split_times, *count_flag;
+    int32_t *origin, *nsteps_tracked, *nsteps_pushed, *tag_injected, *tag_splitted;
+    uint32_t *rng_step;
+} sp_particles;
+
+typedef struct sp_mover_stats {
+    int64_t steps;          /* particle-steps executed by this call (redo steps included) */
+    int64_t nptl_current;   /* particles left after removal */
+    int64_t nptl_escaped;   /* cumulative count of escaped particles */
+    double leak;            /* cumulative weight leaked through open boundaries */
+    double leak_negp;       /* cumulative weight removed for p < 0 */
+    double kernel_ms;       /* device time of the push kernel (HIP events, context stream) */
+    double total_ms;        /* device time of the whole call (push + boundary pass + removal) */
+} sp_mover_stats;
+
+/* Local distribution set k (diagnostics.f90:2049-2222): bins over
+ * (mu, p, x, y, z) with npbins momentum bins on [pmin, pmax] and rx*ry*rz
+ * grid cells per spatial bin. */
+typedef struct sp_local_dist {
+    int32_t enabled, npbins, nmu, rx, ry, rz;
+    double pmin, pmax;
+} sp_local_dist;
+
+typedef struct sp_dist_config {
+    int32_t npp_global, nmu_global;
+    int32_t local_dist;           /* -ld */
+    int32_t reserved0;
+    sp_local_dist local[4];
+} sp_dist_config;
+
+/* ---- context --------------------------------------------------------- */
+/* Replaces the module initialisation chain: set_mhd_config / set_field_configuration
+ * (mhd_config.f90:43-74, simulation_setup.f90:171-253), init_field_data
+ * (mhd_data_parallel.f90:66-101), init_particles (particle_module.f90:171-241),
+ * read_particle_params (particle_module.f90:2778-2878) and the set_* calls
+ * (particle_module.f90:260-335). */
+int sp_ctx_create(const sp_config* cfg, void** out_ctx);
+/* free_particles / free_field_data (particle_module.f90:246-251,
+ * mhd_data_parallel.f90:187-196) */
+int sp_ctx_destroy(void* ctx);
+const char* sp_last_error(void);
+int sp_abi_version(void);
+
+/* ---- fields (mhd_data_parallel) --------------------------------------- */
+/* read_field_data_parallel (mhd_data_parallel.f90:224-297): the 8 node fields
+ * vx, vy, vz, rho, bx, by, bz, |B| as Fortran farray(1:8, -1:nx+2, -1:ny+2,
+ * nz-range), FP32, ghost cells included.  slot 0 = farray1, 1 = farray2. */
+int sp_fields_upload(void* ctx, int32_t slot, const float* nodes8, int64_t nvals);
+/* calc_fields_gradients (mhd_data_parallel.f90:504-605) on the device.  When
+ * farray32 is non-NULL the full 32-component farray (8 fields + 24
+ * gradients) is also copied back. */
+int sp_fields_gradients(void* ctx, int32_t slot, float* farray32);
+/* copy_fields (mhd_data_parallel.f90:1920-1923): farray1 = farray2 */
+int sp_fields_copy(void* ctx);
+
+/* ---- particles (particle_module) -------------------------------------- */
+int sp_particles_upload(void* ctx, const sp_particles* host, int64_t n);
+int sp_particles_download(void* ctx, sp_particles* host, int64_t capacity, int64_t* n);
+/* nptl_current, nptl_split, nptl_escaped, tag_max (module globals,
+ * particle_module.f90:73-85) */
+int sp_particles_counts(void* ctx, int64_t* nptl_current, int64_t* nptl_split,
+                        int64_t* nptl_escaped, int32_t* tag_max);
+/* inject_particles_spatial_uniform (particle_module.f90:454-530): the
+ * reference takes the frame index ct_mhd and reads t0 = tstamps_mhd(ct_mhd),
+ * dtf = tstamps_mhd(ct_mhd+1) - t0; they are passed directly here. */
+int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int32_t dist_flag,
+                                        double particle_v0, double t0, double dtf,
+                                        const double part_box[6], double power_index);
+/* particle_mover (particle_module.f90:1846-1974) for Parker transport with the
+ * standard kappa (nlgc = .false.), one rank: push every particle to t0 + dtf,
+ * apply the exact-bounds boundary pass and remove non-INBOX particles. */
+int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
+                      int32_t nsteps_interval, sp_mover_stats* stats);
+/* split_particle (particle_module.f90:5430-5480) */
+int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
+/* One gather + kappa + push per particle from caller-supplied states and
+ * increments (ran1, ran2, ran3, ran_p per particle, interleaved), as one
+ * iteration of the hot loop (particle_module.f90:1615-1690).  state rows are
+ * (x, y, z, p, t, dt) per particle and are updated in place; deltas receives
+ * (dx, dy, dz, dp) per particle.  fixed_dt is per particle (0/1). */
+int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
+                 const double* rands, const int8_t* fixed_dt, double* deltas);
+
+/* ---- diagnostics ------------------------------------------------------ */
+/* init_particle_distributions (diagnostics.f90:178-226) */
+int sp_distributions_init(void* ctx, const sp_dist_config* dc);
+/* calc_particle_distributions (diagnostics.f90:738-906): fglobal is
+ * (nmu_global, npp_global); flocal[k] is (nmu, npbins, nrx, nry, nrz), all
+ * Fortran order; NULL skips a set.  The device arrays are cleared afterwards
+ * (clean_particle_distributions). */
+int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal2,
+                     double* flocal3, double* flocal4);
+int sp_local_dist_dims(void* ctx, int32_t set, int64_t dims[5]);
+/* quick_check + get_pmax_global (diagnostics.f90:116-170,1691-1719):
+ * out = {nptl_current, nptl_split, ntot, leak, leak_negp, pdt_min, pdt_max,
+ *        pdt_avg, pmax} */
+int sp_quick_check(void* ctx, double out[9]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,643 @@
+// sp_api.hip -- the extern "C" entry points of libsp_hip.so (include/sp_hip.h).
+//
+// A context owns one HIP stream (default priority), the field records of one
+// or two frames, two particle buffer sets (compaction writes the other set
+// and swaps), the device counters and the diagnostics arrays.  Every entry
+// point validates its arguments, enqueues its kernels on the context stream
+// and synchronises before returning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sp_internal.h"
+#include "sp_particles.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define SP_HIP(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(SP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr double kInvFx = 1.0 / sp::kFxScale;
+
+struct Ctx {
+    sp_config cfg{};
+    sp::DevParams dp{};
+    int device = 0;
+    int ncu = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    long long nnodes = 0;
+    int recw = 0;
+    float* n8[2] = {nullptr, nullptr};
+    float* rec[2] = {nullptr, nullptr};
+    bool have_fields[2] = {false, false};
+    sp::Ptl P[2]{};
+    void* pbuf[2] = {nullptr, nullptr};
+    int cur = 0;
+    long long nptl = 0, nmax = 0, nsplit = 0, nesc = 0;
+    int tag_max = 0;
+    sp::Counters* ctr = nullptr;
+    unsigned* tile_cnt = nullptr;
+    unsigned* tile_esc = nullptr;
+    unsigned long long* tile_off = nullptr;
+    unsigned long long* totals = nullptr;
+    double* stats_part = nullptr;
+    int stats_grid = 1024;
+    int mover_blocks_per_cu[2] = {0, 0};
+    // distributions
+    bool dist_ready = false;
+    sp::DistArgs da{};
+    long long nglob = 0;
+    long long loc_size[4] = {0, 0, 0, 0};
+    long long loc_dims[4][5] = {};
+    unsigned long long* dglob = nullptr;
+    unsigned long long* dloc[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+void free_dev(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// One allocation per particle buffer set; arrays 256-byte aligned.
+int alloc_particles(long long nmax, sp::Ptl& P, void*& base) {
+    const size_t n = (size_t)nmax;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t sz_d = al(n * 8), sz_b = al(n), sz_i = al(n * 4);
+    const size_t total = 9 * sz_d + 2 * sz_b + 6 * sz_i;
+    if (hipMalloc(&base, total) != hipSuccess)
+        return fail(SP_ERR_NOMEM, "device allocation of " + std::to_string(total) + " bytes for particles failed");
+    char* c = static_cast<char*>(base);
+    double** d[9] = {&P.x, &P.y, &P.z, &P.p, &P.v, &P.mu, &P.w, &P.t, &P.dt};
+    for (auto* q : d) { *q = reinterpret_cast<double*>(c); c += sz_d; }
+    P.split_times = reinterpret_cast<int8_t*>(c); c += sz_b;
+    P.count_flag = reinterpret_cast<int8_t*>(c); c += sz_b;
+    int32_t** i4[5] = {&P.origin, &P.nsteps_tracked, &P.nsteps_pushed, &P.tag_injected, &P.tag_splitted};
+    for (auto* q : i4) { *q = reinterpret_cast<int32_t*>(c); c += sz_i; }
+    P.rng_step = reinterpret_cast<uint32_t*>(c);
+    return SP_OK;
+}
+
+int validate(const sp_config* c) {
+    if (!c) return fail(SP_ERR_ARG, "config is NULL");
+    if (c->ndim != 2 && c->ndim != 3) return fail(SP_ERR_UNSUPPORTED, "ndim must be 2 or 3");
+    if (c->nx < 1 || c->ny < 1 || c->nz < 1) return fail(SP_ERR_ARG, "nx, ny, nz must be >= 1");
+    if (c->ndim == 2 && c->nz != 1) return fail(SP_ERR_ARG, "a 2D grid has nz == 1");
+    if (!(c->dx > 0) || !(c->dy > 0) || !(c->dz > 0)) return fail(SP_ERR_ARG, "dx, dy, dz must be > 0");
+    if (!(c->lx > 0) || !(c->ly > 0) || !(c->lz > 0)) return fail(SP_ERR_ARG, "lx, ly, lz must be > 0");
+    if (!(c->p0 > 0) || !(c->pmin > 0) || !(c->pmax > c->pmin)) return fail(SP_ERR_ARG, "need p0 > 0 and 0 < pmin < pmax");
+    if (!(c->kpara0 > 0)) return fail(SP_ERR_ARG, "kpara0 must be > 0");
+    if (!(c->kret >= 0) || c->kret > 1) return fail(SP_ERR_ARG, "kret must be in [0, 1]");
+    if (!(c->dt_min_rel > 0) || !(c->dt_max_rel >= c->dt_min_rel)) return fail(SP_ERR_ARG, "need 0 < dt_min_rel <= dt_max_rel");
+    if (c->charge == 0) return fail(SP_ERR_ARG, "charge must be non-zero (the drift prefactor divides by it)");
+    if (c->drift1 == 0 && c->drift2 == 0) return fail(SP_ERR_ARG, "drift1 and drift2 are both zero (the drift speed would be infinite)");
+    if (c->pbcx < 0 || c->pbcx > 1 || c->pbcy < 0 || c->pbcy > 1 || c->pbcz < 0 || c->pbcz > 1) return fail(SP_ERR_ARG, "pbc must be 0 or 1");
+    if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32) return fail(SP_ERR_ARG, "precision must be SP_PREC_F64 or SP_PREC_F32");
+    if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
+    return SP_OK;
+}
+
+sp::DevParams make_params(const sp_config& c) {
+    sp::DevParams d{};
+    d.ndim = c.ndim;
+    d.nxg = c.nx + 4;
+    d.nyg = c.ny + 4;
+    d.nzg = c.ndim == 3 ? c.nz + 4 : c.nz;
+    d.dx = c.dx; d.dy = c.dy; d.dz = c.dz;
+    d.idx = 1.0 / c.dx; d.idy = 1.0 / c.dy; d.idz = 1.0 / c.dz;
+    d.xmin = c.xmin; d.ymin = c.ymin; d.zmin = c.zmin;
+    d.xmax = c.xmax; d.ymax = c.ymax; d.zmax = c.zmax;
+    d.lx = c.lx; d.ly = c.ly; d.lz = c.lz;
+    d.pbc[0] = c.pbcx; d.pbc[1] = c.pbcy; d.pbc[2] = c.pbcz;
+    d.momentum_dependency = c.momentum_dependency;
+    d.mag_dependency = c.mag_dependency;
+    d.pcharge = c.charge;
+    d.p0 = c.p0; d.pmin = c.pmin; d.pmax = c.pmax;
+    d.gamma_turb = c.gamma_turb;
+    d.pindex = 3.0 - c.gamma_turb;  // particle_module.f90:2798
+    d.kpara0 = c.kpara0; d.kret = c.kret;
+    d.drift1 = c.drift1; d.drift2 = c.drift2;
+    d.check_drift_2d = c.check_drift_2d;
+    d.acc_region_flag = c.acc_region_flag;
+    d.time_interp = c.time_interp;
+    d.precision = c.precision;
+    for (int i = 0; i < 6; ++i) d.acc_region[i] = c.acc_region[i];
+    d.key0 = (uint32_t)c.seed;
+    d.key1 = (uint32_t)(c.seed >> 32);
+    d.stream = c.stream;
+    return d;
+}
+
+Ctx* live_ctx(void* p, int& rc) {
+    if (!p) {
+        rc = fail(SP_ERR_ARG, "context is NULL");
+        return nullptr;
+    }
+    rc = SP_OK;
+    return as_ctx(p);
+}
+
+// Occupancy-sized persistent grid for the mover.
+int mover_grid(Ctx* c, long long n) {
+    const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
+    const int per_cu = std::max(1, c->mover_blocks_per_cu[prec]);
+    const long long need = (n + 255) / 256;
+    const long long cap = (long long)c->ncu * per_cu;
+    return (int)std::max<long long>(1, std::min(need, cap));
+}
+
+void destroy(Ctx* c) {
+    if (!c) return;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (int i = 0; i < 2; ++i) {
+        free_dev(c->n8[i]);
+        free_dev(c->rec[i]);
+        free_dev(c->pbuf[i]);
+    }
+    free_dev(c->ctr);
+    free_dev(c->tile_cnt);
+    free_dev(c->tile_esc);
+    free_dev(c->tile_off);
+    free_dev(c->totals);
+    free_dev(c->stats_part);
+    free_dev(c->dglob);
+    for (auto* q : c->dloc) free_dev(q);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sp_last_error(void) { return g_err.c_str(); }
+
+int sp_abi_version(void) { return SP_HIP_ABI_VERSION; }
+
+int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
+    if (!out_ctx) return fail(SP_ERR_ARG, "out_ctx is NULL");
+    *out_ctx = nullptr;
+    int rc = validate(cfg);
+    if (rc != SP_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(SP_ERR_HIP, "no HIP device is available");
+    Ctx* c = new Ctx();
+    c->cfg = *cfg;
+    c->dp = make_params(*cfg);
+    c->nmax = cfg->nptl_max;
+    c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
+    c->recw = cfg->ndim == 2 ? sp::kRec2D : sp::kRec3D;
+    auto bail = [&](int code) {
+        destroy(c);
+        return code;
+    };
+#define SP_HIP_C(expr)                                                                           \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return bail(fail(SP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_))); \
+    } while (0)
+    SP_HIP_C(hipGetDevice(&c->device));
+    hipDeviceProp_t prop;
+    SP_HIP_C(hipGetDeviceProperties(&prop, c->device));
+    c->ncu = prop.multiProcessorCount;
+    SP_HIP_C(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    for (auto& e : c->ev) SP_HIP_C(hipEventCreate(&e));
+    SP_HIP_C(hipMalloc(&c->ctr, sizeof(sp::Counters)));
+    SP_HIP_C(hipMemsetAsync(c->ctr, 0, sizeof(sp::Counters), c->st));
+    for (int i = 0; i < 2; ++i) {
+        rc = alloc_particles(c->nmax, c->P[i], c->pbuf[i]);
+        if (rc != SP_OK) return bail(rc);
+    }
+    const long long ntiles = std::max<long long>(1, sp::num_tiles(c->nmax));
+    SP_HIP_C(hipMalloc(&c->tile_cnt, sizeof(unsigned) * ntiles));
+    SP_HIP_C(hipMalloc(&c->tile_esc, sizeof(unsigned) * ntiles));
+    SP_HIP_C(hipMalloc(&c->tile_off, sizeof(unsigned long long) * ntiles));
+    SP_HIP_C(hipMalloc(&c->totals, sizeof(unsigned long long) * 2));
+    SP_HIP_C(hipMalloc(&c->stats_part, sizeof(double) * 5 * c->stats_grid));
+    // occupancy of the persistent mover decides its grid
+    c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, cfg->time_interp != 0));
+    c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, cfg->time_interp != 0));
+    SP_HIP_C(hipStreamSynchronize(c->st));
+#undef SP_HIP_C
+    *out_ctx = c;
+    return SP_OK;
+}
+
+int sp_ctx_destroy(void* ctx) {
+    if (!ctx) return SP_OK;
+    destroy(as_ctx(ctx));
+    return SP_OK;
+}
+
+int sp_fields_upload(void* ctx, int32_t slot, const float* nodes8, int64_t nvals) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (slot < 0 || slot > 1) return fail(SP_ERR_ARG, "slot must be 0 or 1");
+    if (slot == 1 && !c->cfg.time_interp) return fail(SP_ERR_STATE, "slot 1 needs time_interp");
+    if (!nodes8) return fail(SP_ERR_ARG, "nodes8 is NULL");
+    if (nvals != c->nnodes * 8)
+        return fail(SP_ERR_ARG, "nodes8 must hold 8 * nxg * nyg * nzg = " + std::to_string(c->nnodes * 8) + " floats");
+    if (!c->n8[slot]) SP_HIP(hipMalloc(&c->n8[slot], sizeof(float) * 8 * c->nnodes));
+    SP_HIP(hipMemcpyAsync(c->n8[slot], nodes8, sizeof(float) * 8 * c->nnodes, hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->have_fields[slot] = false;
+    return SP_OK;
+}
+
+int sp_fields_gradients(void* ctx, int32_t slot, float* farray32) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (slot < 0 || slot > 1) return fail(SP_ERR_ARG, "slot must be 0 or 1");
+    if (!c->n8[slot]) return fail(SP_ERR_STATE, "fields of this slot were not uploaded");
+    if (!c->rec[slot]) SP_HIP(hipMalloc(&c->rec[slot], sizeof(float) * c->recw * c->nnodes));
+    float* fa = nullptr;
+    if (farray32) SP_HIP(hipMalloc(&fa, sizeof(float) * 32 * c->nnodes));
+    hipError_t e = sp::launch_gradients(c->dp, c->n8[slot], c->rec[slot], fa, c->st);
+    if (e == hipSuccess && farray32)
+        e = hipMemcpyAsync(farray32, fa, sizeof(float) * 32 * c->nnodes, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    free_dev(fa);
+    if (e != hipSuccess) return fail(SP_ERR_HIP, std::string("field gradients failed: ") + hipGetErrorString(e));
+    c->have_fields[slot] = true;
+    return SP_OK;
+}
+
+int sp_fields_copy(void* ctx) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->cfg.time_interp) return fail(SP_ERR_STATE, "copy_fields needs time_interp");
+    if (!c->have_fields[1]) return fail(SP_ERR_STATE, "frame 2 is not prepared");
+    if (!c->rec[0]) SP_HIP(hipMalloc(&c->rec[0], sizeof(float) * c->recw * c->nnodes));
+    if (!c->n8[0]) SP_HIP(hipMalloc(&c->n8[0], sizeof(float) * 8 * c->nnodes));
+    SP_HIP(hipMemcpyAsync(c->rec[0], c->rec[1], sizeof(float) * c->recw * c->nnodes, hipMemcpyDeviceToDevice, c->st));
+    SP_HIP(hipMemcpyAsync(c->n8[0], c->n8[1], sizeof(float) * 8 * c->nnodes, hipMemcpyDeviceToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->have_fields[0] = true;
+    return SP_OK;
+}
+
+int sp_particles_upload(void* ctx, const sp_particles* h, int64_t n) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!h) return fail(SP_ERR_ARG, "particles is NULL");
+    if (n < 0 || n > c->nmax) return fail(SP_ERR_CAPACITY, "n exceeds nptl_max");
+    const sp::Ptl& P = c->P[c->cur];
+    const void* src[17] = {h->x, h->y, h->z, h->p, h->v, h->mu, h->weight, h->t, h->dt,
+                           h->split_times, h->count_flag, h->origin, h->nsteps_tracked,
+                           h->nsteps_pushed, h->tag_injected, h->tag_splitted, h->rng_step};
+    void* dst[17] = {P.x, P.y, P.z, P.p, P.v, P.mu, P.w, P.t, P.dt, P.split_times, P.count_flag,
+                     P.origin, P.nsteps_tracked, P.nsteps_pushed, P.tag_injected, P.tag_splitted,
+                     P.rng_step};
+    const size_t sz[17] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 1, 1, 4, 4, 4, 4, 4, 4};
+    for (int k = 0; k < 17; ++k)
+        if (!src[k] && n > 0) return fail(SP_ERR_ARG, "every particle array is required on upload");
+    for (int k = 0; k < 17 && n > 0; ++k)
+        SP_HIP(hipMemcpyAsync(dst[k], src[k], sz[k] * (size_t)n, hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->nptl = n;
+    for (int64_t i = 0; i < n; ++i) c->tag_max = std::max(c->tag_max, h->tag_injected[i] + 1);
+    return SP_OK;
+}
+
+int sp_particles_download(void* ctx, sp_particles* h, int64_t capacity, int64_t* n) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!h) return fail(SP_ERR_ARG, "particles is NULL");
+    if (n) *n = c->nptl;
+    if (capacity < c->nptl) return fail(SP_ERR_CAPACITY, "host capacity is smaller than nptl_current");
+    const sp::Ptl& P = c->P[c->cur];
+    void* dst[17] = {h->x, h->y, h->z, h->p, h->v, h->mu, h->weight, h->t, h->dt,
+                     h->split_times, h->count_flag, h->origin, h->nsteps_tracked,
+                     h->nsteps_pushed, h->tag_injected, h->tag_splitted, h->rng_step};
+    const void* src[17] = {P.x, P.y, P.z, P.p, P.v, P.mu, P.w, P.t, P.dt, P.split_times,
+                           P.count_flag, P.origin, P.nsteps_tracked, P.nsteps_pushed,
+                           P.tag_injected, P.tag_splitted, P.rng_step};
+    const size_t sz[17] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 1, 1, 4, 4, 4, 4, 4, 4};
+    for (int k = 0; k < 17 && c->nptl > 0; ++k)
+        if (dst[k]) SP_HIP(hipMemcpyAsync(dst[k], src[k], sz[k] * (size_t)c->nptl, hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    return SP_OK;
+}
+
+int sp_particles_counts(void* ctx, int64_t* nptl_current, int64_t* nptl_split,
+                        int64_t* nptl_escaped, int32_t* tag_max) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (nptl_current) *nptl_current = c->nptl;
+    if (nptl_split) *nptl_split = c->nsplit;
+    if (nptl_escaped) *nptl_escaped = c->nesc;
+    if (tag_max) *tag_max = c->tag_max;
+    return SP_OK;
+}
+
+int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int32_t dist_flag,
+                                        double particle_v0, double t0, double dtf,
+                                        const double part_box[6], double power_index) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (nptl < 0) return fail(SP_ERR_ARG, "nptl must be >= 0");
+    if (dist_flag < 0 || dist_flag > 2) return fail(SP_ERR_ARG, "dist_flag must be 0, 1 or 2");
+    if (!part_box) return fail(SP_ERR_ARG, "part_box is NULL");
+    if ((long long)c->tag_max + nptl > 2147483647ll) return fail(SP_ERR_CAPACITY, "tag_max would overflow int32");
+    sp::InjectArgs ia{};
+    for (int i = 0; i < 6; ++i) ia.box[i] = part_box[i];
+    ia.dt = dt; ia.v0 = particle_v0; ia.t0 = t0; ia.dtf = dtf; ia.power_index = power_index;
+    ia.dist_flag = dist_flag;
+    SP_HIP(sp::launch_inject(c->dp, c->P[c->cur], c->nptl, nptl, c->nmax, c->tag_max, ia, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->nptl = std::min<long long>(c->nptl + nptl, c->nmax);
+    c->tag_max += (int)nptl;
+    return SP_OK;
+}
+
+int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
+                      int32_t nsteps_interval, sp_mover_stats* stats) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
+    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
+    if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
+    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    sp::MoverArgs a{};
+    a.prm = c->dp;
+    a.P = c->P[c->cur];
+    a.rec1 = c->rec[0];
+    a.rec2 = c->cfg.time_interp ? c->rec[1] : c->rec[0];
+    a.ctr = c->ctr;
+    a.n = c->nptl;
+    a.t0 = t0;
+    a.dtf = dtf;
+    a.dt_fine = dtf / num_fine_steps;
+    a.dt_limit = dtf + a.dt_fine * (double)0.1f;  // 0.1 is a default-real literal (:1596)
+    a.dt_min = c->cfg.dt_min_rel * dtf;            // set_dt_min_max (:5519-5524)
+    a.dt_max = c->cfg.dt_max_rel * dtf;
+    a.nsteps_interval = nsteps_interval;
+    // clear the work and step counters (leak counters are cumulative)
+    SP_HIP(hipMemsetAsync(&c->ctr->work, 0, 2 * sizeof(unsigned long long), c->st));
+    SP_HIP(hipEventRecord(c->ev[0], c->st));
+    if (c->nptl > 0) {
+        const int grid = mover_grid(c, c->nptl);
+        if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(a, grid, 256, c->st));
+        else SP_HIP(sp::launch_mover_f64(a, grid, 256, c->st));
+    }
+    SP_HIP(hipEventRecord(c->ev[1], c->st));
+    // remove_particles: stable compaction into the other buffer set
+    long long kept = 0, esc = 0;
+    if (c->nptl > 0) {
+        const long long nt = sp::num_tiles(c->nptl);
+        const sp::SplitArgs none{};
+        SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 0, none, c->tile_cnt, c->tile_esc, c->st));
+        SP_HIP(sp::launch_tile_scan(c->tile_cnt, c->tile_esc, nt, c->tile_off, c->totals, c->st));
+        SP_HIP(sp::launch_compact_scatter(c->P[c->cur], c->P[1 - c->cur], c->nptl, c->tile_off, c->st));
+    }
+    SP_HIP(hipEventRecord(c->ev[2], c->st));
+    sp::Counters hc{};
+    SP_HIP(hipMemcpyAsync(&hc, c->ctr, sizeof(hc), hipMemcpyDeviceToHost, c->st));
+    unsigned long long tot[2] = {0, 0};
+    if (c->nptl > 0) SP_HIP(hipMemcpyAsync(tot, c->totals, sizeof(tot), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    if (c->nptl > 0) {
+        kept = (long long)tot[0];
+        esc = (long long)tot[1];
+        c->cur = 1 - c->cur;
+        c->nptl = kept;
+        c->nesc += esc;
+    }
+    float ms_push = 0.f, ms_all = 0.f;
+    SP_HIP(hipEventElapsedTime(&ms_push, c->ev[0], c->ev[1]));
+    SP_HIP(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
+    if (stats) {
+        stats->steps = (int64_t)hc.steps;
+        stats->nptl_current = c->nptl;
+        stats->nptl_escaped = c->nesc;
+        stats->leak = (double)hc.leak_fx * kInvFx;
+        stats->leak_negp = (double)hc.negp_fx * kInvFx;
+        stats->kernel_ms = ms_push;
+        stats->total_ms = ms_all;
+    }
+    return SP_OK;
+}
+
+int sp_split_particle(void* ctx, double split_ratio, double pmin_split) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (c->nptl == 0) return SP_OK;
+    sp::SplitArgs sa{split_ratio, pmin_split, c->cfg.p0, c->cfg.pmax};
+    const long long nt = sp::num_tiles(c->nptl);
+    SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 1, sa, c->tile_cnt, nullptr, c->st));
+    SP_HIP(sp::launch_tile_scan(c->tile_cnt, nullptr, nt, c->tile_off, c->totals, c->st));
+    SP_HIP(sp::launch_split_apply(c->P[c->cur], c->nptl, c->nmax, sa, c->tile_off, c->st));
+    unsigned long long tot[2] = {0, 0};
+    SP_HIP(hipMemcpyAsync(tot, c->totals, sizeof(tot), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    const long long ncand = (long long)tot[0];
+    const long long room = c->nmax - c->nptl;
+    const long long done = std::min(ncand, room);
+    c->nsplit += done;
+    c->nptl += done;
+    return SP_OK;
+}
+
+int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, const double* rands,
+                 const int8_t* fixed_dt, double* deltas) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (n < 0) return fail(SP_ERR_ARG, "n must be >= 0");
+    if (n == 0) return SP_OK;
+    if (!state || !rands || !fixed_dt || !deltas) return fail(SP_ERR_ARG, "NULL buffer");
+    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
+    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
+    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    double *ds = nullptr, *dr = nullptr, *dd = nullptr;
+    int8_t* df = nullptr;
+    hipError_t e = hipMalloc(&ds, sizeof(double) * 6 * n);
+    if (e == hipSuccess) e = hipMalloc(&dr, sizeof(double) * 4 * n);
+    if (e == hipSuccess) e = hipMalloc(&dd, sizeof(double) * 4 * n);
+    if (e == hipSuccess) e = hipMalloc(&df, (size_t)n);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, state, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, rands, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(df, fixed_dt, (size_t)n, hipMemcpyHostToDevice, c->st);
+    const double dt_min = c->cfg.dt_min_rel * dtf, dt_max = c->cfg.dt_max_rel * dtf;
+    const float* r2 = c->cfg.time_interp ? c->rec[1] : c->rec[0];
+    if (e == hipSuccess) {
+        if (c->cfg.precision == SP_PREC_F32)
+            e = sp::launch_step_once_f32(c->dp, c->rec[0], r2, n, t0, dtf, dt_min, dt_max, ds, dr, df, dd, c->st);
+        else
+            e = sp::launch_step_once_f64(c->dp, c->rec[0], r2, n, t0, dtf, dt_min, dt_max, ds, dr, df, dd, c->st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(state, ds, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(deltas, dd, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    free_dev(ds);
+    free_dev(dr);
+    free_dev(dd);
+    free_dev(df);
+    if (e != hipSuccess) return fail(SP_ERR_HIP, std::string("step_once failed: ") + hipGetErrorString(e));
+    return SP_OK;
+}
+
+int sp_distributions_init(void* ctx, const sp_dist_config* dc) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!dc) return fail(SP_ERR_ARG, "dist config is NULL");
+    if (dc->npp_global < 1 || dc->nmu_global < 1) return fail(SP_ERR_ARG, "npp_global and nmu_global must be >= 1");
+    if ((long long)dc->npp_global * dc->nmu_global > 8192)
+        return fail(SP_ERR_UNSUPPORTED, "npp_global * nmu_global is limited to 8192 bins");
+    sp::DistArgs& d = c->da;
+    d = sp::DistArgs{};
+    const sp_config& cf = c->cfg;
+    d.npp = dc->npp_global;
+    d.nmu = dc->nmu_global;
+    d.local_dist = dc->local_dist;
+    d.pmin = cf.pmin;
+    d.pmax = cf.pmax;
+    // init_particle_distributions (diagnostics.f90:196-207)
+    d.pmin_log = std::log10(cf.pmin);
+    d.dp_log = (std::log10(cf.pmax) - d.pmin_log) / dc->npp_global;
+    d.dmu = (double)(2.0f / (float)dc->nmu_global);
+    d.xmin = cf.xmin; d.ymin = cf.ymin; d.zmin = cf.zmin;
+    for (auto*& q : c->dloc) { free_dev(q); q = nullptr; }
+    free_dev(c->dglob);
+    c->dglob = nullptr;
+    c->nglob = (long long)d.npp * d.nmu;
+    SP_HIP(hipMalloc(&c->dglob, sizeof(unsigned long long) * c->nglob));
+    for (int k = 0; k < 4; ++k) {
+        const sp_local_dist& L = dc->local[k];
+        sp::LocalArgs& A = d.loc[k];
+        c->loc_size[k] = 0;
+        std::memset(c->loc_dims[k], 0, sizeof(c->loc_dims[k]));
+        A.enabled = dc->local_dist && L.enabled;
+        if (!A.enabled) continue;
+        if (L.npbins < 1 || L.nmu < 1 || L.rx < 1 || L.ry < 1 || L.rz < 1 || !(L.pmin > 0) || !(L.pmax > L.pmin))
+            return fail(SP_ERR_ARG, "invalid local distribution set " + std::to_string(k + 1));
+        // init_local_particle_distributions (diagnostics.f90:263-284) and
+        // read_diagnostics_params (:2121-2126)
+        A.npbins = L.npbins;
+        A.nmu = L.nmu;
+        A.nr[0] = (cf.nx + L.rx - 1) / L.rx;
+        A.nr[1] = (cf.ny + L.ry - 1) / L.ry;
+        A.nr[2] = (cf.nz + L.rz - 1) / L.rz;
+        A.dd[0] = cf.lx / (double)A.nr[0];
+        A.dd[1] = cf.ly / (double)A.nr[1];
+        A.dd[2] = cf.lz / (double)A.nr[2];
+        A.pmin_log = std::log10(L.pmin);
+        A.dp_log = (std::log10(L.pmax) - A.pmin_log) / L.npbins;
+        A.dmu = (double)(2.0f / (float)L.nmu);
+        c->loc_dims[k][0] = L.nmu;
+        c->loc_dims[k][1] = L.npbins;
+        c->loc_dims[k][2] = A.nr[0];
+        c->loc_dims[k][3] = A.nr[1];
+        c->loc_dims[k][4] = A.nr[2];
+        c->loc_size[k] = (long long)L.nmu * L.npbins * A.nr[0] * A.nr[1] * A.nr[2];
+        SP_HIP(hipMalloc(&c->dloc[k], sizeof(unsigned long long) * c->loc_size[k]));
+    }
+    c->dist_ready = true;
+    return SP_OK;
+}
+
+int sp_local_dist_dims(void* ctx, int32_t set, int64_t dims[5]) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->dist_ready) return fail(SP_ERR_STATE, "distributions are not initialised");
+    if (set < 1 || set > 4 || !dims) return fail(SP_ERR_ARG, "set must be 1..4");
+    for (int i = 0; i < 5; ++i) dims[i] = c->loc_dims[set - 1][i];
+    return SP_OK;
+}
+
+int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal2, double* flocal3,
+                     double* flocal4) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->dist_ready) return fail(SP_ERR_STATE, "distributions are not initialised");
+    SP_HIP(hipMemsetAsync(c->dglob, 0, sizeof(unsigned long long) * c->nglob, c->st));
+    unsigned long long* gl[4];
+    for (int k = 0; k < 4; ++k) {
+        gl[k] = c->dloc[k];
+        if (c->dloc[k]) SP_HIP(hipMemsetAsync(c->dloc[k], 0, sizeof(unsigned long long) * c->loc_size[k], c->st));
+    }
+    const int grid = (int)std::max<long long>(1, std::min<long long>((c->nptl + 255) / 256, (long long)c->ncu * 8));
+    SP_HIP(sp::launch_dist(c->P[c->cur], c->nptl, c->da, c->dglob, gl, grid, c->st));
+    std::vector<unsigned long long> hb((size_t)c->nglob);
+    SP_HIP(hipMemcpyAsync(hb.data(), c->dglob, sizeof(unsigned long long) * c->nglob, hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    if (fglobal)
+        for (long long i = 0; i < c->nglob; ++i) fglobal[i] = (double)hb[i] * kInvFx;
+    double* outs[4] = {flocal1, flocal2, flocal3, flocal4};
+    for (int k = 0; k < 4; ++k) {
+        if (!outs[k] || !c->dloc[k]) continue;
+        std::vector<unsigned long long> hl((size_t)c->loc_size[k]);
+        SP_HIP(hipMemcpy(hl.data(), c->dloc[k], sizeof(unsigned long long) * c->loc_size[k], hipMemcpyDeviceToHost));
+        for (long long i = 0; i < c->loc_size[k]; ++i) outs[k][i] = (double)hl[i] * kInvFx;
+    }
+    return SP_OK;
+}
+
+int sp_quick_check(void* ctx, double out[9]) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!out) return fail(SP_ERR_ARG, "out is NULL");
+    const int grid = (int)std::max<long long>(1, std::min<long long>((c->nptl + 255) / 256, c->stats_grid));
+    SP_HIP(sp::launch_stats(c->P[c->cur], c->nptl, c->stats_part, grid, c->st));
+    std::vector<double> part((size_t)5 * grid);
+    sp::Counters hc{};
+    SP_HIP(hipMemcpyAsync(part.data(), c->stats_part, sizeof(double) * 5 * grid, hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipMemcpyAsync(&hc, c->ctr, sizeof(hc), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    unsigned long long W = 0;
+    double S = 0.0, mn = 1.0, mx = 0.0, pm = 0.0;  // quick_check starts from 1 and 0
+    for (int b = 0; b < grid; ++b) {
+        unsigned long long w;
+        std::memcpy(&w, &part[5 * b], sizeof(w));
+        W += w;
+        S += part[5 * b + 1];
+        mn = std::min(mn, part[5 * b + 2]);
+        mx = std::max(mx, part[5 * b + 3]);
+        pm = std::max(pm, part[5 * b + 4]);
+    }
+    out[0] = (double)c->nptl;
+    out[1] = (double)c->nsplit;
+    out[2] = (double)W * kInvFx;
+    out[3] = (double)hc.leak_fx * kInvFx;
+    out[4] = (double)hc.negp_fx * kInvFx;
+    out[5] = mn;
+    out[6] = mx;
+    out[7] = c->nptl > 0 ? S / (double)c->nptl : 0.0;
+    out[8] = pm;
+    return SP_OK;
+}
+
+}  // extern "C"

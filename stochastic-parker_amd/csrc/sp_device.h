@@ -1,0 +1,102 @@
+// sp_device.h -- device helpers shared by the mover and particle kernels.
+#pragma once
+
+#include "sp_internal.h"
+
+namespace sp {
+
+constexpr int kWave = 64;
+
+// Philox4x32-10 (Salmon et al., SC'11).  One block of four 32-bit outputs
+// feeds one Parker step: ran1, ran2, ran3 and the momentum draw.
+struct U4 {
+    uint32_t a, b, c, d;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                           uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// genrand_double1 mapping: closed [0, 1]
+__device__ __forceinline__ double u01(uint32_t r) { return (double)r * (1.0 / 4294967295.0); }
+
+// Particle boundary condition for one rank (particle_module.f90:1984-2129):
+// periodic axes wrap by (hi - lo) of the bounds passed in, open axes mark the
+// particle as escaped and add its weight to leak.  Returns the weight leaked.
+struct Bounds {
+    double lo0, lo1, lo2, hi0, hi1, hi2;
+};
+
+__device__ __forceinline__ double boundary_condition(const DevParams& prm, double& x, double& y,
+                                                     double& z, int& cf, double w, Bounds bd) {
+    const double lo[3] = {bd.lo0, bd.lo1, bd.lo2};
+    const double hi[3] = {bd.hi0, bd.hi1, bd.hi2};
+    double leaked = 0.0;
+    if (x < lo[0] && cf == SP_COUNT_FLAG_INBOX) {
+        if (prm.pbc[0] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LX; }
+        else x = x - lo[0] + hi[0];
+    } else if (x > hi[0] && cf == SP_COUNT_FLAG_INBOX) {
+        if (prm.pbc[0] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HX; }
+        else x = x - hi[0] + lo[0];
+    }
+    if (prm.ndim > 1) {
+        if (y < lo[1] && cf == SP_COUNT_FLAG_INBOX) {
+            if (prm.pbc[1] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LY; }
+            else y = y - lo[1] + hi[1];
+        } else if (y > hi[1] && cf == SP_COUNT_FLAG_INBOX) {
+            if (prm.pbc[1] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HY; }
+            else y = y - hi[1] + lo[1];
+        }
+    }
+    if (prm.ndim == 3) {
+        if (z < lo[2] && cf == SP_COUNT_FLAG_INBOX) {
+            if (prm.pbc[2] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LZ; }
+            else z = z - lo[2] + hi[2];
+        } else if (z > hi[2] && cf == SP_COUNT_FLAG_INBOX) {
+            if (prm.pbc[2] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HZ; }
+            else z = z - hi[2] + lo[2];
+        }
+    }
+    return leaked;
+}
+
+// Weight -> fixed point (2^-32 units).  Split weights are 2^-k, k <= 32 in
+// any run the reference supports, so the conversion and every sum are exact.
+__device__ __forceinline__ unsigned long long weight_fx(double w) {
+    return (unsigned long long)(w * kFxScale);
+}
+
+// Cell index of a position along one axis, clamped into the node array so
+// that a corrupt or out-of-range position can never address memory outside
+// it.  For in-range positions the clamp is inactive and the value equals
+// the reference's floor(p).
+__device__ __forceinline__ int clamp_cell(double fl, int ncell_max) {
+    // fmax/fmin also map NaN to the lower bound
+    const double c = fmin(fmax(fl, -2.0), (double)ncell_max);
+    return (int)c;
+}
+
+// Per-particle dynamic state carried through the steps of one interval.
+struct State {
+    double x, y, z, p, t, dt;
+};
+
+}  // namespace sp

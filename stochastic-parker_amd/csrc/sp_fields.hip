@@ -1,0 +1,102 @@
+// sp_fields.hip -- field preparation on the device: the 24 gradient
+// components of calc_fields_gradients (mhd_data_parallel.f90:504-605) and the
+// packing of each node into the record the mover gathers.
+//
+// Built with -ffp-contract=off: the FP32 differences, the FP64 scaling and the
+// FP32 store round exactly as in the Fortran, so the gradients are bit-exact
+// with the CPU restatement.
+#include "sp_internal.h"
+
+namespace sp {
+
+namespace {
+
+// Derivative of 8 node fields along one axis at position i of an axis with
+// n nodes, stride `st` (in nodes) in the node-major 8-field array.
+__device__ __forceinline__ void axis_grad(const float* __restrict__ n8, long long node, int i, int n,
+                                          long long st, double scale, float* out, int comp0) {
+    if (n <= 1) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) out[comp0 + 3 * c] = 0.0f;
+        return;
+    }
+    if (i >= 1 && i <= n - 2) {
+        const float* a = n8 + (node + st) * 8;
+        const float* b = n8 + (node - st) * 8;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float d = a[c] - b[c];
+            out[comp0 + 3 * c] = (float)((double)d * scale);
+        }
+    } else if (i == 0) {
+        const float* f0 = n8 + node * 8;
+        const float* f1 = n8 + (node + st) * 8;
+        const float* f2 = n8 + (node + 2 * st) * 8;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float d = (-3.0f * f0[c] + 4.0f * f1[c]) - f2[c];
+            out[comp0 + 3 * c] = (float)((double)d * scale);
+        }
+    } else {
+        const float* f0 = n8 + node * 8;
+        const float* f1 = n8 + (node - st) * 8;
+        const float* f2 = n8 + (node - 2 * st) * 8;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const float d = (3.0f * f0[c] - 4.0f * f1[c]) + f2[c];
+            out[comp0 + 3 * c] = (float)((double)d * scale);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) gradients_kernel(DevParams prm, double idxh, double idyh,
+                                                        double idzh, const float* __restrict__ n8,
+                                                        float* __restrict__ rec,
+                                                        float* __restrict__ fa32) {
+    const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= total) return;
+    const int i = (int)(node % prm.nxg);
+    const int j = (int)((node / prm.nxg) % prm.nyg);
+    const int k = (int)(node / ((long long)prm.nxg * prm.nyg));
+    float out[32];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = n8[node * 8 + c];
+    axis_grad(n8, node, i, prm.nxg, 1, idxh, out, 8);
+    axis_grad(n8, node, j, prm.nyg, prm.nxg, idyh, out, 9);
+    axis_grad(n8, node, k, prm.nzg, (long long)prm.nxg * prm.nyg, idzh, out, 10);
+    if (prm.ndim == 2) {
+        float4* r4 = reinterpret_cast<float4*>(rec + node * kRec2D);
+        float v[kRec2D];
+#pragma unroll
+        for (int s = 0; s < kRec2D; ++s) v[s] = s < 15 ? out[rec2_src(s)] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kRec2D / 4; ++u) r4[u] = make_float4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+    } else {
+        float4* r4 = reinterpret_cast<float4*>(rec + node * kRec3D);
+        float v[kRec3D];
+#pragma unroll
+        for (int s = 0; s < kRec3D; ++s) v[s] = s < 21 ? out[rec3_src(s)] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < kRec3D / 4; ++u) r4[u] = make_float4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+    }
+    if (fa32) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) fa32[node * 32 + c] = out[c];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gradients(const DevParams& prm, const float* nodes8, float* rec, float* farray32,
+                            hipStream_t s) {
+    const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
+    const int block = 256;
+    const long long grid = (total + block - 1) / block;
+    const double idxh = 0.5 / prm.dx, idyh = 0.5 / prm.dy, idzh = 0.5 / prm.dz;
+    hipLaunchKernelGGL(gradients_kernel, dim3((unsigned)grid), dim3(block), 0, s, prm, idxh, idyh,
+                       idzh, nodes8, rec, farray32);
+    return hipGetLastError();
+}
+
+}  // namespace sp

@@ -1,0 +1,444 @@
+// sp_particles.hip -- particle population kernels: injection, removal of
+// non-INBOX particles (stable stream compaction), splitting, distribution
+// histograms and the quick_check reductions.
+//
+// Compaction and splitting share one three-pass scheme: a per-tile count, a
+// single-workgroup exclusive scan of the tile counts, and a per-tile scatter
+// that recomputes its in-tile offsets from wave ballots.  Orders are stable,
+// so the outcome is deterministic; the reference's remove_particles swaps
+// removed particles to the tail instead, which changes only the array order.
+#include "sp_device.h"
+#include "sp_particles.h"
+
+namespace sp {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = kBlock * kTileItems;
+
+// ---------------------------------------------------------------------------
+// Injection: inject_particles_spatial_uniform + inject_one_particle
+// (particle_module.f90:385-530).  Particle i of the batch takes tag
+// tag0 + i and draws x, y, z, mu, [momentum draws], t from its own Philox
+// stream (counter = tag, 0, block, stream), 4 draws per block.
+struct DrawStream {
+    uint32_t tag, blk, k0, k1, stream;
+    U4 buf;
+    int used;
+    __device__ double next() {
+        if (used == 4) {
+            buf = philox4x32_10(tag, 0u, blk, stream, k0, k1);
+            ++blk;
+            used = 0;
+        }
+        const uint32_t r = used == 0 ? buf.a : used == 1 ? buf.b : used == 2 ? buf.c : buf.d;
+        ++used;
+        return u01(r);
+    }
+};
+
+__global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, long long start,
+                                                        long long nptl, long long nmax, int tag0,
+                                                        InjectArgs ia) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nptl) return;
+    long long slot = start + i;
+    if (slot >= nmax) {
+        // past nptl_max every new particle lands in the last slot; the last one stays
+        if (i != nptl - 1) return;
+        slot = nmax - 1;
+    }
+    DrawStream ds{(uint32_t)(tag0 + (int)i), 0u, prm.key0, prm.key1, prm.stream, U4{0, 0, 0, 0}, 4};
+    const double mu_max = (double)0.99f;
+    const double xt = ds.next() * (ia.box[3] - ia.box[0]) + ia.box[0];
+    const double yt = ds.next() * (ia.box[4] - ia.box[1]) + ia.box[1];
+    const double zt = ds.next() * (ia.box[5] - ia.box[2]) + ia.box[2];
+    const double mu = mu_max * (2.0 * ds.next() - 1.0);
+    double p = prm.p0;
+    if (ia.dist_flag == 0) {
+        double ftest = 1.0, fxp = 0.5, ptmp = 0.0;
+        int guard = 0;
+        while (ftest > fxp && guard < 100000) {
+            ptmp = (ds.next() * (prm.pmax - prm.pmin) + prm.pmin) / prm.p0;
+            fxp = (ptmp * ptmp) * exp(-(ptmp * ptmp));
+            ftest = ds.next() * (double)0.37f;
+            ++guard;
+        }
+        p = ptmp * prm.p0;
+    } else if (ia.dist_flag == 1) {
+        p = prm.p0;
+    } else if (ia.dist_flag == 2) {
+        const double r01 = ds.next();
+        const double pi = ia.power_index;
+        if ((int)pi == 1) {
+            p = pow(prm.pmax / prm.p0, r01) * prm.p0;
+        } else {
+            const double norm = pow(prm.pmax, -pi + 1) - pow(prm.p0, -pi + 1);
+            p = pow(r01 * norm + pow(prm.p0, -pi + 1), 1.0 / (-pi + 1));
+        }
+    }
+    P.x[slot] = xt;
+    P.y[slot] = yt;
+    P.z[slot] = zt;
+    P.p[slot] = p;
+    P.v[slot] = ia.v0 * p / prm.p0;
+    P.mu[slot] = mu;
+    P.w[slot] = 1.0;
+    P.t[slot] = ia.t0 + ds.next() * ia.dtf;
+    P.dt[slot] = ia.dt;
+    P.split_times[slot] = 0;
+    P.count_flag[slot] = SP_COUNT_FLAG_INBOX;
+    P.origin[slot] = 0;
+    P.nsteps_tracked[slot] = 0;
+    P.nsteps_pushed[slot] = 0;
+    P.tag_injected[slot] = tag0 + (int)i;
+    P.tag_splitted[slot] = 1;
+    P.rng_step[slot] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Predicates for the scan passes.
+// mode 0: keep (count_flag == INBOX); mode 1: split candidate.
+__device__ __forceinline__ double powi_rl(double x, int m) {
+    // right-to-left binary powering, the method of libgcc's __powidf2 that
+    // gfortran uses for real ** integer (particle_module.f90:5441)
+    unsigned n = m < 0 ? (unsigned)(-m) : (unsigned)m;
+    double y = (n % 2u) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n % 2u) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+__device__ __forceinline__ bool predicate(const Ptl& P, long long i, int mode, const SplitArgs& sa) {
+    if (mode == 0) return P.count_flag[i] == SP_COUNT_FLAG_INBOX;
+    const double thr = sa.pmin_split * sa.p0 * powi_rl(sa.split_ratio, (int)P.split_times[i]);
+    const double p = P.p[i];
+    return p > thr && p <= sa.pmax;
+}
+
+// In-block exclusive prefix of a per-thread count (kBlock threads, 4 waves).
+__device__ __forceinline__ unsigned block_exclusive(unsigned v, unsigned* lds_wave, unsigned& total) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) lds_wave[wv] = x;
+    __syncthreads();
+    unsigned wbase = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) {
+        const unsigned t = lds_wave[k];
+        if (k < wv) wbase += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return wbase + x - v;
+}
+
+__global__ void __launch_bounds__(kBlock) tile_count_kernel(Ptl P, long long n, int mode,
+                                                            SplitArgs sa, unsigned* tile_cnt,
+                                                            unsigned* tile_esc) {
+    __shared__ unsigned lds[kBlock / kWave];
+    const long long t0 = (long long)blockIdx.x * kTile;
+    unsigned c = 0, e = 0;
+#pragma unroll
+    for (int it = 0; it < kTileItems; ++it) {
+        const long long i = t0 + (long long)it * kBlock + threadIdx.x;
+        if (i < n) {
+            c += predicate(P, i, mode, sa) ? 1u : 0u;
+            if (mode == 0) e += P.count_flag[i] < 0 ? 1u : 0u;
+        }
+    }
+    unsigned tc = 0, te = 0;
+    block_exclusive(c, lds, tc);
+    if (mode == 0) block_exclusive(e, lds, te);
+    if (threadIdx.x == 0) {
+        tile_cnt[blockIdx.x] = tc;
+        if (mode == 0) tile_esc[blockIdx.x] = te;
+    }
+}
+
+// Exclusive scan of ntiles counts by one workgroup; writes offsets and the
+// grand totals (cnt -> totals[0], esc -> totals[1]).
+__global__ void __launch_bounds__(1024) tile_scan_kernel(const unsigned* tile_cnt,
+                                                         const unsigned* tile_esc, long long ntiles,
+                                                         unsigned long long* tile_off,
+                                                         unsigned long long* totals) {
+    __shared__ unsigned long long wsum[1024 / kWave];
+    __shared__ unsigned long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    unsigned long long esc = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (long long b = 0; b < ntiles; b += 1024) {
+        const long long i = b + threadIdx.x;
+        const unsigned long long v = i < ntiles ? tile_cnt[i] : 0ull;
+        if (tile_esc && i < ntiles) esc += tile_esc[i];
+        unsigned long long x = v;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned long long y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == kWave - 1) wsum[wv] = x;
+        __syncthreads();
+        unsigned long long wbase = 0, tot = 0;
+        for (int k = 0; k < 1024 / kWave; ++k) {
+            if (k < wv) wbase += wsum[k];
+            tot += wsum[k];
+        }
+        if (i < ntiles) tile_off[i] = carry + wbase + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    // escaped total: block reduction
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) esc += __shfl_xor(esc, off);
+    if (lane == 0) wsum[wv] = esc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long e = 0;
+        for (int k = 0; k < 1024 / kWave; ++k) e += wsum[k];
+        totals[0] = carry;
+        totals[1] = e;
+    }
+}
+
+__device__ __forceinline__ void copy_particle(const Ptl& S, long long i, const Ptl& D, long long o) {
+    D.x[o] = S.x[i];
+    D.y[o] = S.y[i];
+    D.z[o] = S.z[i];
+    D.p[o] = S.p[i];
+    D.v[o] = S.v[i];
+    D.mu[o] = S.mu[i];
+    D.w[o] = S.w[i];
+    D.t[o] = S.t[i];
+    D.dt[o] = S.dt[i];
+    D.split_times[o] = S.split_times[i];
+    D.count_flag[o] = S.count_flag[i];
+    D.origin[o] = S.origin[i];
+    D.nsteps_tracked[o] = S.nsteps_tracked[i];
+    D.nsteps_pushed[o] = S.nsteps_pushed[i];
+    D.tag_injected[o] = S.tag_injected[i];
+    D.tag_splitted[o] = S.tag_splitted[i];
+    D.rng_step[o] = S.rng_step[i];
+}
+
+__global__ void __launch_bounds__(kBlock) compact_scatter_kernel(Ptl S, Ptl D, long long n,
+                                                                 const unsigned long long* tile_off) {
+    __shared__ unsigned lds[kBlock / kWave];
+    const long long t0 = (long long)blockIdx.x * kTile;
+    unsigned long long base = tile_off[blockIdx.x];
+    const SplitArgs none{};
+#pragma unroll 1
+    for (int it = 0; it < kTileItems; ++it) {
+        const long long i = t0 + (long long)it * kBlock + threadIdx.x;
+        const bool keep = i < n && predicate(S, i, 0, none);
+        unsigned tot = 0;
+        const unsigned ex = block_exclusive(keep ? 1u : 0u, lds, tot);
+        if (keep) copy_particle(S, i, D, (long long)(base + ex));
+        base += tot;
+    }
+}
+
+// split_particle (particle_module.f90:5430-5480): a candidate with global
+// exclusive rank r is split iff n + r < nmax (the reference returns at the
+// first candidate that would overflow nptl_max); its copy goes to n + r.
+__global__ void __launch_bounds__(kBlock) split_apply_kernel(Ptl P, long long n, long long nmax,
+                                                             SplitArgs sa,
+                                                             const unsigned long long* tile_off) {
+    __shared__ unsigned lds[kBlock / kWave];
+    const long long t0 = (long long)blockIdx.x * kTile;
+    unsigned long long base = tile_off[blockIdx.x];
+#pragma unroll 1
+    for (int it = 0; it < kTileItems; ++it) {
+        const long long i = t0 + (long long)it * kBlock + threadIdx.x;
+        const bool cand = i < n && predicate(P, i, 1, sa);
+        unsigned tot = 0;
+        const unsigned ex = block_exclusive(cand ? 1u : 0u, lds, tot);
+        if (cand) {
+            const long long o = n + (long long)(base + ex);
+            if (o < nmax) {
+                const int k = (int)P.split_times[i];
+                const double wnew = ldexp(1.0, -(1 + k));  // 0.5**(1.0 + split_times)
+                P.w[i] = wnew;
+                P.split_times[i] = (int8_t)(k + 1);
+                copy_particle(P, i, P, o);
+                P.tag_splitted[o] = P.tag_splitted[i] + (int32_t)(1u << k);
+            }
+        }
+        base += tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Distributions (diagnostics.f90:738-906).  Weights are accumulated as
+// unsigned 2^-32 fixed point, so every sum is exact and independent of the
+// order of the atomics.
+__device__ __forceinline__ long long floor_ll(double v) {
+    const double f = floor(v);
+    if (!(f > -9.0e18 && f < 9.0e18)) return LLONG_MIN;
+    return (long long)f;
+}
+
+__global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, DistArgs da,
+                                                      unsigned long long* gglob,
+                                                      unsigned long long* gloc0,
+                                                      unsigned long long* gloc1,
+                                                      unsigned long long* gloc2,
+                                                      unsigned long long* gloc3) {
+    extern __shared__ unsigned long long sh[];
+    const int nglob = da.npp * da.nmu;
+    for (int b = threadIdx.x; b < nglob; b += blockDim.x) sh[b] = 0ull;
+    __syncthreads();
+    unsigned long long* gl[4] = {gloc0, gloc1, gloc2, gloc3};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double p = P.p[i], mu = P.mu[i];
+        const unsigned long long wfx = weight_fx(P.w[i]);
+        const double lp = log10(p);
+        if (p > da.pmin && p <= da.pmax && mu >= -1.0 && mu <= 1.0) {
+            const long long ip = floor_ll((lp - da.pmin_log) / da.dp_log) + 1;
+            const long long imu = floor_ll((mu + 1.0) / da.dmu) + 1;
+            if (ip >= 1 && ip <= da.npp && imu >= 1 && imu <= da.nmu)
+                atomicAdd(&sh[(imu - 1) + (long long)da.nmu * (ip - 1)], wfx);
+        }
+        if (!da.local_dist) continue;
+        const double x = P.x[i], y = P.y[i], z = P.z[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const LocalArgs& L = da.loc[k];
+            if (!L.enabled) continue;
+            const long long ix = floor_ll((x - da.xmin) / L.dd[0]) + 1;
+            const long long iy = floor_ll((y - da.ymin) / L.dd[1]) + 1;
+            const long long iz = floor_ll((z - da.zmin) / L.dd[2]) + 1;
+            const long long ip = floor_ll((lp - L.pmin_log) / L.dp_log) + 1;
+            const long long imu = floor_ll((mu + 1.0) / L.dmu) + 1;
+            const bool ok = ix >= 1 && ix <= L.nr[0] && iy >= 1 && iy <= L.nr[1] && iz >= 1 &&
+                            iz <= L.nr[2] && ip > 0 && ip < L.npbins && imu >= 1 && imu <= L.nmu;
+            if (ok) {
+                const long long idx =
+                    (imu - 1) + (long long)L.nmu * ((ip - 1) + (long long)L.npbins *
+                                 ((ix - 1) + L.nr[0] * ((iy - 1) + L.nr[1] * (iz - 1))));
+                atomicAdd(&gl[k][idx], wfx);
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nglob; b += blockDim.x)
+        if (sh[b]) atomicAdd(&gglob[b], sh[b]);
+}
+
+// quick_check / get_pmax_global partials, one row per block:
+// {sum w (fixed point, as double bits), sum dt, min dt, max dt, max p}
+__global__ void __launch_bounds__(kBlock) stats_kernel(Ptl P, long long n, double* part) {
+    __shared__ double s_sum[kBlock / kWave], s_min[kBlock / kWave], s_max[kBlock / kWave],
+        s_pmx[kBlock / kWave];
+    __shared__ unsigned long long s_w[kBlock / kWave];
+    unsigned long long wsum = 0;
+    double dsum = 0.0, dmin = 1.0e300, dmax = -1.0e300, pmx = -1.0e300;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        wsum += weight_fx(P.w[i]);
+        const double dt = P.dt[i];
+        dsum += dt;
+        dmin = fmin(dmin, dt);
+        dmax = fmax(dmax, dt);
+        pmx = fmax(pmx, P.p[i]);
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        wsum += __shfl_xor(wsum, off);
+        dsum += __shfl_xor(dsum, off);
+        dmin = fmin(dmin, __shfl_xor(dmin, off));
+        dmax = fmax(dmax, __shfl_xor(dmax, off));
+        pmx = fmax(pmx, __shfl_xor(pmx, off));
+    }
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_w[wv] = wsum; s_sum[wv] = dsum; s_min[wv] = dmin; s_max[wv] = dmax; s_pmx[wv] = pmx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long W = 0;
+        double S = 0.0, MN = 1.0e300, MX = -1.0e300, PM = -1.0e300;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            W += s_w[k]; S += s_sum[k];
+            MN = fmin(MN, s_min[k]); MX = fmax(MX, s_max[k]); PM = fmax(PM, s_pmx[k]);
+        }
+        double* row = part + 5 * (long long)blockIdx.x;
+        row[0] = __longlong_as_double((long long)W);
+        row[1] = S; row[2] = MN; row[3] = MX; row[4] = PM;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, long long nptl,
+                         long long nmax, int tag0, const InjectArgs& ia, hipStream_t s) {
+    if (nptl <= 0) return hipSuccess;
+    const long long grid = (nptl + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(inject_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s, prm, P, start, nptl,
+                       nmax, tag0, ia);
+    return hipGetLastError();
+}
+
+long long num_tiles(long long n) { return (n + kTile - 1) / kTile; }
+
+hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,
+                             unsigned* tile_cnt, unsigned* tile_esc, hipStream_t s) {
+    const long long nt = num_tiles(n);
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_count_kernel, dim3((unsigned)nt), dim3(kBlock), 0, s, P, n, mode, sa,
+                       tile_cnt, tile_esc);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_scan(const unsigned* tile_cnt, const unsigned* tile_esc, long long ntiles,
+                            unsigned long long* tile_off, unsigned long long* totals, hipStream_t s) {
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, s, tile_cnt, tile_esc, ntiles,
+                       tile_off, totals);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_scatter(const Ptl& S, const Ptl& D, long long n,
+                                  const unsigned long long* tile_off, hipStream_t s) {
+    const long long nt = num_tiles(n);
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_scatter_kernel, dim3((unsigned)nt), dim3(kBlock), 0, s, S, D, n,
+                       tile_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_apply(const Ptl& P, long long n, long long nmax, const SplitArgs& sa,
+                              const unsigned long long* tile_off, hipStream_t s) {
+    const long long nt = num_tiles(n);
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(split_apply_kernel, dim3((unsigned)nt), dim3(kBlock), 0, s, P, n, nmax, sa,
+                       tile_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist(const Ptl& P, long long n, const DistArgs& da, unsigned long long* gglob,
+                       unsigned long long* const gloc[4], int grid, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const size_t shm = sizeof(unsigned long long) * (size_t)da.npp * da.nmu;
+    hipLaunchKernelGGL(dist_kernel, dim3(grid), dim3(kBlock), shm, s, P, n, da, gglob, gloc[0],
+                       gloc[1], gloc[2], gloc[3]);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats(const Ptl& P, long long n, double* part, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(stats_kernel, dim3(grid), dim3(kBlock), 0, s, P, n, part);
+    return hipGetLastError();
+}
+
+}  // namespace sp

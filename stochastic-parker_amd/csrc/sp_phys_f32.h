@@ -1,0 +1,289 @@
+// sp_phys_f32.h -- FP32 gather, diffusion tensor and Parker step.
+//
+// Same physics as sp_phys_f64.h (particle_module.f90:2208-2450, :3358-3606,
+// :4625-4907) with the gather, the coefficient arithmetic and the increments
+// in FP32; position, momentum and elapsed time stay FP64 accumulators.
+// Arithmetic is regrouped for the hardware: reciprocals and square roots use
+// the single-instruction v_rcp_f32 / v_rsq_f32 / v_sqrt_f32, the momentum
+// power is exp2(pindex * log2(p/p0)), and contraction into FMA is allowed.
+#pragma once
+
+#include "sp_device.h"
+
+namespace sp {
+namespace f32 {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+struct Interp {
+    int ix, iy, iz;
+    float w[8];
+};
+
+__device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, double y,
+                                                double z) {
+    Interp I;
+    const double px = (x - prm.xmin) * prm.idx;
+    const double py = (y - prm.ymin) * prm.idy;
+    const double fx = floor(px), fy = floor(py);
+    const float rx = (float)(px - fx), ry = (float)(py - fy);
+    I.ix = clamp_cell(fx, prm.nxg - 4) + 2;
+    I.iy = clamp_cell(fy, prm.nyg - 4) + 2;
+    I.iz = 0;
+    float rz = 0.0f;
+    if (prm.ndim == 3) {
+        const double pz = (z - prm.zmin) * prm.idz;
+        const double fz = floor(pz);
+        rz = (float)(pz - fz);
+        I.iz = clamp_cell(fz, prm.nzg - 4) + 2;
+    }
+    const float rx1 = 1.0f - rx, ry1 = 1.0f - ry, rz1 = 1.0f - rz;
+    I.w[0] = rx1 * ry1 * rz1;
+    I.w[1] = rx * ry1 * rz1;
+    I.w[2] = rx1 * ry * rz1;
+    I.w[3] = rx * ry * rz1;
+    I.w[4] = rx1 * ry1 * rz;
+    I.w[5] = rx * ry1 * rz;
+    I.w[6] = rx1 * ry * rz;
+    I.w[7] = rx * ry * rz;
+    return I;
+}
+
+template <int NDIM, bool TINTERP>
+__device__ __forceinline__ void gather(const DevParams& prm, const float* __restrict__ rec1,
+                                       const float* __restrict__ rec2, const Interp& I, float rt,
+                                       float* __restrict__ f) {
+    constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
+    constexpr int NC = NDIM == 2 ? 15 : 21;
+    constexpr int NCORN = NDIM == 2 ? 4 : 8;
+    const long long sx = 1, sy = prm.nxg, sz = (long long)prm.nxg * prm.nyg;
+    const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
+    float g[TINTERP ? NC : 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        f[c] = 0.0f;
+        if (TINTERP) g[c] = 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < NCORN; ++q) {
+        const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
+        const long long node = base + i * sx + j * sy + k * sz;
+        const float4* r4 = reinterpret_cast<const float4*>(rec1 + node * W);
+        float v[W];
+#pragma unroll
+        for (int u = 0; u < W / 4; ++u) {
+            const float4 t = r4[u];
+            v[4 * u] = t.x; v[4 * u + 1] = t.y; v[4 * u + 2] = t.z; v[4 * u + 3] = t.w;
+        }
+        const float wq = I.w[q];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c] += v[c] * wq;
+        if (TINTERP) {
+            const float4* s4 = reinterpret_cast<const float4*>(rec2 + node * W);
+#pragma unroll
+            for (int u = 0; u < W / 4; ++u) {
+                const float4 t = s4[u];
+                v[4 * u] = t.x; v[4 * u + 1] = t.y; v[4 * u + 2] = t.z; v[4 * u + 3] = t.w;
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g[c] += v[c] * wq;
+        }
+    }
+    if (TINTERP) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c] = f[c] * (1.0f - rt) + g[c] * rt;
+    }
+}
+
+
+// One Parker step in FP32 coefficients.  Semantics match f64::push: same dt
+// rule (including the dt_min fallback when a rate is exactly zero), same
+// increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.
+template <int NDIM>
+__device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, float dt_max,
+                                           State& s, const float* f, bool fixed_dt,
+                                           const float r[4], double d[4]) {
+    const float bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
+    const float by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
+    const float bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
+    const float b2 = bx * bx + by * by + bz * bz;
+    const float b = fsqrt(b2);
+    const bool bzero = b < (float)__DBL_EPSILON__;
+    const float ibk = bzero ? 1.0f : frcp(b);   // kappa's 1/B (1 when B = 0)
+    const float ib = bzero ? 0.0f : ibk;         // pusher's 1/B (0 when B = 0)
+    const float ib2 = ibk * ibk, ib3 = ibk * ib2;
+    const float pf = (float)s.p;
+    const float pn = (float)(s.p / prm.p0);
+
+    // --- kappa ---------------------------------------------------------
+    float knorm = 1.0f;
+    if (prm.mag_dependency == 1) knorm = __powf(b, (float)(prm.gamma_turb - 2.0));
+    if (prm.momentum_dependency == 1) knorm *= __powf(pn, (float)prm.pindex);
+    const float kpara = (float)prm.kpara0 * knorm;
+    const float kperp = kpara * (float)prm.kret;
+    const float skpara = fsqrt(2.0f * kpara);
+    const float skperp = fsqrt(2.0f * kperp);
+    const float skpp = fsqrt(2.0f * (kpara - kperp));
+    const float kpp = kpara - kperp;
+    const float gm2 = (float)(prm.gamma_turb - 2.0);
+
+    float dkxx_dx, dkyy_dy, dkxy_dx, dkxy_dy;
+    float dkzz_dz = 0.f, dkxz_dx = 0.f, dkxz_dz = 0.f, dkyz_dy = 0.f, dkyz_dz = 0.f;
+    float dx_dt, dy_dt, dz_dt, divv;
+    // drift prefactor vdp (FP32 quotient as in the Fortran)
+    const float pre = 1.0f / (float)(3 * prm.pcharge);
+    const float ip = frcp(pf);
+    const float t1 = (float)(prm.drift1 * prm.p0) * ip;
+    const float t2 = (float)(prm.drift2 * prm.p0 * prm.p0) * ip * ip;
+    const float vdp = pre * frsq(t1 * t1 + t2 * t2);
+    float bxn = 0.f, byn = 0.f, bzn = 0.f, bxyn = 0.f, ibxyn = 0.f;
+    if (NDIM == 2) {
+        const float dbx_dx = f[R2_DBXDX], dbx_dy = f[R2_DBXDY];
+        const float dby_dx = f[R2_DBYDX], dby_dy = f[R2_DBYDY];
+        const float dbz_dx = f[R2_DBZDX], dbz_dy = f[R2_DBZDY];
+        const float db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
+        float dkdx = 0.f, dkdy = 0.f;
+        if (prm.mag_dependency == 1) { dkdx = db_dx * ibk * gm2; dkdy = db_dy * ibk * gm2; }
+        const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
+        dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+        dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+        dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+        dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        const float ibp2 = ib * ib, ibp3 = ib * ibp2;
+        const float vdx = vdp * (dbz_dy * ibp2 - 2.0f * bz * db_dy * ibp3);
+        const float vdy = vdp * (-(dbz_dx * ibp2) + 2.0f * bz * db_dx * ibp3);
+        float vdz = 0.f;
+        if (prm.check_drift_2d)
+            vdz = vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3);
+        dx_dt = f[R2_VX] + vdx + dkxx_dx + dkxy_dy;
+        dy_dt = f[R2_VY] + vdy + dkxy_dx + dkyy_dy;
+        dz_dt = vdz;
+        divv = f[R2_DVXDX] + f[R2_DVYDY];
+    } else {
+        const float dbx_dx = f[R3_DBXDX], dbx_dy = f[R3_DBXDY], dbx_dz = f[R3_DBXDZ];
+        const float dby_dx = f[R3_DBYDX], dby_dy = f[R3_DBYDY], dby_dz = f[R3_DBYDZ];
+        const float dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
+        const float db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
+        float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
+        if (prm.mag_dependency == 1) { dkdx = db_dx * gm2; dkdy = db_dy * gm2; dkdz = db_dz * gm2; }
+        const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
+        dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+        dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+        dkzz_dz = kperp * dkdz + kb2 * dkdz * bz * bz + kb3 * bz * (dbz_dz * b - bz * db_dz);
+        dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+        dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        dkxz_dx = kb2 * (dkdx * bx * bz + dbx_dx * bz + bx * dbz_dx) - kb3 * bx * bz * db_dx;
+        dkxz_dz = kb2 * (dkdz * bx * bz + dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
+        dkyz_dy = kb2 * (dkdy * by * bz + dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
+        dkyz_dz = kb2 * (dkdz * by * bz + dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
+        bxn = bx * ib; byn = by * ib; bzn = bz * ib;
+        const float bxy2 = bxn * bxn + byn * byn;
+        bxyn = fsqrt(bxy2);
+        ibxyn = bxyn < (float)__DBL_EPSILON__ ? 0.0f : frcp(bxyn);
+        const float ibp2 = ib * ib, ibp3 = ib * ibp2;
+        const float vdx = vdp * ((dbz_dy - dby_dz) * ibp2 - 2.0f * (bz * db_dy - by * db_dz) * ibp3);
+        const float vdy = vdp * ((dbx_dz - dbz_dx) * ibp2 - 2.0f * (bx * db_dz - bz * db_dx) * ibp3);
+        const float vdz = vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3);
+        dx_dt = f[R3_VX] + vdx + dkxx_dx + dkxy_dy + dkxz_dz;
+        dy_dt = f[R3_VY] + vdy + dkxy_dx + dkyy_dy + dkyz_dz;
+        dz_dt = f[R3_VZ] + vdz + dkxz_dx + dkyz_dy + dkzz_dz;
+        divv = f[R3_DVXDX] + f[R3_DVYDY] + f[R3_DVZDZ];
+    }
+    const float dp_dt = -pf * divv * (1.0f / 3.0f);
+
+    float dtf32;
+    if (!fixed_dt) {
+        float m;
+        if (dx_dt != 0.0f && dy_dt != 0.0f && dp_dt != 0.0f) {
+            const float sk = skperp > 0.0f ? skperp : skpara;
+            const float isk = frcp(skpara);
+            float a = 0.5f * (float)prm.dx * isk;
+            m = a * a;
+            a = 0.5f * (float)prm.dy * isk; m = fminf(m, a * a);
+            if (NDIM == 3) { a = 0.5f * (float)prm.dz * isk; m = fminf(m, a * a); }
+            a = sk * frcp(dx_dt); m = fminf(m, a * a);
+            a = sk * frcp(dy_dt); m = fminf(m, a * a);
+            if (NDIM == 3) { a = sk * frcp(dz_dt); m = fminf(m, a * a); }
+            m = fminf(m, 0.1f * pf * frcp(fabsf(dp_dt)));
+        } else {
+            m = dt_min;
+        }
+        m = fminf(fmaxf(m, dt_min), dt_max);
+        dtf32 = m;
+        s.dt = (double)m;
+    } else {
+        dtf32 = (float)s.dt;
+    }
+    const float sdt = fsqrt(dtf32);
+    float deltax, deltay, deltaz;
+    if (NDIM == 2) {
+        const float a = r[2] * skpp * ib;
+        deltax = dx_dt * dtf32 + (r[0] * skperp + a * bx) * sdt;
+        deltay = dy_dt * dtf32 + (r[1] * skperp + a * by) * sdt;
+        deltaz = dz_dt * dtf32;
+    } else {
+        const float sp = skperp * ibxyn;
+        deltax = dx_dt * dtf32 + (bxn * skpara * r[0] - bxn * bzn * sp * r[1] - byn * sp * r[2]) * sdt;
+        deltay = dy_dt * dtf32 + (byn * skpara * r[0] - byn * bzn * sp * r[1] + bxn * sp * r[2]) * sdt;
+        deltaz = dz_dt * dtf32 + (bzn * skpara * r[0] + bxyn * skperp * r[1]) * sdt;
+    }
+    s.x += (double)deltax;
+    s.y += (double)deltay;
+    s.z += (double)deltaz;
+    s.t += s.dt;
+    double dp = (double)(dp_dt * dtf32);
+    if (prm.acc_region_flag == 1) {
+        const double xn = (s.x - prm.xmin) / prm.lx;
+        bool in = (xn >= prm.acc_region[0]) && (xn <= prm.acc_region[1]);
+        if (prm.ndim > 1) {
+            const double yn = (s.y - prm.ymin) / prm.ly;
+            in = in && (yn >= prm.acc_region[2]) && (yn <= prm.acc_region[3]);
+        }
+        if (prm.ndim == 3) {
+            const double zn = (s.z - prm.zmin) / prm.lz;
+            in = in && (zn >= prm.acc_region[4]) && (zn <= prm.acc_region[5]);
+        }
+        if (in) s.p += dp; else dp = 0.0;
+    } else {
+        s.p += dp;
+    }
+    if (s.p < 0.25 * prm.p0) {
+        s.p -= dp;
+        dp = 0.25 * prm.p0 - s.p;
+        s.p = 0.25 * prm.p0;
+    }
+    d[0] = (double)deltax;
+    d[1] = (double)deltay;
+    d[2] = NDIM == 2 ? 0.0 : (double)deltaz;
+    d[3] = dp;
+}
+
+// (2u - 1) sqrt(3) in FP32 from one Philox block: u = r / (2^32 - 1).
+__device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i, uint32_t tag_s,
+                                           uint32_t step, float r[4]) {
+    const U4 u = philox4x32_10(tag_i, tag_s, step, prm.stream, prm.key0, prm.key1);
+    const float sc = (float)(2.0 * 1.7320508075688772 / 4294967295.0);
+    const float of = -1.7320508075688772f;
+    r[0] = (float)u.a * sc + of;
+    r[1] = (float)u.b * sc + of;
+    r[2] = (float)u.c * sc + of;
+    r[3] = (float)u.d * sc + of;
+}
+
+template <int NDIM, bool TINTERP>
+__device__ __forceinline__ void gather_kappa_push(const DevParams& prm, const float* rec1,
+                                                  const float* rec2, double t0, double dtf,
+                                                  float dt_min, float dt_max, State& s,
+                                                  bool fixed_dt, const float r[4], double d[4]) {
+    constexpr int NC = NDIM == 2 ? 15 : 21;
+    float f[NC];
+    const Interp I = interp_params(prm, s.x, s.y, s.z);
+    const float rt = TINTERP ? (float)((s.t - t0) / dtf) : 0.0f;
+    gather<NDIM, TINTERP>(prm, rec1, rec2, I, rt, f);
+    kappa_push<NDIM>(prm, dt_min, dt_max, s, f, fixed_dt, r, d);
+}
+
+}  // namespace f32
+}  // namespace sp

@@ -1,0 +1,351 @@
+// sp_phys_f64.h -- FP64 gather, diffusion tensor and Parker step in the
+// reference's operation order.  Included only by translation units built
+// with -ffp-contract=off, so every +, -, *, / is one IEEE-rounded operation,
+// as in the reference Fortran evaluated without contraction.
+#pragma once
+
+#include "sp_device.h"
+
+namespace sp {
+namespace f64 {
+
+// get_interp_paramters (particle_module.f90:642-674) on a uniform grid.
+// Returns the 0-based node index of the lower corner and the weights.
+struct Interp {
+    int ix, iy, iz;
+    double w[8];
+};
+
+__device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, double y,
+                                                double z) {
+    Interp I;
+    const double px = (x - prm.xmin) / prm.dx;
+    const double py = (y - prm.ymin) / prm.dy;
+    const double fx = floor(px), fy = floor(py);
+    const double posx = fx + 1.0, posy = fy + 1.0;   // Fortran pos (exact integers)
+    const double rx = (px - posx) + 1.0;
+    const double ry = (py - posy) + 1.0;
+    double rz = 0.0;
+    I.ix = clamp_cell(fx, prm.nxg - 4) + 2;
+    I.iy = clamp_cell(fy, prm.nyg - 4) + 2;
+    I.iz = 0;
+    if (prm.ndim == 3) {
+        const double pz = (z - prm.zmin) / prm.dz;
+        const double fz = floor(pz);
+        rz = (pz - (fz + 1.0)) + 1.0;
+        I.iz = clamp_cell(fz, prm.nzg - 4) + 2;
+    }
+    const double rx1 = 1.0 - rx, ry1 = 1.0 - ry, rz1 = 1.0 - rz;
+    I.w[0] = rx1 * ry1 * rz1;
+    I.w[1] = rx * ry1 * rz1;
+    I.w[2] = rx1 * ry * rz1;
+    I.w[3] = rx * ry * rz1;
+    I.w[4] = rx1 * ry1 * rz;
+    I.w[5] = rx * ry1 * rz;
+    I.w[6] = rx1 * ry * rz;
+    I.w[7] = rx * ry * rz;
+    return I;
+}
+
+// interp_fields (mhd_data_parallel.f90:1751-1793) over node records:
+// fields = 0; for corners (k, j, i; i fastest) fields += f_node * w; then the
+// optional (1-rt, rt) blend of the two frames.
+template <int NDIM, bool TINTERP>
+__device__ __forceinline__ void gather(const DevParams& prm, const float* __restrict__ rec1,
+                                       const float* __restrict__ rec2, const Interp& I, double rt,
+                                       double* __restrict__ f) {
+    constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
+    constexpr int NC = NDIM == 2 ? 15 : 21;
+    constexpr int NCORN = NDIM == 2 ? 4 : 8;
+    double g[TINTERP ? NC : 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        f[c] = 0.0;
+        if (TINTERP) g[c] = 0.0;
+    }
+    const long long sx = 1, sy = prm.nxg, sz = (long long)prm.nxg * prm.nyg;
+    const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
+#pragma unroll
+    for (int q = 0; q < NCORN; ++q) {
+        const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
+        const long long node = base + i * sx + j * sy + k * sz;
+        const float4* r4 = reinterpret_cast<const float4*>(rec1 + node * W);
+        float v[W];
+#pragma unroll
+        for (int u = 0; u < W / 4; ++u) {
+            const float4 t = r4[u];
+            v[4 * u] = t.x; v[4 * u + 1] = t.y; v[4 * u + 2] = t.z; v[4 * u + 3] = t.w;
+        }
+        const double wq = I.w[q];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c] = f[c] + (double)v[c] * wq;
+        if (TINTERP) {
+            const float4* s4 = reinterpret_cast<const float4*>(rec2 + node * W);
+#pragma unroll
+            for (int u = 0; u < W / 4; ++u) {
+                const float4 t = s4[u];
+                v[4 * u] = t.x; v[4 * u + 1] = t.y; v[4 * u + 2] = t.z; v[4 * u + 3] = t.w;
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) g[c] = g[c] + (double)v[c] * wq;
+        }
+    }
+    if (TINTERP) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c] = f[c] * (1.0 - rt) + g[c] * rt;
+    }
+}
+
+struct Kappa {
+    double kpara, kperp, skpara, skperp, skpara_perp;
+    double kxx, kyy, kzz, kxy, kxz, kyz;
+    double dkxx_dx, dkyy_dy, dkzz_dz, dkxy_dx, dkxy_dy, dkxz_dx, dkxz_dz, dkyz_dy, dkyz_dz;
+};
+
+// calc_spatial_diffusion_coefficients (particle_module.f90:2208-2450),
+// Cartesian Parker transport without delta-B / correlation-length maps.
+template <int NDIM>
+__device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const double* f) {
+    Kappa k;
+    const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
+    const double by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
+    const double bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
+    const double b = sqrt(bx * bx + by * by + bz * bz);
+    double ib1;
+    if (b < __DBL_EPSILON__) ib1 = 1.0; else ib1 = 1.0 / b;
+    const double ib2 = ib1 * ib1;
+    const double ib3 = ib1 * ib2;
+    double knorm_para = 1.0;
+    if (prm.mag_dependency == 1) knorm_para = knorm_para * pow(b, prm.gamma_turb - 2.0);
+    double knorm;
+    if (prm.momentum_dependency == 1) knorm = knorm_para * pow(p / prm.p0, prm.pindex);
+    else knorm = knorm_para;
+    k.kpara = prm.kpara0 * knorm;
+    k.kperp = k.kpara * prm.kret;
+    k.skpara = sqrt(2.0 * k.kpara);
+    k.skperp = sqrt(2.0 * k.kperp);
+    k.skpara_perp = sqrt(2.0 * (k.kpara - k.kperp));
+    const double kpp = k.kpara - k.kperp;
+    const double kperp = k.kperp;
+    if (NDIM == 2) {
+        const double dbx_dx = f[R2_DBXDX], dbx_dy = f[R2_DBXDY];
+        const double dby_dx = f[R2_DBYDX], dby_dy = f[R2_DBYDY];
+        const double db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
+        double dkdx = 0.0, dkdy = 0.0;
+        if (prm.mag_dependency == 1) {
+            dkdx = db_dx * ib1 * (prm.gamma_turb - 2.0);
+            dkdy = db_dy * ib1 * (prm.gamma_turb - 2.0);
+        }
+        k.dkxx_dx = kperp * dkdx + kpp * dkdx * (bx * bx) * ib2 +
+                    2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+        k.dkyy_dy = kperp * dkdy + kpp * dkdy * (by * by) * ib2 +
+                    2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+        k.dkxy_dx = kpp * dkdx * bx * by * ib2 +
+                    kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+        k.dkxy_dy = kpp * dkdy * bx * by * ib2 +
+                    kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+        k.kxx = kperp + kpp * bx * bx * ib2;
+        k.kyy = kperp + kpp * by * by * ib2;
+        k.kxy = kpp * bx * by * ib2;
+        k.kzz = k.kxz = k.kyz = 0.0;
+        k.dkzz_dz = k.dkxz_dx = k.dkxz_dz = k.dkyz_dy = k.dkyz_dz = 0.0;
+    } else {
+        const double dbx_dx = f[R3_DBXDX], dbx_dy = f[R3_DBXDY], dbx_dz = f[R3_DBXDZ];
+        const double dby_dx = f[R3_DBYDX], dby_dy = f[R3_DBYDY], dby_dz = f[R3_DBYDZ];
+        const double dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
+        const double db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
+        double dkdx = 0.0, dkdy = 0.0, dkdz = 0.0;
+        if (prm.mag_dependency == 1) {  // no 1/B in the 3D branch (particle_module.f90:2405-2409)
+            dkdx = db_dx * (prm.gamma_turb - 2.0);
+            dkdy = db_dy * (prm.gamma_turb - 2.0);
+            dkdz = db_dz * (prm.gamma_turb - 2.0);
+        }
+        k.dkxx_dx = kperp * dkdx + kpp * dkdx * (bx * bx) * ib2 +
+                    2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+        k.dkyy_dy = kperp * dkdy + kpp * dkdy * (by * by) * ib2 +
+                    2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+        k.dkzz_dz = kperp * dkdz + kpp * dkdz * (bz * bz) * ib2 +
+                    2.0 * kpp * bz * (dbz_dz * b - bz * db_dz) * ib3;
+        k.dkxy_dx = kpp * dkdx * bx * by * ib2 +
+                    kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+        k.dkxy_dy = kpp * dkdy * bx * by * ib2 +
+                    kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+        k.dkxz_dx = kpp * dkdx * bx * bz * ib2 +
+                    kpp * ((dbx_dx * bz + bx * dbz_dx) * ib2 - 2.0 * bx * bz * db_dx * ib3);
+        k.dkxz_dz = kpp * dkdz * bx * bz * ib2 +
+                    kpp * ((dbx_dz * bz + bx * dbz_dz) * ib2 - 2.0 * bx * bz * db_dz * ib3);
+        k.dkyz_dy = kpp * dkdy * by * bz * ib2 +
+                    kpp * ((dby_dy * bz + by * dbz_dy) * ib2 - 2.0 * by * bz * db_dy * ib3);
+        k.dkyz_dz = kpp * dkdz * by * bz * ib2 +
+                    kpp * ((dby_dz * bz + by * dbz_dz) * ib2 - 2.0 * by * bz * db_dz * ib3);
+        k.kxx = kperp + kpp * bx * bx * ib2;
+        k.kyy = kperp + kpp * by * by * ib2;
+        k.kzz = kperp + kpp * bz * bz * ib2;
+        k.kxy = kpp * bx * by * ib2;
+        k.kxz = kpp * bx * bz * ib2;
+        k.kyz = kpp * by * bz * ib2;
+    }
+    return k;
+}
+
+__device__ __forceinline__ bool in_acc_region(const DevParams& prm, double x, double y,
+                                              double z) {
+    const double xn = (x - prm.xmin) / prm.lx;
+    bool in = (xn >= prm.acc_region[0]) && (xn <= prm.acc_region[1]);
+    if (prm.ndim > 1) {
+        const double yn = (y - prm.ymin) / prm.ly;
+        in = in && (yn >= prm.acc_region[2]) && (yn <= prm.acc_region[3]);
+    }
+    if (prm.ndim == 3) {
+        const double zn = (z - prm.zmin) / prm.lz;
+        in = in && (zn >= prm.acc_region[4]) && (zn <= prm.acc_region[5]);
+    }
+    return in;
+}
+
+
+__device__ __forceinline__ double fmin_seq(double a, double b) { return b < a ? b : a; }
+
+// push_particle_2d (particle_module.f90:3358-3606) / push_particle_3d
+// (:4625-4907), Cartesian, uniform grid, dpp terms off.  r = (ran1, ran2,
+// ran3, ran_p).  d receives (deltax, deltay, deltaz, deltap); the 2D pusher
+// leaves deltaz at zero like the reference (its z drift is not undone).
+template <int NDIM>
+__device__ __forceinline__ void push(const DevParams& prm, double dt_min, double dt_max,
+                                     State& s, const double* f, const Kappa& k, bool fixed_dt,
+                                     const double r[4], double d[4]) {
+    const double vx = NDIM == 2 ? f[R2_VX] : f[R3_VX];
+    const double vy = NDIM == 2 ? f[R2_VY] : f[R3_VY];
+    const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
+    const double by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
+    const double bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
+    const double b = sqrt(bx * bx + by * by + bz * bz);
+    double ib;
+    if (b < __DBL_EPSILON__) ib = 0.0; else ib = 1.0 / b;
+    const double ib2 = ib * ib;
+    const double ib3 = ib * ib2;
+    // vdp = 1.0/(3*pcharge) / sqrt(...); the first quotient is FP32 in the Fortran
+    const float pre = 1.0f / (float)(3 * prm.pcharge);
+    const double t1 = prm.drift1 * prm.p0 / s.p;
+    const double t2 = prm.drift2 * (prm.p0 * prm.p0) / (s.p * s.p);
+    const double vdp = (double)pre / sqrt(t1 * t1 + t2 * t2);
+
+    double dx_dt, dy_dt, dz_dt, divv;
+    double bxn = 0.0, byn = 0.0, bzn = 0.0, bxyn = 0.0, ibxyn = 0.0;
+    if (NDIM == 2) {
+        const double dbz_dx = f[R2_DBZDX], dbz_dy = f[R2_DBZDY];
+        const double db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
+        const double vdx = vdp * (dbz_dy * ib2 - 2.0 * bz * db_dy * ib3);
+        const double vdy = vdp * (-(dbz_dx * ib2) + 2.0 * bz * db_dx * ib3);
+        double vdz = 0.0;
+        if (prm.check_drift_2d) {
+            const double dbx_dy = f[R2_DBXDY], dby_dx = f[R2_DBYDX];
+            vdz = vdp * ((dby_dx - dbx_dy) * ib2 - 2.0 * (by * db_dx - bx * db_dy) * ib3);
+        }
+        dx_dt = vx + vdx + k.dkxx_dx + k.dkxy_dy;
+        dy_dt = vy + vdy + k.dkxy_dx + k.dkyy_dy;
+        dz_dt = vdz;
+        divv = f[R2_DVXDX] + f[R2_DVYDY];
+    } else {
+        const double vz = f[R3_VZ];
+        bxn = bx * ib; byn = by * ib; bzn = bz * ib;
+        bxyn = sqrt(bxn * bxn + byn * byn);
+        if (bxyn < __DBL_EPSILON__) ibxyn = 0.0; else ibxyn = 1.0 / bxyn;
+        const double dbx_dy = f[R3_DBXDY], dbx_dz = f[R3_DBXDZ];
+        const double dby_dx = f[R3_DBYDX], dby_dz = f[R3_DBYDZ];
+        const double dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY];
+        const double db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
+        const double vdx = vdp * ((dbz_dy - dby_dz) * ib2 - 2.0 * (bz * db_dy - by * db_dz) * ib3);
+        const double vdy = vdp * ((dbx_dz - dbz_dx) * ib2 - 2.0 * (bx * db_dz - bz * db_dx) * ib3);
+        const double vdz = vdp * ((dby_dx - dbx_dy) * ib2 - 2.0 * (by * db_dx - bx * db_dy) * ib3);
+        dx_dt = vx + vdx + k.dkxx_dx + k.dkxy_dy + k.dkxz_dz;
+        dy_dt = vy + vdy + k.dkxy_dx + k.dkyy_dy + k.dkyz_dz;
+        dz_dt = vz + vdz + k.dkxz_dx + k.dkyz_dy + k.dkzz_dz;
+        divv = f[R3_DVXDX] + f[R3_DVYDY] + f[R3_DVZDZ];
+    }
+    const double dp_dt = -(s.p * divv / 3.0);
+
+    if (!fixed_dt) {
+        if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
+            const double sk = k.skperp > 0.0 ? k.skperp : k.skpara;
+            double a = 0.5 * prm.dx / k.skpara;
+            double m = a * a;
+            a = 0.5 * prm.dy / k.skpara; m = fmin_seq(m, a * a);
+            if (NDIM == 3) { a = 0.5 * prm.dz / k.skpara; m = fmin_seq(m, a * a); }
+            a = sk / dx_dt; m = fmin_seq(m, a * a);
+            a = sk / dy_dt; m = fmin_seq(m, a * a);
+            if (NDIM == 3) { a = sk / dz_dt; m = fmin_seq(m, a * a); }
+            m = fmin_seq(m, (double)0.1f * s.p / fabs(dp_dt));
+            s.dt = m;
+        } else {
+            s.dt = dt_min;
+        }
+        if (s.dt < dt_min) s.dt = dt_min;
+        if (s.dt > dt_max) s.dt = dt_max;
+    }
+
+    const double sdt = sqrt(s.dt);
+    double deltax, deltay, deltaz;
+    if (NDIM == 2) {
+        deltax = dx_dt * s.dt + r[0] * k.skperp * sdt + r[2] * k.skpara_perp * sdt * bx * ib;
+        deltay = dy_dt * s.dt + r[1] * k.skperp * sdt + r[2] * k.skpara_perp * sdt * by * ib;
+        deltaz = dz_dt * s.dt;
+    } else {
+        deltax = dx_dt * s.dt + (bxn * k.skpara * r[0] - bxn * bzn * k.skperp * ibxyn * r[1] -
+                                 byn * k.skperp * ibxyn * r[2]) * sdt;
+        deltay = dy_dt * s.dt + (byn * k.skpara * r[0] - byn * bzn * k.skperp * ibxyn * r[1] +
+                                 bxn * k.skperp * ibxyn * r[2]) * sdt;
+        deltaz = dz_dt * s.dt + (bzn * k.skpara * r[0] + bxyn * k.skperp * r[1]) * sdt;
+    }
+    s.x = s.x + deltax;
+    s.y = s.y + deltay;
+    s.z = s.z + deltaz;
+    s.t = s.t + s.dt;
+    const double dpp = 0.0;
+    double dp = dp_dt * s.dt + r[3] * sqrt(2.0 * dpp) * sdt;
+    if (prm.acc_region_flag == 1) {
+        if (in_acc_region(prm, s.x, s.y, s.z)) s.p = s.p + dp;
+        else dp = 0.0;
+    } else {
+        s.p = s.p + dp;
+    }
+    if (s.p < 0.25 * prm.p0) {
+        s.p = s.p - dp;
+        dp = 0.25 * prm.p0 - s.p;
+        s.p = 0.25 * prm.p0;
+    }
+    d[0] = deltax;
+    d[1] = deltay;
+    d[2] = NDIM == 2 ? 0.0 : deltaz;
+    d[3] = dp;
+}
+
+// Steps' increments from one Philox block: (2u - 1) * sqrt(3) for each
+// of the four words (particle_module.f90:3548-3550, :3589).
+__device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i, uint32_t tag_s,
+                                           uint32_t step, double r[4]) {
+    const U4 u = philox4x32_10(tag_i, tag_s, step, prm.stream, prm.key0, prm.key1);
+    const double sqrt3 = sqrt(3.0);
+    r[0] = (2.0 * u01(u.a) - 1.0) * sqrt3;
+    r[1] = (2.0 * u01(u.b) - 1.0) * sqrt3;
+    r[2] = (2.0 * u01(u.c) - 1.0) * sqrt3;
+    r[3] = (2.0 * u01(u.d) - 1.0) * sqrt3;
+}
+
+// The full per-step chain: interpolation parameters, gather, kappa, push.
+template <int NDIM, bool TINTERP>
+__device__ __forceinline__ void gather_kappa_push(const DevParams& prm, const float* rec1,
+                                                  const float* rec2, double t0, double dtf,
+                                                  double dt_min, double dt_max, State& s,
+                                                  bool fixed_dt, const double r[4], double d[4]) {
+    constexpr int NC = NDIM == 2 ? 15 : 21;
+    double f[NC];
+    const Interp I = interp_params(prm, s.x, s.y, s.z);
+    const double rt = (s.t - t0) / dtf;
+    gather<NDIM, TINTERP>(prm, rec1, rec2, I, rt, f);
+    const Kappa k = kappa<NDIM>(prm, s.p, f);
+    push<NDIM>(prm, dt_min, dt_max, s, f, k, fixed_dt, r, d);
+}
+
+}  // namespace f64
+}  // namespace sp

@@ -1,0 +1,222 @@
+"""Host-side mirror of the reference's pusher modules over libsp_hip.so.
+
+``ParkerPusher`` exposes the procedures the reference driver calls
+(stochastic-mhd.f90:397-566) with the reference's names and argument meaning:
+
+=====================================  ==============================================
+reference procedure                    method
+=====================================  ==============================================
+read_field_data_parallel               upload_fields
+calc_fields_gradients                  calc_fields_gradients
+copy_fields                            copy_fields
+inject_particles_spatial_uniform       inject_particles_spatial_uniform
+particle_mover                         particle_mover
+split_particle                         split_particle
+init_particle_distributions            init_particle_distributions
+calc_particle_distributions            calc_particle_distributions
+quick_check / get_pmax_global          quick_check
+=====================================  ==============================================
+
+Failures raise ``SpError`` with the library's message; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import PARTICLE_FIELDS, SpConfig, SpDistConfig, SpMoverStats, SpParticles, check, dptr
+
+CONFIG_DEFAULTS = dict(
+    ndim=2, nx=64, ny=64, nz=1, dx=1.0 / 64, dy=1.0 / 64, dz=1.0,
+    xmin=0.0, ymin=0.0, zmin=0.0, xmax=1.0, ymax=1.0, zmax=1.0, lx=1.0, ly=1.0, lz=1.0,
+    pbcx=0, pbcy=0, pbcz=0, momentum_dependency=1, mag_dependency=0, charge=-1,
+    p0=1.0, pmin=1e-2, pmax=1e4, b0=1.0, gamma_turb=5.0 / 3.0, kpara0=1e-3, kret=0.01,
+    dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
+    acc_region_flag=0, acc_region=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), time_interp=0,
+    precision=_abi.SP_PREC_F64, seed=20261202, stream=0, nptl_max=1000,
+)
+
+
+def make_config(**kw) -> SpConfig:
+    d = dict(CONFIG_DEFAULTS)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise KeyError(f"unknown config keys: {sorted(unknown)}")
+    d.update(kw)
+    if isinstance(d["precision"], str):
+        d["precision"] = {"f64": _abi.SP_PREC_F64, "f32": _abi.SP_PREC_F32}[d["precision"]]
+    c = SpConfig()
+    for k, v in d.items():
+        if k == "acc_region":
+            for i in range(6):
+                c.acc_region[i] = float(v[i])
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def empty_particles(n: int) -> dict:
+    return {name: np.zeros(n, dtype=dt) for name, dt in PARTICLE_FIELDS}
+
+
+def _soa_struct(arrs: dict) -> SpParticles:
+    s = SpParticles()
+    for name, dt in PARTICLE_FIELDS:
+        a = arrs.get(name)
+        if a is not None:
+            ct = np.ctypeslib.as_ctypes_type(dt)
+            setattr(s, name, a.ctypes.data_as(C.POINTER(ct)))
+    return s
+
+
+class ParkerPusher:
+    def __init__(self, **config):
+        self.lib = _abi.load()
+        self.cfg = make_config(**config)
+        ctx = C.c_void_p()
+        check(self.lib, "sp_ctx_create", self.lib.sp_ctx_create(C.byref(self.cfg), C.byref(ctx)))
+        self.ctx = ctx
+        nxg = self.cfg.nx + 4
+        nyg = self.cfg.ny + 4
+        nzg = self.cfg.nz + 4 if self.cfg.ndim == 3 else self.cfg.nz
+        self.shape = (nzg, nyg, nxg)
+        self._dist_dims = None
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if self.ctx:
+            self.lib.sp_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        check(self.lib, name, getattr(self.lib, name)(self.ctx, *args))
+
+    # -- fields -----------------------------------------------------------
+    def upload_fields(self, nodes8: np.ndarray, slot: int = 0):
+        """read_field_data_parallel: nodes8 is (nzg, nyg, nxg, 8) float32."""
+        a = np.ascontiguousarray(nodes8, dtype=np.float32)
+        if a.shape != self.shape + (8,):
+            raise ValueError(f"nodes8 must have shape {self.shape + (8,)}, got {a.shape}")
+        self._call("sp_fields_upload", slot, a.ctypes.data_as(C.POINTER(C.c_float)), a.size)
+
+    def calc_fields_gradients(self, slot: int = 0, return_farray: bool = False):
+        out = None
+        ptr = None
+        if return_farray:
+            out = np.zeros(self.shape + (32,), dtype=np.float32)
+            ptr = out.ctypes.data_as(C.POINTER(C.c_float))
+        self._call("sp_fields_gradients", slot, ptr)
+        return out
+
+    def copy_fields(self):
+        self._call("sp_fields_copy")
+
+    # -- particles --------------------------------------------------------
+    def set_particles(self, arrs: dict):
+        n = len(arrs["x"])
+        full = {name: np.ascontiguousarray(arrs[name], dtype=dt) for name, dt in PARTICLE_FIELDS}
+        s = _soa_struct(full)
+        self._call("sp_particles_upload", C.byref(s), n)
+
+    def counts(self) -> dict:
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        t = C.c_int32()
+        self._call("sp_particles_counts", C.byref(a), C.byref(b), C.byref(c), C.byref(t))
+        return dict(nptl_current=a.value, nptl_split=b.value, nptl_escaped=c.value, tag_max=t.value)
+
+    def get_particles(self) -> dict:
+        n = self.counts()["nptl_current"]
+        arrs = empty_particles(n)
+        s = _soa_struct(arrs)
+        got = C.c_int64()
+        self._call("sp_particles_download", C.byref(s), n, C.byref(got))
+        return arrs
+
+    def inject_particles_spatial_uniform(self, nptl, dt, dist_flag, particle_v0, t0, dtf,
+                                         part_box, power_index=7.0):
+        box = np.ascontiguousarray(part_box, dtype=np.float64)
+        self._call("sp_inject_particles_spatial_uniform", int(nptl), float(dt), int(dist_flag),
+                   float(particle_v0), float(t0), float(dtf), dptr(box), float(power_index))
+
+    def particle_mover(self, t0, dtf, num_fine_steps=1, nsteps_interval=10) -> dict:
+        st = SpMoverStats()
+        self._call("sp_particle_mover", float(t0), float(dtf), int(num_fine_steps),
+                   int(nsteps_interval), C.byref(st))
+        return {k: getattr(st, k) for k, _ in SpMoverStats._fields_}
+
+    def split_particle(self, split_ratio, pmin_split):
+        self._call("sp_split_particle", float(split_ratio), float(pmin_split))
+
+    def step_once(self, state: np.ndarray, rands: np.ndarray, fixed: np.ndarray, t0: float,
+                  dtf: float):
+        st = np.ascontiguousarray(state, dtype=np.float64).copy()
+        r = np.ascontiguousarray(rands, dtype=np.float64)
+        fx = np.ascontiguousarray(fixed, dtype=np.int8)
+        n = st.shape[0]
+        d = np.zeros((n, 4), dtype=np.float64)
+        self._call("sp_step_once", n, float(t0), float(dtf), dptr(st), dptr(r),
+                   fx.ctypes.data_as(C.POINTER(C.c_int8)), dptr(d))
+        return st, d
+
+    # -- diagnostics ------------------------------------------------------
+    def init_particle_distributions(self, npp_global=256, nmu_global=1, local_dist=0,
+                                    local=()):
+        dc = SpDistConfig()
+        dc.npp_global = npp_global
+        dc.nmu_global = nmu_global
+        dc.local_dist = int(local_dist)
+        for k, L in enumerate(local):
+            if L is None:
+                continue
+            for key in ("enabled", "npbins", "nmu", "rx", "ry", "rz", "pmin", "pmax"):
+                setattr(dc.local[k], key, L[key])
+        self._call("sp_distributions_init", C.byref(dc))
+        self._npp = npp_global * nmu_global
+        dims = []
+        for k in range(4):
+            d = (C.c_int64 * 5)()
+            self._call("sp_local_dist_dims", k + 1, d)
+            dims.append(tuple(d))
+        self._dist_dims = dims
+
+    def calc_particle_distributions(self):
+        """Returns (fglobal, [flocal1..4]); flocal arrays are Fortran-ordered
+        (nmu, npbins, nrx, nry, nrz) reshaped to C order (nrz, nry, nrx, npbins, nmu)."""
+        fg = np.zeros(self._npp, dtype=np.float64)
+        outs = []
+        ptrs = []
+        for d in self._dist_dims:
+            size = int(np.prod(d)) if d[0] else 0
+            if size:
+                a = np.zeros(size, dtype=np.float64)
+                outs.append(a)
+                ptrs.append(dptr(a))
+            else:
+                outs.append(None)
+                ptrs.append(None)
+        self._call("sp_distributions", dptr(fg), *ptrs)
+        shaped = []
+        for a, d in zip(outs, self._dist_dims):
+            shaped.append(None if a is None else a.reshape(d[4], d[3], d[2], d[1], d[0]))
+        return fg, shaped
+
+    def quick_check(self) -> dict:
+        out = np.zeros(9, dtype=np.float64)
+        self._call("sp_quick_check", dptr(out))
+        keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
+                "pdt_avg", "pmax")
+        return dict(zip(keys, out.tolist()))

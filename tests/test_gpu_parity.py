@@ -1,0 +1,252 @@
+"""GPU parity of libsp_hip.so against the CPU oracle, through the C ABI.
+
+Integer/byte results (counts, flags, tags, step counters, histogram bins)
+must be identical.  FP64-mode floating point must agree with the oracle to
+1e-12 relative per step (both evaluate the same unfused IEEE operations;
+pow/exp/log10 come from different math libraries) and to 1e-9 over whole
+intervals of a few hundred steps."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import cfg2d, cfg3d, nodes_for, random_states
+from sp_hip import ParkerPusher
+
+pytestmark = pytest.mark.gpu
+
+SQ3 = np.sqrt(3.0)
+STATE_COLS = ("x", "y", "z", "p", "t", "dt")
+
+
+def make(c, nodes, precision="f64", nptl_max=20000, nodes2=None):
+    cc = dict(c)
+    cc["precision"] = precision
+    cc["nptl_max"] = nptl_max
+    pz = ParkerPusher(**cc)
+    pz.upload_fields(nodes, 0)
+    fa_gpu = pz.calc_fields_gradients(0, return_farray=True)
+    if nodes2 is not None:
+        pz.upload_fields(nodes2, 1)
+        pz.calc_fields_gradients(1)
+    return pz, fa_gpu
+
+
+def rel_err(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def by_id(arrs):
+    key = np.asarray(arrs["tag_injected"], np.int64) * (1 << 32) + np.asarray(arrs["tag_splitted"], np.int64)
+    o = np.argsort(key)
+    return {k: np.asarray(v)[o] for k, v in arrs.items()}
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_gradients_bit_exact(ndim):
+    c = cfg2d(48) if ndim == 2 else cfg3d(12)
+    nodes = nodes_for(c)
+    pz, fa_gpu = make(c, nodes)
+    fa = oracle.gradients(oracle.params(c), nodes)
+    np.testing.assert_array_equal(fa_gpu, fa)
+    pz.close()
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("law", ["uniform", "gauss"])
+def test_step_once_f64_matches_oracle(ndim, law):
+    c = cfg2d(32) if ndim == 2 else cfg3d(12)
+    c = dict(c, check_drift_2d=1, mag_dependency=1 if ndim == 2 else 0)
+    nodes = nodes_for(c)
+    pz, _ = make(c, nodes)
+    prm = oracle.params(c)
+    fa = oracle.gradients(prm, nodes)
+    n = 3000
+    a = random_states(c, n, seed=17)
+    rng = np.random.default_rng(18)
+    if law == "uniform":
+        r = rng.uniform(-SQ3, SQ3, size=(n, 4))
+    else:
+        r = rng.standard_normal(size=(n, 4))
+    fixed = rng.integers(0, 2, n).astype(np.int8)
+    t0, dtf = 0.0, 0.05
+    state = np.stack([a[k] for k in STATE_COLS], 1)
+    st_gpu, d_gpu = pz.step_once(state, r, fixed, t0, dtf)
+    st_ref = np.zeros_like(state)
+    d_ref = np.zeros((n, 4))
+    for i in range(n):
+        row, d = oracle.step_once(prm, fa, a[i], t0, dtf, fixed[i], r[i])
+        st_ref[i] = [row[k] for k in STATE_COLS]
+        d_ref[i] = d
+    assert np.max(rel_err(st_gpu, st_ref)) < 1e-12
+    scale = np.abs(d_ref).max(axis=0)
+    assert np.all(np.abs(d_gpu - d_ref) <= 1e-12 * np.maximum(np.abs(d_ref), 1e-6 * scale) + 1e-300)
+    pz.close()
+
+
+@pytest.mark.parametrize("dist_flag", [0, 1, 2])
+def test_injection_matches_oracle(dist_flag):
+    c = cfg2d(32, pmin=1e-2, pmax=10.0, p0=0.1)
+    pz, _ = make(c, nodes_for(c), nptl_max=6000)
+    box = (0.1, 0.2, 0.0, 0.7, 0.8, 1.0)
+    pz.inject_particles_spatial_uniform(5000, 1e-7, dist_flag, 5.0, 0.5, 0.25, box, 3.0)
+    g = pz.get_particles()
+    ref, tag_max = oracle.inject(oracle.params(c), np.zeros(0, oracle.PARTICLE_DTYPE), 6000, 5000,
+                                 1e-7, dist_flag, 5.0, 0.5, 0.25, box, 3.0)
+    assert pz.counts()["tag_max"] == tag_max == 5000
+    for k in ("x", "y", "z", "mu", "t", "dt", "weight"):
+        np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
+    for k in ("count_flag", "tag_injected", "tag_splitted", "split_times", "rng_step"):
+        np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
+    # momentum laws use exp/pow from the two math libraries
+    np.testing.assert_allclose(g["p"], ref["p"], rtol=1e-14)
+    np.testing.assert_allclose(g["v"], ref["v"], rtol=1e-14)
+    pz.close()
+
+
+def _mover_case(c, nodes, n, dtf, t0=0.0, nodes2=None, mixed=False, intervals=1, split=None,
+                nfine=1):
+    pz, _ = make(c, nodes, nptl_max=4 * n, nodes2=nodes2)
+    prm = oracle.params(c)
+    fa = oracle.gradients(prm, nodes)
+    fa2 = oracle.gradients(prm, nodes2) if nodes2 is not None else None
+    box = (c["xmin"], c["ymin"], c["zmin"], c["xmax"], c["ymax"], c["zmax"])
+    pz.inject_particles_spatial_uniform(n, 1e-7, 1, 5.0, t0, dtf, box)
+    ref, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), 4 * n, n, 1e-7, 1, 5.0, t0,
+                           dtf, box, 7.0)
+    if mixed:
+        g = pz.get_particles()
+        rng = np.random.default_rng(3)
+        flags = rng.choice([1] * 8 + [0, -1], size=n).astype(np.int8)
+        neg = rng.random(n) < 0.02
+        g["count_flag"] = flags
+        ref["count_flag"] = flags
+        g["p"] = np.where(neg, -0.5, g["p"])
+        ref["p"] = np.where(neg, -0.5, ref["p"])
+        pz.set_particles(g)
+    steps_ref = 0
+    leak_ref = negp_ref = 0.0
+    nsplit_ref = 0
+    for it in range(intervals):
+        t_a = t0 + it * dtf
+        st = pz.particle_mover(t_a, dtf, nfine, 10)
+        ref, acc = oracle.mover(prm, fa, ref, t_a, dtf, num_fine_steps=nfine, nthreads=8, fa2=fa2)
+        steps_ref += acc.steps
+        leak_ref += acc.leak
+        negp_ref += acc.leak_negp
+        assert st["steps"] == acc.steps
+        if split is not None:
+            pz.split_particle(*split)
+            ref, ns = oracle.split(prm, ref, 4 * n, *split)
+            nsplit_ref += ns
+    g = by_id(pz.get_particles())
+    r = by_id(oracle.soa_from_particles(ref))
+    assert len(g["x"]) == len(r["x"])
+    for k in ("count_flag", "tag_injected", "tag_splitted", "split_times", "rng_step",
+              "nsteps_pushed", "weight"):
+        np.testing.assert_array_equal(g[k], r[k], err_msg=k)
+    for k in STATE_COLS:
+        err = rel_err(g[k], r[k]) if k not in ("x", "y", "z") else np.abs(g[k] - r[k])
+        assert np.max(err, initial=0.0) < 1e-9, (k, np.max(err))
+    q = pz.quick_check()
+    assert q["leak"] == leak_ref and q["leak_negp"] == negp_ref
+    assert q["nptl_split"] == nsplit_ref
+    pz.close()
+    return steps_ref
+
+
+# Test intervals: dt_max_rel = 1e-2 caps a step at dtf / 100, so one interval
+# of dtf = 0.2 is ~100-200 steps per particle on these coarse grids.
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_mover_f64_matches_oracle(ndim):
+    if ndim == 2:
+        c = cfg2d(32, pbcy=1, dt_max_rel=1e-2)
+    else:
+        c = cfg3d(12, pbcx=1, dt_max_rel=1e-2)
+    steps = _mover_case(c, nodes_for(c), 2000, 0.2)
+    assert steps > 2000 * 50
+
+
+def test_mover_f64_flags_negative_p_and_fine_steps():
+    c = cfg2d(32, pbcy=1, dt_max_rel=1e-2)
+    _mover_case(c, nodes_for(c), 1500, 0.2, mixed=True, nfine=3)
+
+
+def test_mover_f64_multi_interval_with_split():
+    # the snapshot's compressions move momenta by ~10-20 % per interval, enough
+    # to cross split thresholds at 1.05 p0 * 1.2^k
+    c = cfg2d(32, pbcy=1, p0=1.0, dt_max_rel=1e-2)
+    _mover_case(c, nodes_for(c), 1500, 0.2, intervals=3, split=(1.2, 1.05))
+
+
+def test_mover_f64_time_interpolation():
+    c = cfg2d(32, pbcy=1, time_interp=1, dt_max_rel=1e-2)
+    n1 = nodes_for(c, seed=1)
+    n2 = nodes_for(c, seed=2)
+    _mover_case(c, n1, 1500, 0.2, nodes2=n2)
+
+
+def test_distributions_and_quick_check_bit_exact():
+    c = cfg2d(64)
+    pz, _ = make(c, nodes_for(c), nptl_max=50000)
+    a = random_states(c, 40000, seed=31, slack=False)
+    rng = np.random.default_rng(31)
+    a["p"] = np.exp(rng.uniform(np.log(5e-3), np.log(2e4), len(a)))
+    a["weight"] = 0.5 ** rng.integers(0, 6, len(a))
+    a["mu"] = rng.uniform(-0.99, 0.99, len(a))
+    pz.set_particles(oracle.soa_from_particles(a))
+    dcfg = dict(npp_global=256, nmu_global=1, local_dist=1,
+                local=[dict(enabled=1, npbins=16, nmu=1, rx=4, ry=4, rz=1, pmin=1e-2, pmax=1e4),
+                       dict(enabled=1, npbins=8, nmu=2, rx=8, ry=16, rz=1, pmin=0.1, pmax=1e3),
+                       None, None])
+    pz.init_particle_distributions(**dcfg)
+    fg, locs = pz.calc_particle_distributions()
+    fg_ref, locs_ref = oracle.distributions(oracle.params(c), dcfg, a)
+    np.testing.assert_array_equal(fg, fg_ref)
+    np.testing.assert_array_equal(locs[0].ravel(), locs_ref[0])
+    np.testing.assert_array_equal(locs[1].ravel(), locs_ref[1])
+    # arrays are cleared after each call: a second call gives the same result
+    fg2, _ = pz.calc_particle_distributions()
+    np.testing.assert_array_equal(fg2, fg_ref)
+    q = pz.quick_check()
+    qr = oracle.quick_check(a, 0, 0.0, 0.0)
+    for k in ("nptl_current", "ntot", "pdt_min", "pdt_max", "pmax"):
+        assert q[k] == qr[k], k
+    assert q["pdt_avg"] == pytest.approx(qr["pdt_avg"], rel=1e-12)
+    pz.close()
+
+
+def test_split_matches_oracle_including_capacity():
+    c = cfg2d(16)
+    a = random_states(c, 5000, seed=40)
+    rng = np.random.default_rng(41)
+    k = rng.integers(0, 3, len(a)).astype(np.int8)
+    a["split_times"] = k
+    a["weight"] = 0.5 ** k
+    a["p"] = np.exp(rng.uniform(np.log(0.3), np.log(40.0), len(a)))
+    for cap in (20000, 5000 + 123):
+        pz, _ = make(c, nodes_for(c), nptl_max=cap)
+        pz.set_particles(oracle.soa_from_particles(a))
+        pz.split_particle(2.72, 2.0)
+        ref, ns = oracle.split(oracle.params(c), a, cap, 2.72, 2.0)
+        assert pz.counts()["nptl_split"] == ns
+        g = by_id(pz.get_particles())
+        r = by_id(oracle.soa_from_particles(ref))
+        for key in r:
+            np.testing.assert_array_equal(g[key], r[key], err_msg=key)
+        pz.close()
+
+
+def test_empty_population_and_state_errors():
+    c = cfg2d(16)
+    pz = ParkerPusher(**dict(c, nptl_max=10))
+    with pytest.raises(Exception, match="not prepared"):
+        pz.particle_mover(0.0, 1.0)
+    pz.upload_fields(nodes_for(c))
+    pz.calc_fields_gradients()
+    st = pz.particle_mover(0.0, 1.0)
+    assert st["steps"] == 0 and st["nptl_current"] == 0
+    pz.split_particle(2.0, 2.0)
+    assert pz.counts()["nptl_current"] == 0
+    q = pz.quick_check()
+    assert q["nptl_current"] == 0 and q["pdt_min"] == 1.0 and q["pdt_max"] == 0.0
+    pz.close()

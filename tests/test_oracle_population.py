@@ -1,0 +1,164 @@
+"""Pins the oracle's population routines: injection laws (KS against the
+analytic CDFs), removal and split invariants, and the distribution
+diagnostics against numpy histograms."""
+import numpy as np
+import pytest
+from scipy import stats
+
+import oracle
+from helpers import cfg2d, random_states
+
+BOX = (0.1, 0.2, 0.0, 0.6, 0.9, 1.0)
+
+
+def test_inject_delta_uniform_box():
+    c = cfg2d(32)
+    prm = oracle.params(c)
+    n = 5000
+    out, tag_max = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), 10000, n, 1e-7, 1, 5.0,
+                                 0.5, 0.25, BOX, 7.0)
+    assert len(out) == n and tag_max == n
+    np.testing.assert_array_equal(out["tag_injected"], np.arange(n))
+    assert np.all(out["tag_splitted"] == 1) and np.all(out["count_flag"] == 1)
+    assert np.all(out["p"] == c["p0"]) and np.all(out["weight"] == 1.0)
+    assert np.all(out["v"] == 5.0 * out["p"] / c["p0"])
+    assert np.all(out["dt"] == 1e-7)
+    mu_max = float(np.float32(0.99))
+    assert np.all(np.abs(out["mu"]) <= mu_max)
+    for key, lo, hi in (("x", BOX[0], BOX[3]), ("y", BOX[1], BOX[4]), ("z", BOX[2], BOX[5]),
+                        ("t", 0.5, 0.75)):
+        u = (out[key] - lo) / (hi - lo)
+        assert stats.kstest(u, "uniform").pvalue > 1e-3, key
+
+
+def test_inject_power_law_and_maxwellian_laws():
+    c = cfg2d(32, pmin=1e-2, pmax=10.0, p0=0.1)
+    prm = oracle.params(c)
+    n = 6000
+    # power law f(p) ~ p^-q on [p0, pmax] by inversion
+    q = 3.0
+    out, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), n, n, 1e-7, 2, 1.0, 0.0, 1.0,
+                           BOX, q)
+    p0, pm = c["p0"], c["pmax"]
+
+    def cdf_pl(p):
+        return (p ** (1 - q) - p0 ** (1 - q)) / (pm ** (1 - q) - p0 ** (1 - q))
+
+    assert stats.kstest(out["p"], cdf_pl).pvalue > 1e-3
+    # Maxwellian rejection sampling of x^2 exp(-x^2), x = p/p0 on [pmin, pmax]/p0
+    out, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), n, n, 1e-7, 0, 1.0, 0.0, 1.0,
+                           BOX, q)
+    xs = np.linspace(c["pmin"] / p0, pm / p0, 200001)
+    pdf = xs ** 2 * np.exp(-xs ** 2)
+    cdf = np.concatenate([[0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(xs))])
+    cdf /= cdf[-1]
+    assert stats.kstest(out["p"] / p0, lambda v: np.interp(v, xs, cdf)).pvalue > 1e-3
+
+
+def test_inject_past_capacity_overwrites_last_slot():
+    prm = oracle.params(cfg2d(16))
+    out, tag_max = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), 5, 8, 1e-7, 1, 1.0, 0.0,
+                                 1.0, BOX, 7.0)
+    assert len(out) == 5 and tag_max == 8
+    np.testing.assert_array_equal(out["tag_injected"], [0, 1, 2, 3, 7])
+
+
+def test_remove_particles_invariants():
+    c = cfg2d(16)
+    a = random_states(c, 1000, seed=4)
+    rng = np.random.default_rng(4)
+    a["count_flag"] = rng.choice([1, 1, 1, 0, -1, -2, -3, -4], size=len(a))
+    kept, nesc = oracle.remove(a)
+    assert np.all(kept["count_flag"] == 1)
+    assert nesc == int(np.sum(a["count_flag"] < 0))
+    assert sorted(kept["tag_injected"]) == sorted(a["tag_injected"][a["count_flag"] == 1])
+    # the survivors carry their state unchanged
+    order = np.argsort(kept["tag_injected"])
+    src = a[a["count_flag"] == 1]
+    np.testing.assert_array_equal(kept[order]["x"], src[np.argsort(src["tag_injected"])]["x"])
+
+
+def test_split_particle_invariants():
+    c = cfg2d(16, p0=1.0, pmax=1e4)
+    prm = oracle.params(c)
+    a = random_states(c, 2000, seed=8)
+    a["p"] = np.exp(np.random.default_rng(8).uniform(np.log(0.3), np.log(40.0), len(a)))
+    rng = np.random.default_rng(9)
+    k = rng.integers(0, 3, len(a)).astype(np.int8)
+    a["split_times"] = k
+    a["weight"] = 0.5 ** k
+    a["tag_splitted"] = 1
+    ratio, pmin_split = 2.72, 2.0
+    thr = pmin_split * 1.0 * ratio ** k
+    cand = (a["p"] > thr) & (a["p"] <= c["pmax"])
+    out, nsplit = oracle.split(prm, a, 10000, ratio, pmin_split)
+    assert nsplit == cand.sum() and len(out) == len(a) + nsplit
+    # total weight conserved; children follow in candidate order
+    assert out["weight"].sum() == a["weight"].sum()
+    children = out[len(a):]
+    np.testing.assert_array_equal(children["tag_injected"], a["tag_injected"][cand])
+    np.testing.assert_array_equal(children["tag_splitted"], 1 + 2 ** k[cand])
+    np.testing.assert_array_equal(children["split_times"], k[cand] + 1)
+    np.testing.assert_array_equal(out["split_times"][: len(a)][cand], k[cand] + 1)
+    np.testing.assert_array_equal(children["p"], a["p"][cand])
+    # (tag_injected, tag_splitted) stays a unique particle id
+    ids = set(zip(out["tag_injected"].tolist(), out["tag_splitted"].tolist()))
+    assert len(ids) == len(out)
+    # at capacity, only the first candidates in array order split
+    room = 17
+    out2, ns2 = oracle.split(prm, a, len(a) + room, ratio, pmin_split)
+    assert ns2 == room and len(out2) == len(a) + room
+    first = np.flatnonzero(cand)[:room]
+    np.testing.assert_array_equal(out2[len(a):]["tag_injected"], a["tag_injected"][first])
+    assert np.all(out2["split_times"][np.flatnonzero(cand)[room:]] == k[cand][room:])
+
+
+def _dist_cfg(map_bins=8, npbins=16):
+    return dict(npp_global=64, nmu_global=1, local_dist=1,
+                local=[dict(enabled=1, npbins=npbins, nmu=1, rx=4, ry=4, rz=1, pmin=1e-2, pmax=1e4),
+                       None, None,
+                       dict(enabled=1, npbins=4, nmu=2, rx=8, ry=2, rz=1, pmin=0.1, pmax=100.0)])
+
+
+def test_distributions_match_numpy_histograms():
+    c = cfg2d(32)
+    prm = oracle.params(c)
+    a = random_states(c, 20000, seed=12, slack=False)
+    rng = np.random.default_rng(12)
+    a["p"] = np.exp(rng.uniform(np.log(5e-3), np.log(2e4), len(a)))
+    a["weight"] = 0.5 ** rng.integers(0, 4, len(a))
+    a["mu"] = rng.uniform(-1, 1, len(a))
+    dcfg = _dist_cfg()
+    fg, locs = oracle.distributions(prm, dcfg, a)
+    edges = 10.0 ** np.linspace(-2, 4, 65)
+    sel = (a["p"] > 1e-2) & (a["p"] <= 1e4)
+    h, _ = np.histogram(a["p"][sel], bins=edges, weights=a["weight"][sel])
+    np.testing.assert_allclose(fg, h, rtol=0, atol=1e-12)
+    # local set 1: (ip, ix, iy) with ip < npbins; top momentum bin dropped
+    f1 = locs[0].reshape(1, 8, 8, 16, 1)[0]  # (iy, ix, ip, imu)
+    pe = 10.0 ** np.linspace(-2, 4, 17)
+    H, _ = np.histogramdd(np.stack([a["y"], a["x"], a["p"]], 1),
+                          bins=[np.linspace(0, 1, 9), np.linspace(0, 1, 9), pe], weights=a["weight"])
+    H[..., -1] = 0.0
+    np.testing.assert_allclose(f1[..., 0], H, atol=1e-12)
+    # local set 4 with two mu bins
+    f4 = locs[3].reshape(16, 4, 4, 2)  # (iy, ix, ip, imu)
+    pe4 = 10.0 ** np.linspace(-1, 2, 5)
+    H4, _ = np.histogramdd(np.stack([a["y"], a["x"], a["p"], a["mu"]], 1),
+                           bins=[np.linspace(0, 1, 17), np.linspace(0, 1, 5), pe4,
+                                 np.linspace(-1, 1, 3)], weights=a["weight"])
+    H4[:, :, -1, :] = 0.0
+    np.testing.assert_allclose(f4, H4, atol=1e-12)
+
+
+def test_quick_check_matches_numpy():
+    c = cfg2d(16)
+    a = random_states(c, 3000, seed=13)
+    a["weight"] = 0.5 ** np.random.default_rng(1).integers(0, 3, len(a))
+    q = oracle.quick_check(a, 7, 1.5, 0.25)
+    assert q["nptl_current"] == len(a) and q["nptl_split"] == 7
+    assert q["ntot"] == a["weight"].sum()
+    assert q["leak"] == 1.5 and q["leak_negp"] == 0.25
+    assert q["pdt_min"] == min(1.0, a["dt"].min()) and q["pdt_max"] == a["dt"].max()
+    assert q["pdt_avg"] == pytest.approx(a["dt"].mean(), rel=1e-13)
+    assert q["pmax"] == a["p"].max()
