@@ -82,7 +82,7 @@ typedef struct sp_config {
     int32_t precision;          /* sp_precision */
     uint64_t seed;              /* Philox key */
     uint32_t stream;            /* Philox stream word (run / variant id) */
-    int32_t reserved0;
+    int32_t sort_every;         /* order particles spatially before every k-th mover call (0: never) */
     int64_t nptl_max;           /* -nm */
 } sp_config;
 
@@ -164,6 +164,12 @@ int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int3
  * apply the exact-bounds boundary pass and remove non-INBOX particles. */
 int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
                       int32_t nsteps_interval, sp_mover_stats* stats);
+/* No reference counterpart (the reference never reorders particles; its
+ * quicksort_particle, particle_module.f90:5485-5512, is unused): orders the
+ * particle arrays by a Morton key of coarse grid blocks so that particles
+ * pushed together gather from neighbouring field records.  Changes the array
+ * order only; every particle's result is unchanged. */
+int sp_sort_particles(void* ctx);
 /* split_particle (particle_module.f90:5430-5480) */
 int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
 /* One gather + kappa + push per particle from caller-supplied states and
@@ -184,6 +190,10 @@ int sp_distributions_init(void* ctx, const sp_dist_config* dc);
 int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal2,
                      double* flocal3, double* flocal4);
 int sp_local_dist_dims(void* ctx, int32_t set, int64_t dims[5]);
+/* No reference counterpart: a plain device-to-device streaming copy of
+ * `bytes` (multiple of 16) on the context stream, timed with HIP events.  It
+ * calibrates hardware memory counters against a kernel of known traffic. */
+int sp_calibration_copy(void* ctx, int64_t bytes, double* ms);
 /* quick_check + get_pmax_global (diagnostics.f90:116-170,1691-1719):
  * out = {nptl_current, nptl_split, ntot, leak, leak_negp, pdt_min, pdt_max,
  *        pdt_avg, pmax} */

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -33,6 +34,7 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr double kInvFx = 1.0 / sp::kFxScale;
+constexpr int kMoverBlocksPerCU = 2;
 
 struct Ctx {
     sp_config cfg{};
@@ -58,6 +60,12 @@ struct Ctx {
     unsigned long long* totals = nullptr;
     double* stats_part = nullptr;
     int stats_grid = 1024;
+    // spatial sort scratch
+    uint32_t* skey[2] = {nullptr, nullptr};
+    uint32_t* sidx[2] = {nullptr, nullptr};
+    unsigned* shist = nullptr;
+    unsigned long long* soff = nullptr;
+    long long mover_calls = 0;
     int mover_blocks_per_cu[2] = {0, 0};
     // distributions
     bool dist_ready = false;
@@ -75,22 +83,28 @@ void free_dev(void* p) {
     if (p) (void)hipFree(p);
 }
 
-// One allocation per particle buffer set; arrays 256-byte aligned.
+// One allocation per particle buffer set: capacity rounded up to 256 so every
+// array starts 256-byte aligned; array order and offsets are those of
+// sp::PtlBase (9 FP64 arrays, 2 int8 arrays, 6 int32 arrays).
+long long particle_cap(long long nmax) { return (nmax + 255) / 256 * 256; }
+
 int alloc_particles(long long nmax, sp::Ptl& P, void*& base) {
-    const size_t n = (size_t)nmax;
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t sz_d = al(n * 8), sz_b = al(n), sz_i = al(n * 4);
-    const size_t total = 9 * sz_d + 2 * sz_b + 6 * sz_i;
+    const long long cap = particle_cap(nmax);
+    const size_t total = (size_t)98 * (size_t)cap;
     if (hipMalloc(&base, total) != hipSuccess)
         return fail(SP_ERR_NOMEM, "device allocation of " + std::to_string(total) + " bytes for particles failed");
-    char* c = static_cast<char*>(base);
-    double** d[9] = {&P.x, &P.y, &P.z, &P.p, &P.v, &P.mu, &P.w, &P.t, &P.dt};
-    for (auto* q : d) { *q = reinterpret_cast<double*>(c); c += sz_d; }
-    P.split_times = reinterpret_cast<int8_t*>(c); c += sz_b;
-    P.count_flag = reinterpret_cast<int8_t*>(c); c += sz_b;
-    int32_t** i4[5] = {&P.origin, &P.nsteps_tracked, &P.nsteps_pushed, &P.tag_injected, &P.tag_splitted};
-    for (auto* q : i4) { *q = reinterpret_cast<int32_t*>(c); c += sz_i; }
-    P.rng_step = reinterpret_cast<uint32_t*>(c);
+    const sp::PtlBase B{static_cast<char*>(base), cap};
+    P.x = B.d(sp::PD_X); P.y = B.d(sp::PD_Y); P.z = B.d(sp::PD_Z); P.p = B.d(sp::PD_P);
+    P.v = B.d(sp::PD_V); P.mu = B.d(sp::PD_MU); P.w = B.d(sp::PD_W); P.t = B.d(sp::PD_T);
+    P.dt = B.d(sp::PD_DT);
+    P.split_times = B.b(sp::PB_SPLIT);
+    P.count_flag = B.b(sp::PB_FLAG);
+    P.origin = B.i(sp::PI_ORIGIN);
+    P.nsteps_tracked = B.i(sp::PI_TRACKED);
+    P.nsteps_pushed = B.i(sp::PI_PUSHED);
+    P.tag_injected = B.i(sp::PI_TAGI);
+    P.tag_splitted = B.i(sp::PI_TAGS);
+    P.rng_step = reinterpret_cast<uint32_t*>(B.i(sp::PI_RSTEP));
     return SP_OK;
 }
 
@@ -110,6 +124,7 @@ int validate(const sp_config* c) {
     if (c->pbcx < 0 || c->pbcx > 1 || c->pbcy < 0 || c->pbcy > 1 || c->pbcz < 0 || c->pbcz > 1) return fail(SP_ERR_ARG, "pbc must be 0 or 1");
     if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32) return fail(SP_ERR_ARG, "precision must be SP_PREC_F64 or SP_PREC_F32");
     if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
+    if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
     return SP_OK;
 }
 
@@ -156,10 +171,112 @@ Ctx* live_ctx(void* p, int& rc) {
 // Occupancy-sized persistent grid for the mover.
 int mover_grid(Ctx* c, long long n) {
     const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
-    const int per_cu = std::max(1, c->mover_blocks_per_cu[prec]);
+    // Resident 256-lane blocks per CU.  Particles in flight are a window of
+    // consecutive (spatially sorted) indices; a smaller window keeps its field
+    // records in L2.  Two blocks (8 waves) per CU measured best on MI355X for
+    // both precisions on the 2048^2 snapshot; SP_MOVER_BLOCKS_PER_CU overrides.
+    int per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), kMoverBlocksPerCU);
+    if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {
+        const int v = std::atoi(e);
+        if (v > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), v);
+    }
     const long long need = (n + 255) / 256;
     const long long cap = (long long)c->ncu * per_cu;
     return (int)std::max<long long>(1, std::min(need, cap));
+}
+
+// Orders the current particle set by the Morton key of 2^shift-cell blocks
+// (at most 64 blocks per axis) into the spare buffer set and swaps.
+int spatial_sort(Ctx* c) {
+    if (c->nptl <= 1) return SP_OK;
+    if (c->nptl >= (1ll << 32)) return fail(SP_ERR_UNSUPPORTED, "sorting needs fewer than 2^32 particles");
+    const long long cap = particle_cap(c->nmax);
+    if (!c->skey[0]) {
+        const long long nt = std::max<long long>(1, sp::num_tiles(c->nmax));
+        for (int i = 0; i < 2; ++i) {
+            SP_HIP(hipMalloc(&c->skey[i], sizeof(uint32_t) * cap));
+            SP_HIP(hipMalloc(&c->sidx[i], sizeof(uint32_t) * cap));
+        }
+        SP_HIP(hipMalloc(&c->shist, sizeof(unsigned) * 16 * nt));
+        SP_HIP(hipMalloc(&c->soff, sizeof(unsigned long long) * 16 * nt));
+    }
+    const sp_config& f = c->cfg;
+    sp::SortArgs sa{};
+    sa.ndim = f.ndim;
+    sa.nx = f.nx; sa.ny = f.ny; sa.nz = f.ndim == 3 ? f.nz : 1;
+    int nmaxc = std::max(sa.nx, std::max(sa.ny, sa.nz)), bits = 0;
+    while ((1 << bits) < nmaxc) ++bits;
+    // blocks of 2^shift cells per axis; SP_SORT_SHIFT overrides the default
+    sa.shift = std::max(0, bits - 6);
+    if (const char* e = std::getenv("SP_SORT_SHIFT")) sa.shift = std::max(0, std::atoi(e));
+    const int axis_bits = std::max(1, bits - sa.shift);
+    if (axis_bits > (f.ndim == 3 ? 10 : 16)) return fail(SP_ERR_UNSUPPORTED, "sort key too wide");
+    sa.passes = (f.ndim * axis_bits + 3) / 4;
+    sa.xmin = f.xmin; sa.ymin = f.ymin; sa.zmin = f.zmin;
+    sa.idx = 1.0 / f.dx; sa.idy = 1.0 / f.dy; sa.idz = 1.0 / f.dz;
+    SP_HIP(sp::launch_spatial_sort(c->P[c->cur], c->P[1 - c->cur], c->nptl, sa, c->skey[0],
+                                   c->sidx[0], c->skey[1], c->sidx[1], c->shist, c->soff,
+                                   c->totals, c->st));
+    c->cur = 1 - c->cur;
+    return SP_OK;
+}
+
+// Kernel arguments of the mover / single-step kernels for one interval.
+sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_interval) {
+    const sp_config& f = c->cfg;
+    const sp::DevParams& d = c->dp;
+    sp::MoverParams m{};
+    m.P = sp::PtlBase{static_cast<char*>(c->pbuf[c->cur]), particle_cap(c->nmax)};
+    m.n = c->nptl;
+    m.rec1 = c->rec[0];
+    m.rec2 = f.time_interp ? c->rec[1] : c->rec[0];
+    m.ctr = c->ctr;
+    m.nxg = d.nxg; m.nyg = d.nyg; m.nzg = d.nzg;
+    m.nsteps_interval = nsteps_interval;
+    m.flags = (f.pbcx == 1 ? sp::MF_OPEN_X : 0) | (f.pbcy == 1 ? sp::MF_OPEN_Y : 0) |
+              (f.ndim == 3 && f.pbcz == 1 ? sp::MF_OPEN_Z : 0) |
+              (f.momentum_dependency == 1 ? sp::MF_MOMENTUM_DEP : 0) |
+              (f.mag_dependency == 1 ? sp::MF_MAG_DEP : 0) |
+              (f.ndim == 2 && f.check_drift_2d ? sp::MF_DRIFT_2D : 0);
+    m.key0 = d.key0; m.key1 = d.key1; m.stream = d.stream;
+    m.t0 = t0;
+    m.dtf = dtf;
+    m.dt_fine = dtf / num_fine_steps;
+    m.dt_limit = dtf + m.dt_fine * (double)0.1f;  // 0.1 is a default-real literal (:1596)
+    m.xmin = f.xmin; m.ymin = f.ymin; m.zmin = f.zmin;
+    m.dx = f.dx; m.dy = f.dy; m.dz = f.dz;
+    m.idx = d.idx; m.idy = d.idy; m.idz = d.idz;
+    // slack bounds xmin - dx*0.5 ... (particle_module.f90:1527-1532)
+    m.slo[0] = f.xmin - f.dx * 0.5; m.slo[1] = f.ymin - f.dy * 0.5; m.slo[2] = f.zmin - f.dz * 0.5;
+    m.shi[0] = f.xmax + f.dx * 0.5; m.shi[1] = f.ymax + f.dy * 0.5; m.shi[2] = f.zmax + f.dz * 0.5;
+    m.elo[0] = f.xmin; m.elo[1] = f.ymin; m.elo[2] = f.zmin;
+    m.ehi[0] = f.xmax; m.ehi[1] = f.ymax; m.ehi[2] = f.zmax;
+    m.p0 = f.p0;
+    m.p_floor = 0.25 * f.p0;
+    m.kpara0 = f.kpara0;
+    m.kret = f.kret;
+    m.pindex = d.pindex;
+    m.gamma_m2 = f.gamma_turb - 2.0;
+    m.vdp_pre = (double)(1.0f / (float)(3 * f.charge));
+    m.d1p0 = f.drift1 * f.p0;
+    m.d2p02 = f.drift2 * (f.p0 * f.p0);
+    m.half_dx = 0.5 * f.dx; m.half_dy = 0.5 * f.dy; m.half_dz = 0.5 * f.dz;
+    m.dt_min = f.dt_min_rel * dtf;  // set_dt_min_max (:5519-5524)
+    m.dt_max = f.dt_max_rel * dtf;
+    m.f_kpara0 = (float)f.kpara0;
+    m.f_kret = (float)f.kret;
+    m.f_pindex = (float)d.pindex;
+    m.f_gm2 = (float)(f.gamma_turb - 2.0);
+    m.f_vdp_pre = 1.0f / (float)(3 * f.charge);
+    m.f_d1p0 = (float)m.d1p0;
+    m.f_d2p02 = (float)m.d2p02;
+    m.f_half_dx = (float)m.half_dx; m.f_half_dy = (float)m.half_dy; m.f_half_dz = (float)m.half_dz;
+    m.f_dt_min = (float)m.dt_min;
+    m.f_dt_max = (float)m.dt_max;
+    m.inv_p0 = 1.0 / f.p0;
+    for (int i = 0; i < 6; ++i) m.acc_region[i] = f.acc_region[i];
+    m.lx = f.lx; m.ly = f.ly; m.lz = f.lz;
+    return m;
 }
 
 void destroy(Ctx* c) {
@@ -176,6 +293,12 @@ void destroy(Ctx* c) {
     free_dev(c->tile_off);
     free_dev(c->totals);
     free_dev(c->stats_part);
+    for (int i = 0; i < 2; ++i) {
+        free_dev(c->skey[i]);
+        free_dev(c->sidx[i]);
+    }
+    free_dev(c->shist);
+    free_dev(c->soff);
     free_dev(c->dglob);
     for (auto* q : c->dloc) free_dev(q);
     for (auto& e : c->ev)
@@ -235,8 +358,11 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     SP_HIP_C(hipMalloc(&c->totals, sizeof(unsigned long long) * 2));
     SP_HIP_C(hipMalloc(&c->stats_part, sizeof(double) * 5 * c->stats_grid));
     // occupancy of the persistent mover decides its grid
-    c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, cfg->time_interp != 0));
-    c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, cfg->time_interp != 0));
+    {
+        const bool ti = cfg->time_interp != 0, acc = cfg->acc_region_flag == 1;
+        c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, acc));
+        c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, acc));
+    }
     SP_HIP_C(hipStreamSynchronize(c->st));
 #undef SP_HIP_C
     *out_ctx = c;
@@ -386,27 +512,20 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
     if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
     if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
-    sp::MoverArgs a{};
-    a.prm = c->dp;
-    a.P = c->P[c->cur];
-    a.rec1 = c->rec[0];
-    a.rec2 = c->cfg.time_interp ? c->rec[1] : c->rec[0];
-    a.ctr = c->ctr;
-    a.n = c->nptl;
-    a.t0 = t0;
-    a.dtf = dtf;
-    a.dt_fine = dtf / num_fine_steps;
-    a.dt_limit = dtf + a.dt_fine * (double)0.1f;  // 0.1 is a default-real literal (:1596)
-    a.dt_min = c->cfg.dt_min_rel * dtf;            // set_dt_min_max (:5519-5524)
-    a.dt_max = c->cfg.dt_max_rel * dtf;
-    a.nsteps_interval = nsteps_interval;
+    if (c->cfg.sort_every > 0 && c->mover_calls % c->cfg.sort_every == 0) {
+        rc = spatial_sort(c);
+        if (rc != SP_OK) return rc;
+    }
+    ++c->mover_calls;
+    const sp::MoverParams m = mover_params(c, t0, dtf, num_fine_steps, nsteps_interval);
+    const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
     // clear the work and step counters (leak counters are cumulative)
     SP_HIP(hipMemsetAsync(&c->ctr->work, 0, 2 * sizeof(unsigned long long), c->st));
     SP_HIP(hipEventRecord(c->ev[0], c->st));
     if (c->nptl > 0) {
         const int grid = mover_grid(c, c->nptl);
-        if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(a, grid, 256, c->st));
-        else SP_HIP(sp::launch_mover_f64(a, grid, 256, c->st));
+        if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
+        else SP_HIP(sp::launch_mover_f64(m, ti, acc, c->cfg.ndim, grid, c->st));
     }
     SP_HIP(hipEventRecord(c->ev[1], c->st));
     // remove_particles: stable compaction into the other buffer set
@@ -467,6 +586,16 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split) {
     return SP_OK;
 }
 
+int sp_sort_particles(void* ctx) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    rc = spatial_sort(c);
+    if (rc != SP_OK) return rc;
+    SP_HIP(hipStreamSynchronize(c->st));
+    return SP_OK;
+}
+
 int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, const double* rands,
                  const int8_t* fixed_dt, double* deltas) {
     int rc;
@@ -487,13 +616,14 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     if (e == hipSuccess) e = hipMemcpyAsync(ds, state, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(dr, rands, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(df, fixed_dt, (size_t)n, hipMemcpyHostToDevice, c->st);
-    const double dt_min = c->cfg.dt_min_rel * dtf, dt_max = c->cfg.dt_max_rel * dtf;
-    const float* r2 = c->cfg.time_interp ? c->rec[1] : c->rec[0];
+    sp::MoverParams m = mover_params(c, t0, dtf, 1, 1);
+    m.n = n;
+    const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
     if (e == hipSuccess) {
         if (c->cfg.precision == SP_PREC_F32)
-            e = sp::launch_step_once_f32(c->dp, c->rec[0], r2, n, t0, dtf, dt_min, dt_max, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f32(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
         else
-            e = sp::launch_step_once_f64(c->dp, c->rec[0], r2, n, t0, dtf, dt_min, dt_max, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f64(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(state, ds, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(deltas, dd, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->st);
@@ -602,6 +732,28 @@ int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal
         SP_HIP(hipMemcpy(hl.data(), c->dloc[k], sizeof(unsigned long long) * c->loc_size[k], hipMemcpyDeviceToHost));
         for (long long i = 0; i < c->loc_size[k]; ++i) outs[k][i] = (double)hl[i] * kInvFx;
     }
+    return SP_OK;
+}
+
+int sp_calibration_copy(void* ctx, int64_t bytes, double* ms) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (bytes < 16 || bytes % 16 != 0) return fail(SP_ERR_ARG, "bytes must be a positive multiple of 16");
+    void *a = nullptr, *b = nullptr;
+    hipError_t e = hipMalloc(&a, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, (size_t)bytes, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->st);
+    if (e == hipSuccess) e = sp::launch_copy(a, b, bytes / 16, c->st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+    free_dev(a);
+    free_dev(b);
+    if (e != hipSuccess) return fail(SP_ERR_HIP, std::string("calibration copy failed: ") + hipGetErrorString(e));
+    if (ms) *ms = t;
     return SP_OK;
 }
 

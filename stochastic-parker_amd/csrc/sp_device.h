@@ -40,39 +40,34 @@ __device__ __forceinline__ double u01(uint32_t r) { return (double)r * (1.0 / 42
 
 // Particle boundary condition for one rank (particle_module.f90:1984-2129):
 // periodic axes wrap by (hi - lo) of the bounds passed in, open axes mark the
-// particle as escaped and add its weight to leak.  Returns the weight leaked.
-struct Bounds {
-    double lo0, lo1, lo2, hi0, hi1, hi2;
-};
-
-__device__ __forceinline__ double boundary_condition(const DevParams& prm, double& x, double& y,
-                                                     double& z, int& cf, double w, Bounds bd) {
-    const double lo[3] = {bd.lo0, bd.lo1, bd.lo2};
-    const double hi[3] = {bd.hi0, bd.hi1, bd.hi2};
+// particle as escaped.  Returns the weight leaked through an open face.
+template <int NDIM>
+__device__ __forceinline__ double boundary_condition(int flags, double& x, double& y, double& z,
+                                                     int& cf, double w, double lo0, double lo1,
+                                                     double lo2, double hi0, double hi1,
+                                                     double hi2) {
     double leaked = 0.0;
-    if (x < lo[0] && cf == SP_COUNT_FLAG_INBOX) {
-        if (prm.pbc[0] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LX; }
-        else x = x - lo[0] + hi[0];
-    } else if (x > hi[0] && cf == SP_COUNT_FLAG_INBOX) {
-        if (prm.pbc[0] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HX; }
-        else x = x - hi[0] + lo[0];
+    if (x < lo0 && cf == SP_COUNT_FLAG_INBOX) {
+        if (flags & MF_OPEN_X) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LX; }
+        else x = x - lo0 + hi0;
+    } else if (x > hi0 && cf == SP_COUNT_FLAG_INBOX) {
+        if (flags & MF_OPEN_X) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HX; }
+        else x = x - hi0 + lo0;
     }
-    if (prm.ndim > 1) {
-        if (y < lo[1] && cf == SP_COUNT_FLAG_INBOX) {
-            if (prm.pbc[1] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LY; }
-            else y = y - lo[1] + hi[1];
-        } else if (y > hi[1] && cf == SP_COUNT_FLAG_INBOX) {
-            if (prm.pbc[1] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HY; }
-            else y = y - hi[1] + lo[1];
-        }
+    if (y < lo1 && cf == SP_COUNT_FLAG_INBOX) {
+        if (flags & MF_OPEN_Y) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LY; }
+        else y = y - lo1 + hi1;
+    } else if (y > hi1 && cf == SP_COUNT_FLAG_INBOX) {
+        if (flags & MF_OPEN_Y) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HY; }
+        else y = y - hi1 + lo1;
     }
-    if (prm.ndim == 3) {
-        if (z < lo[2] && cf == SP_COUNT_FLAG_INBOX) {
-            if (prm.pbc[2] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LZ; }
-            else z = z - lo[2] + hi[2];
-        } else if (z > hi[2] && cf == SP_COUNT_FLAG_INBOX) {
-            if (prm.pbc[2] == 1) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HZ; }
-            else z = z - hi[2] + lo[2];
+    if (NDIM == 3) {
+        if (z < lo2 && cf == SP_COUNT_FLAG_INBOX) {
+            if (flags & MF_OPEN_Z) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_LZ; }
+            else z = z - lo2 + hi2;
+        } else if (z > hi2 && cf == SP_COUNT_FLAG_INBOX) {
+            if (flags & MF_OPEN_Z) { leaked += w; cf = SP_COUNT_FLAG_ESCAPE_HZ; }
+            else z = z - hi2 + lo2;
         }
     }
     return leaked;

@@ -72,31 +72,79 @@ struct Counters {
 
 constexpr double kFxScale = 4294967296.0;  // 2^32 weight units per unit weight
 
-struct MoverArgs {
-    DevParams prm;
-    Ptl P;
+// Particle arrays of one buffer set live in one allocation with a common
+// capacity `cap` (a multiple of 256): the 9 FP64 arrays first, then the two
+// int8 arrays, then the six 32-bit arrays, in the order of Ptl.  The mover
+// addresses them from (base, cap) alone, which keeps its kernel arguments
+// small enough to stay in scalar registers.
+enum PtlD { PD_X, PD_Y, PD_Z, PD_P, PD_V, PD_MU, PD_W, PD_T, PD_DT };
+enum PtlB { PB_SPLIT, PB_FLAG };
+enum PtlI { PI_ORIGIN, PI_TRACKED, PI_PUSHED, PI_TAGI, PI_TAGS, PI_RSTEP };
+
+struct PtlBase {
+    char* base;
+    long long cap;
+    __host__ __device__ double* d(int k) const { return reinterpret_cast<double*>(base + 8 * cap * k); }
+    __host__ __device__ int8_t* b(int k) const { return reinterpret_cast<int8_t*>(base + 72 * cap + cap * k); }
+    __host__ __device__ int32_t* i(int k) const {
+        return reinterpret_cast<int32_t*>(base + 74 * cap + 4 * cap * k);
+    }
+};
+
+// Bit flags of MoverParams::flags
+enum MoverFlags {
+    MF_OPEN_X = 1, MF_OPEN_Y = 2, MF_OPEN_Z = 4,
+    MF_MOMENTUM_DEP = 8, MF_MAG_DEP = 16, MF_DRIFT_2D = 32
+};
+
+// Kernel arguments of the mover and of the single-step kernel: only the
+// values the step reads, precomputed on the host in the operation order of
+// the reference expressions they stand for.
+struct MoverParams {
+    PtlBase P;
+    long long n;
     const float* rec1;
     const float* rec2;
     Counters* ctr;
-    long long n;
-    double t0, dtf, dt_fine, dt_limit, dt_min, dt_max;
-    int nsteps_interval;
+    int nxg, nyg, nzg, nsteps_interval;
+    int flags;
+    uint32_t key0, key1, stream;
+    double t0, dtf, dt_fine, dt_limit;
+    double xmin, ymin, zmin;
+    double dx, dy, dz;            // FP64 path: p = (x - xmin) / dx
+    double idx, idy, idz;         // FP32 path: p = (x - xmin) * idx
+    double slo[3], shi[3];        // slack (half-cell) bounds used while stepping
+    double elo[3], ehi[3];        // exact bounds of particle_mover's final pass
+    double p0, p_floor;           // p_floor = 0.25 * p0
+    // FP64 physics
+    double kpara0, kret, pindex, gamma_m2;  // gamma_m2 = gamma_turb - 2
+    double vdp_pre;               // (double)(1.0f / (3 * charge)), an FP32 quotient
+    double d1p0, d2p02;           // drift1 * p0, drift2 * (p0 * p0)
+    double half_dx, half_dy, half_dz;
+    double dt_min, dt_max;
+    // FP32 physics
+    float f_kpara0, f_kret, f_pindex, f_gm2, f_vdp_pre, f_d1p0, f_d2p02;
+    float f_half_dx, f_half_dy, f_half_dz, f_dt_min, f_dt_max;
+    double inv_p0;
+    // acceleration region (template-enabled)
+    double acc_region[6];
+    double lx, ly, lz;
 };
 
 // Launchers (defined in the .hip translation units)
 hipError_t launch_gradients(const DevParams& prm, const float* nodes8, float* rec, float* farray32,
                             hipStream_t s);
-int mover_blocks_per_cu_f64(int ndim, bool ti);
-int mover_blocks_per_cu_f32(int ndim, bool ti);
-hipError_t launch_mover_f64(const MoverArgs& a, int grid, int block, hipStream_t s);
-hipError_t launch_mover_f32(const MoverArgs& a, int grid, int block, hipStream_t s);
-hipError_t launch_step_once_f64(const DevParams& prm, const float* rec1, const float* rec2,
-                                long long n, double t0, double dtf, double dt_min, double dt_max,
-                                double* state, const double* rands, const int8_t* fixed,
-                                double* deltas, hipStream_t s);
-hipError_t launch_step_once_f32(const DevParams& prm, const float* rec1, const float* rec2,
-                                long long n, double t0, double dtf, double dt_min, double dt_max,
-                                double* state, const double* rands, const int8_t* fixed,
-                                double* deltas, hipStream_t s);
+int mover_blocks_per_cu_f64(int ndim, bool ti, bool acc);
+int mover_blocks_per_cu_f32(int ndim, bool ti, bool acc);
+hipError_t launch_mover_f64(const MoverParams& m, bool ti, bool acc, int ndim, int grid,
+                            hipStream_t s);
+hipError_t launch_mover_f32(const MoverParams& m, bool ti, bool acc, int ndim, int grid,
+                            hipStream_t s);
+hipError_t launch_step_once_f64(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+                                const double* rands, const int8_t* fixed, double* deltas,
+                                hipStream_t s);
+hipError_t launch_step_once_f32(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+                                const double* rands, const int8_t* fixed, double* deltas,
+                                hipStream_t s);
 
 }  // namespace sp

@@ -10,6 +10,8 @@
 #include "sp_device.h"
 #include "sp_particles.h"
 
+#include <algorithm>
+
 namespace sp {
 
 namespace {
@@ -380,7 +382,22 @@ __global__ void __launch_bounds__(kBlock) stats_kernel(Ptl P, long long n, doubl
     }
 }
 
+// Plain streaming copy (16 B per lane) used to calibrate memory counters.
+__global__ void __launch_bounds__(kBlock) copy_kernel(const float4* __restrict__ a,
+                                                      float4* __restrict__ b, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const void* a, void* b, long long n16, hipStream_t s) {
+    const long long grid = std::min<long long>((n16 + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s,
+                       static_cast<const float4*>(a), static_cast<float4*>(b), n16);
+    return hipGetLastError();
+}
 
 hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, long long nptl,
                          long long nmax, int tag0, const InjectArgs& ia, hipStream_t s) {

@@ -22,22 +22,22 @@ struct Interp {
     float w[8];
 };
 
-__device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, double y,
-                                                double z) {
+template <int NDIM>
+__device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, double y, double z) {
     Interp I;
-    const double px = (x - prm.xmin) * prm.idx;
-    const double py = (y - prm.ymin) * prm.idy;
+    const double px = (x - m.xmin) * m.idx;
+    const double py = (y - m.ymin) * m.idy;
     const double fx = floor(px), fy = floor(py);
     const float rx = (float)(px - fx), ry = (float)(py - fy);
-    I.ix = clamp_cell(fx, prm.nxg - 4) + 2;
-    I.iy = clamp_cell(fy, prm.nyg - 4) + 2;
+    I.ix = clamp_cell(fx, m.nxg - 4) + 2;
+    I.iy = clamp_cell(fy, m.nyg - 4) + 2;
     I.iz = 0;
     float rz = 0.0f;
-    if (prm.ndim == 3) {
-        const double pz = (z - prm.zmin) * prm.idz;
+    if (NDIM == 3) {
+        const double pz = (z - m.zmin) * m.idz;
         const double fz = floor(pz);
         rz = (float)(pz - fz);
-        I.iz = clamp_cell(fz, prm.nzg - 4) + 2;
+        I.iz = clamp_cell(fz, m.nzg - 4) + 2;
     }
     const float rx1 = 1.0f - rx, ry1 = 1.0f - ry, rz1 = 1.0f - rz;
     I.w[0] = rx1 * ry1 * rz1;
@@ -52,13 +52,12 @@ __device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, 
 }
 
 template <int NDIM, bool TINTERP>
-__device__ __forceinline__ void gather(const DevParams& prm, const float* __restrict__ rec1,
-                                       const float* __restrict__ rec2, const Interp& I, float rt,
+__device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, float rt,
                                        float* __restrict__ f) {
     constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
     constexpr int NC = NDIM == 2 ? 15 : 21;
     constexpr int NCORN = NDIM == 2 ? 4 : 8;
-    const long long sx = 1, sy = prm.nxg, sz = (long long)prm.nxg * prm.nyg;
+    const long long sx = 1, sy = m.nxg, sz = (long long)m.nxg * m.nyg;
     const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
     float g[TINTERP ? NC : 1];
 #pragma unroll
@@ -70,7 +69,7 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
     for (int q = 0; q < NCORN; ++q) {
         const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
         const long long node = base + i * sx + j * sy + k * sz;
-        const float4* r4 = reinterpret_cast<const float4*>(rec1 + node * W);
+        const float4* r4 = reinterpret_cast<const float4*>(m.rec1 + node * W);
         float v[W];
 #pragma unroll
         for (int u = 0; u < W / 4; ++u) {
@@ -81,7 +80,7 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
 #pragma unroll
         for (int c = 0; c < NC; ++c) f[c] += v[c] * wq;
         if (TINTERP) {
-            const float4* s4 = reinterpret_cast<const float4*>(rec2 + node * W);
+            const float4* s4 = reinterpret_cast<const float4*>(m.rec2 + node * W);
 #pragma unroll
             for (int u = 0; u < W / 4; ++u) {
                 const float4 t = s4[u];
@@ -101,10 +100,9 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
 // One Parker step in FP32 coefficients.  Semantics match f64::push: same dt
 // rule (including the dt_min fallback when a rate is exactly zero), same
 // increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.
-template <int NDIM>
-__device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, float dt_max,
-                                           State& s, const float* f, bool fixed_dt,
-                                           const float r[4], double d[4]) {
+template <int NDIM, bool ACC>
+__device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const float* f,
+                                           bool fixed_dt, const float r[4], double d[4]) {
     const float bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
     const float by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
     const float bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
@@ -115,29 +113,27 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
     const float ib = bzero ? 0.0f : ibk;         // pusher's 1/B (0 when B = 0)
     const float ib2 = ibk * ibk, ib3 = ibk * ib2;
     const float pf = (float)s.p;
-    const float pn = (float)(s.p / prm.p0);
+    const bool mag = (m.flags & MF_MAG_DEP) != 0;
 
     // --- kappa ---------------------------------------------------------
     float knorm = 1.0f;
-    if (prm.mag_dependency == 1) knorm = __powf(b, (float)(prm.gamma_turb - 2.0));
-    if (prm.momentum_dependency == 1) knorm *= __powf(pn, (float)prm.pindex);
-    const float kpara = (float)prm.kpara0 * knorm;
-    const float kperp = kpara * (float)prm.kret;
+    if (mag) knorm = __powf(b, m.f_gm2);
+    if (m.flags & MF_MOMENTUM_DEP) knorm *= __powf((float)(s.p * m.inv_p0), m.f_pindex);
+    const float kpara = m.f_kpara0 * knorm;
+    const float kperp = kpara * m.f_kret;
     const float skpara = fsqrt(2.0f * kpara);
     const float skperp = fsqrt(2.0f * kperp);
     const float skpp = fsqrt(2.0f * (kpara - kperp));
     const float kpp = kpara - kperp;
-    const float gm2 = (float)(prm.gamma_turb - 2.0);
 
     float dkxx_dx, dkyy_dy, dkxy_dx, dkxy_dy;
     float dkzz_dz = 0.f, dkxz_dx = 0.f, dkxz_dz = 0.f, dkyz_dy = 0.f, dkyz_dz = 0.f;
     float dx_dt, dy_dt, dz_dt, divv;
-    // drift prefactor vdp (FP32 quotient as in the Fortran)
-    const float pre = 1.0f / (float)(3 * prm.pcharge);
+    // drift prefactor vdp = pre / sqrt((d1 p0/p)^2 + (d2 p0^2/p^2)^2)
     const float ip = frcp(pf);
-    const float t1 = (float)(prm.drift1 * prm.p0) * ip;
-    const float t2 = (float)(prm.drift2 * prm.p0 * prm.p0) * ip * ip;
-    const float vdp = pre * frsq(t1 * t1 + t2 * t2);
+    const float t1 = m.f_d1p0 * ip;
+    const float t2 = m.f_d2p02 * ip * ip;
+    const float vdp = m.f_vdp_pre * frsq(t1 * t1 + t2 * t2);
     float bxn = 0.f, byn = 0.f, bzn = 0.f, bxyn = 0.f, ibxyn = 0.f;
     if (NDIM == 2) {
         const float dbx_dx = f[R2_DBXDX], dbx_dy = f[R2_DBXDY];
@@ -145,7 +141,7 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
         const float dbz_dx = f[R2_DBZDX], dbz_dy = f[R2_DBZDY];
         const float db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
         float dkdx = 0.f, dkdy = 0.f;
-        if (prm.mag_dependency == 1) { dkdx = db_dx * ibk * gm2; dkdy = db_dy * ibk * gm2; }
+        if (mag) { dkdx = db_dx * ibk * m.f_gm2; dkdy = db_dy * ibk * m.f_gm2; }
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
         dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
         dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
@@ -155,7 +151,7 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
         const float vdx = vdp * (dbz_dy * ibp2 - 2.0f * bz * db_dy * ibp3);
         const float vdy = vdp * (-(dbz_dx * ibp2) + 2.0f * bz * db_dx * ibp3);
         float vdz = 0.f;
-        if (prm.check_drift_2d)
+        if (m.flags & MF_DRIFT_2D)
             vdz = vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3);
         dx_dt = f[R2_VX] + vdx + dkxx_dx + dkxy_dy;
         dy_dt = f[R2_VY] + vdy + dkxy_dx + dkyy_dy;
@@ -167,7 +163,7 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
         const float dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
         const float db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
         float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
-        if (prm.mag_dependency == 1) { dkdx = db_dx * gm2; dkdy = db_dy * gm2; dkdz = db_dz * gm2; }
+        if (mag) { dkdx = db_dx * m.f_gm2; dkdy = db_dy * m.f_gm2; dkdz = db_dz * m.f_gm2; }
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
         dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
         dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
@@ -195,24 +191,24 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
 
     float dtf32;
     if (!fixed_dt) {
-        float m;
+        float mm;
         if (dx_dt != 0.0f && dy_dt != 0.0f && dp_dt != 0.0f) {
             const float sk = skperp > 0.0f ? skperp : skpara;
             const float isk = frcp(skpara);
-            float a = 0.5f * (float)prm.dx * isk;
-            m = a * a;
-            a = 0.5f * (float)prm.dy * isk; m = fminf(m, a * a);
-            if (NDIM == 3) { a = 0.5f * (float)prm.dz * isk; m = fminf(m, a * a); }
-            a = sk * frcp(dx_dt); m = fminf(m, a * a);
-            a = sk * frcp(dy_dt); m = fminf(m, a * a);
-            if (NDIM == 3) { a = sk * frcp(dz_dt); m = fminf(m, a * a); }
-            m = fminf(m, 0.1f * pf * frcp(fabsf(dp_dt)));
+            float a = m.f_half_dx * isk;
+            mm = a * a;
+            a = m.f_half_dy * isk; mm = fminf(mm, a * a);
+            if (NDIM == 3) { a = m.f_half_dz * isk; mm = fminf(mm, a * a); }
+            a = sk * frcp(dx_dt); mm = fminf(mm, a * a);
+            a = sk * frcp(dy_dt); mm = fminf(mm, a * a);
+            if (NDIM == 3) { a = sk * frcp(dz_dt); mm = fminf(mm, a * a); }
+            mm = fminf(mm, 0.1f * pf * frcp(fabsf(dp_dt)));
         } else {
-            m = dt_min;
+            mm = m.f_dt_min;
         }
-        m = fminf(fmaxf(m, dt_min), dt_max);
-        dtf32 = m;
-        s.dt = (double)m;
+        mm = fminf(fmaxf(mm, m.f_dt_min), m.f_dt_max);
+        dtf32 = mm;
+        s.dt = (double)mm;
     } else {
         dtf32 = (float)s.dt;
     }
@@ -234,25 +230,23 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
     s.z += (double)deltaz;
     s.t += s.dt;
     double dp = (double)(dp_dt * dtf32);
-    if (prm.acc_region_flag == 1) {
-        const double xn = (s.x - prm.xmin) / prm.lx;
-        bool in = (xn >= prm.acc_region[0]) && (xn <= prm.acc_region[1]);
-        if (prm.ndim > 1) {
-            const double yn = (s.y - prm.ymin) / prm.ly;
-            in = in && (yn >= prm.acc_region[2]) && (yn <= prm.acc_region[3]);
-        }
-        if (prm.ndim == 3) {
-            const double zn = (s.z - prm.zmin) / prm.lz;
-            in = in && (zn >= prm.acc_region[4]) && (zn <= prm.acc_region[5]);
+    if (ACC) {
+        const double xn = (s.x - m.xmin) / m.lx;
+        const double yn = (s.y - m.ymin) / m.ly;
+        bool in = (xn >= m.acc_region[0]) && (xn <= m.acc_region[1]) && (yn >= m.acc_region[2]) &&
+                  (yn <= m.acc_region[3]);
+        if (NDIM == 3) {
+            const double zn = (s.z - m.zmin) / m.lz;
+            in = in && (zn >= m.acc_region[4]) && (zn <= m.acc_region[5]);
         }
         if (in) s.p += dp; else dp = 0.0;
     } else {
         s.p += dp;
     }
-    if (s.p < 0.25 * prm.p0) {
+    if (s.p < m.p_floor) {
         s.p -= dp;
-        dp = 0.25 * prm.p0 - s.p;
-        s.p = 0.25 * prm.p0;
+        dp = m.p_floor - s.p;
+        s.p = m.p_floor;
     }
     d[0] = (double)deltax;
     d[1] = (double)deltay;
@@ -261,9 +255,9 @@ __device__ __forceinline__ void kappa_push(const DevParams& prm, float dt_min, f
 }
 
 // (2u - 1) sqrt(3) in FP32 from one Philox block: u = r / (2^32 - 1).
-__device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i, uint32_t tag_s,
+__device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i, uint32_t tag_s,
                                            uint32_t step, float r[4]) {
-    const U4 u = philox4x32_10(tag_i, tag_s, step, prm.stream, prm.key0, prm.key1);
+    const U4 u = philox4x32_10(tag_i, tag_s, step, m.stream, m.key0, m.key1);
     const float sc = (float)(2.0 * 1.7320508075688772 / 4294967295.0);
     const float of = -1.7320508075688772f;
     r[0] = (float)u.a * sc + of;
@@ -272,17 +266,15 @@ __device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i,
     r[3] = (float)u.d * sc + of;
 }
 
-template <int NDIM, bool TINTERP>
-__device__ __forceinline__ void gather_kappa_push(const DevParams& prm, const float* rec1,
-                                                  const float* rec2, double t0, double dtf,
-                                                  float dt_min, float dt_max, State& s,
-                                                  bool fixed_dt, const float r[4], double d[4]) {
+template <int NDIM, bool TINTERP, bool ACC>
+__device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
+                                                  const float r[4], double d[4]) {
     constexpr int NC = NDIM == 2 ? 15 : 21;
     float f[NC];
-    const Interp I = interp_params(prm, s.x, s.y, s.z);
-    const float rt = TINTERP ? (float)((s.t - t0) / dtf) : 0.0f;
-    gather<NDIM, TINTERP>(prm, rec1, rec2, I, rt, f);
-    kappa_push<NDIM>(prm, dt_min, dt_max, s, f, fixed_dt, r, d);
+    const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
+    const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
+    gather<NDIM, TINTERP>(m, I, rt, f);
+    kappa_push<NDIM, ACC>(m, s, f, fixed_dt, r, d);
 }
 
 }  // namespace f32

@@ -1,7 +1,9 @@
 // sp_phys_f64.h -- FP64 gather, diffusion tensor and Parker step in the
 // reference's operation order.  Included only by translation units built
 // with -ffp-contract=off, so every +, -, *, / is one IEEE-rounded operation,
-// as in the reference Fortran evaluated without contraction.
+// as in the reference Fortran evaluated without contraction.  Constants
+// precomputed on the host (MoverParams) are the first operation of the
+// reference expression they replace, so the rounding sequence is unchanged.
 #pragma once
 
 #include "sp_device.h"
@@ -9,31 +11,30 @@
 namespace sp {
 namespace f64 {
 
-// get_interp_paramters (particle_module.f90:642-674) on a uniform grid.
-// Returns the 0-based node index of the lower corner and the weights.
+// get_interp_paramters (particle_module.f90:642-674) on a uniform grid:
+// 0-based node index of the lower corner and the corner weights.
 struct Interp {
     int ix, iy, iz;
     double w[8];
 };
 
-__device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, double y,
-                                                double z) {
+template <int NDIM>
+__device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, double y, double z) {
     Interp I;
-    const double px = (x - prm.xmin) / prm.dx;
-    const double py = (y - prm.ymin) / prm.dy;
+    const double px = (x - m.xmin) / m.dx;
+    const double py = (y - m.ymin) / m.dy;
     const double fx = floor(px), fy = floor(py);
-    const double posx = fx + 1.0, posy = fy + 1.0;   // Fortran pos (exact integers)
-    const double rx = (px - posx) + 1.0;
-    const double ry = (py - posy) + 1.0;
+    const double rx = (px - (fx + 1.0)) + 1.0;  // px - pos + 1 with pos = floor(px) + 1
+    const double ry = (py - (fy + 1.0)) + 1.0;
     double rz = 0.0;
-    I.ix = clamp_cell(fx, prm.nxg - 4) + 2;
-    I.iy = clamp_cell(fy, prm.nyg - 4) + 2;
+    I.ix = clamp_cell(fx, m.nxg - 4) + 2;
+    I.iy = clamp_cell(fy, m.nyg - 4) + 2;
     I.iz = 0;
-    if (prm.ndim == 3) {
-        const double pz = (z - prm.zmin) / prm.dz;
+    if (NDIM == 3) {
+        const double pz = (z - m.zmin) / m.dz;
         const double fz = floor(pz);
         rz = (pz - (fz + 1.0)) + 1.0;
-        I.iz = clamp_cell(fz, prm.nzg - 4) + 2;
+        I.iz = clamp_cell(fz, m.nzg - 4) + 2;
     }
     const double rx1 = 1.0 - rx, ry1 = 1.0 - ry, rz1 = 1.0 - rz;
     I.w[0] = rx1 * ry1 * rz1;
@@ -51,8 +52,7 @@ __device__ __forceinline__ Interp interp_params(const DevParams& prm, double x, 
 // fields = 0; for corners (k, j, i; i fastest) fields += f_node * w; then the
 // optional (1-rt, rt) blend of the two frames.
 template <int NDIM, bool TINTERP>
-__device__ __forceinline__ void gather(const DevParams& prm, const float* __restrict__ rec1,
-                                       const float* __restrict__ rec2, const Interp& I, double rt,
+__device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, double rt,
                                        double* __restrict__ f) {
     constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
     constexpr int NC = NDIM == 2 ? 15 : 21;
@@ -63,13 +63,13 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
         f[c] = 0.0;
         if (TINTERP) g[c] = 0.0;
     }
-    const long long sx = 1, sy = prm.nxg, sz = (long long)prm.nxg * prm.nyg;
-    const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
+    const long long sy = m.nxg, sz = (long long)m.nxg * m.nyg;
+    const long long base = (long long)I.ix + (long long)I.iy * sy + (long long)I.iz * sz;
 #pragma unroll
     for (int q = 0; q < NCORN; ++q) {
         const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-        const long long node = base + i * sx + j * sy + k * sz;
-        const float4* r4 = reinterpret_cast<const float4*>(rec1 + node * W);
+        const long long node = base + i + j * sy + k * sz;
+        const float4* r4 = reinterpret_cast<const float4*>(m.rec1 + node * W);
         float v[W];
 #pragma unroll
         for (int u = 0; u < W / 4; ++u) {
@@ -80,7 +80,7 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
 #pragma unroll
         for (int c = 0; c < NC; ++c) f[c] = f[c] + (double)v[c] * wq;
         if (TINTERP) {
-            const float4* s4 = reinterpret_cast<const float4*>(rec2 + node * W);
+            const float4* s4 = reinterpret_cast<const float4*>(m.rec2 + node * W);
 #pragma unroll
             for (int u = 0; u < W / 4; ++u) {
                 const float4 t = s4[u];
@@ -98,14 +98,15 @@ __device__ __forceinline__ void gather(const DevParams& prm, const float* __rest
 
 struct Kappa {
     double kpara, kperp, skpara, skperp, skpara_perp;
-    double kxx, kyy, kzz, kxy, kxz, kyz;
     double dkxx_dx, dkyy_dy, dkzz_dz, dkxy_dx, dkxy_dy, dkxz_dx, dkxz_dz, dkyz_dy, dkyz_dz;
 };
 
 // calc_spatial_diffusion_coefficients (particle_module.f90:2208-2450),
 // Cartesian Parker transport without delta-B / correlation-length maps.
+// Only the quantities the pusher reads are kept (the diagonal/off-diagonal
+// tensor entries feed the spherical and 3D-FT branches only).
 template <int NDIM>
-__device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const double* f) {
+__device__ __forceinline__ Kappa kappa(const MoverParams& m, double p, const double* f) {
     Kappa k;
     const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
     const double by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
@@ -115,13 +116,14 @@ __device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const dou
     if (b < __DBL_EPSILON__) ib1 = 1.0; else ib1 = 1.0 / b;
     const double ib2 = ib1 * ib1;
     const double ib3 = ib1 * ib2;
+    const bool mag = (m.flags & MF_MAG_DEP) != 0;
     double knorm_para = 1.0;
-    if (prm.mag_dependency == 1) knorm_para = knorm_para * pow(b, prm.gamma_turb - 2.0);
+    if (mag) knorm_para = knorm_para * pow(b, m.gamma_m2);
     double knorm;
-    if (prm.momentum_dependency == 1) knorm = knorm_para * pow(p / prm.p0, prm.pindex);
+    if (m.flags & MF_MOMENTUM_DEP) knorm = knorm_para * pow(p / m.p0, m.pindex);
     else knorm = knorm_para;
-    k.kpara = prm.kpara0 * knorm;
-    k.kperp = k.kpara * prm.kret;
+    k.kpara = m.kpara0 * knorm;
+    k.kperp = k.kpara * m.kret;
     k.skpara = sqrt(2.0 * k.kpara);
     k.skperp = sqrt(2.0 * k.kperp);
     k.skpara_perp = sqrt(2.0 * (k.kpara - k.kperp));
@@ -132,9 +134,9 @@ __device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const dou
         const double dby_dx = f[R2_DBYDX], dby_dy = f[R2_DBYDY];
         const double db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
         double dkdx = 0.0, dkdy = 0.0;
-        if (prm.mag_dependency == 1) {
-            dkdx = db_dx * ib1 * (prm.gamma_turb - 2.0);
-            dkdy = db_dy * ib1 * (prm.gamma_turb - 2.0);
+        if (mag) {
+            dkdx = db_dx * ib1 * m.gamma_m2;
+            dkdy = db_dy * ib1 * m.gamma_m2;
         }
         k.dkxx_dx = kperp * dkdx + kpp * dkdx * (bx * bx) * ib2 +
                     2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
@@ -144,10 +146,6 @@ __device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const dou
                     kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
         k.dkxy_dy = kpp * dkdy * bx * by * ib2 +
                     kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
-        k.kxx = kperp + kpp * bx * bx * ib2;
-        k.kyy = kperp + kpp * by * by * ib2;
-        k.kxy = kpp * bx * by * ib2;
-        k.kzz = k.kxz = k.kyz = 0.0;
         k.dkzz_dz = k.dkxz_dx = k.dkxz_dz = k.dkyz_dy = k.dkyz_dz = 0.0;
     } else {
         const double dbx_dx = f[R3_DBXDX], dbx_dy = f[R3_DBXDY], dbx_dz = f[R3_DBXDZ];
@@ -155,10 +153,10 @@ __device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const dou
         const double dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
         const double db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
         double dkdx = 0.0, dkdy = 0.0, dkdz = 0.0;
-        if (prm.mag_dependency == 1) {  // no 1/B in the 3D branch (particle_module.f90:2405-2409)
-            dkdx = db_dx * (prm.gamma_turb - 2.0);
-            dkdy = db_dy * (prm.gamma_turb - 2.0);
-            dkdz = db_dz * (prm.gamma_turb - 2.0);
+        if (mag) {  // no 1/B in the 3D branch (particle_module.f90:2405-2409)
+            dkdx = db_dx * m.gamma_m2;
+            dkdy = db_dy * m.gamma_m2;
+            dkdz = db_dz * m.gamma_m2;
         }
         k.dkxx_dx = kperp * dkdx + kpp * dkdx * (bx * bx) * ib2 +
                     2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
@@ -178,31 +176,23 @@ __device__ __forceinline__ Kappa kappa(const DevParams& prm, double p, const dou
                     kpp * ((dby_dy * bz + by * dbz_dy) * ib2 - 2.0 * by * bz * db_dy * ib3);
         k.dkyz_dz = kpp * dkdz * by * bz * ib2 +
                     kpp * ((dby_dz * bz + by * dbz_dz) * ib2 - 2.0 * by * bz * db_dz * ib3);
-        k.kxx = kperp + kpp * bx * bx * ib2;
-        k.kyy = kperp + kpp * by * by * ib2;
-        k.kzz = kperp + kpp * bz * bz * ib2;
-        k.kxy = kpp * bx * by * ib2;
-        k.kxz = kpp * bx * bz * ib2;
-        k.kyz = kpp * by * bz * ib2;
     }
     return k;
 }
 
-__device__ __forceinline__ bool in_acc_region(const DevParams& prm, double x, double y,
-                                              double z) {
-    const double xn = (x - prm.xmin) / prm.lx;
-    bool in = (xn >= prm.acc_region[0]) && (xn <= prm.acc_region[1]);
-    if (prm.ndim > 1) {
-        const double yn = (y - prm.ymin) / prm.ly;
-        in = in && (yn >= prm.acc_region[2]) && (yn <= prm.acc_region[3]);
-    }
-    if (prm.ndim == 3) {
-        const double zn = (z - prm.zmin) / prm.lz;
-        in = in && (zn >= prm.acc_region[4]) && (zn <= prm.acc_region[5]);
+// particle_in_acceleration_region (particle_module.f90:2885-2906)
+template <int NDIM>
+__device__ __forceinline__ bool in_acc_region(const MoverParams& m, double x, double y, double z) {
+    const double xn = (x - m.xmin) / m.lx;
+    const double yn = (y - m.ymin) / m.ly;
+    bool in = (xn >= m.acc_region[0]) && (xn <= m.acc_region[1]) && (yn >= m.acc_region[2]) &&
+              (yn <= m.acc_region[3]);
+    if (NDIM == 3) {
+        const double zn = (z - m.zmin) / m.lz;
+        in = in && (zn >= m.acc_region[4]) && (zn <= m.acc_region[5]);
     }
     return in;
 }
-
 
 __device__ __forceinline__ double fmin_seq(double a, double b) { return b < a ? b : a; }
 
@@ -210,10 +200,10 @@ __device__ __forceinline__ double fmin_seq(double a, double b) { return b < a ? 
 // (:4625-4907), Cartesian, uniform grid, dpp terms off.  r = (ran1, ran2,
 // ran3, ran_p).  d receives (deltax, deltay, deltaz, deltap); the 2D pusher
 // leaves deltaz at zero like the reference (its z drift is not undone).
-template <int NDIM>
-__device__ __forceinline__ void push(const DevParams& prm, double dt_min, double dt_max,
-                                     State& s, const double* f, const Kappa& k, bool fixed_dt,
-                                     const double r[4], double d[4]) {
+template <int NDIM, bool ACC>
+__device__ __forceinline__ void push(const MoverParams& m, State& s, const double* f,
+                                     const Kappa& k, bool fixed_dt, const double r[4],
+                                     double d[4]) {
     const double vx = NDIM == 2 ? f[R2_VX] : f[R3_VX];
     const double vy = NDIM == 2 ? f[R2_VY] : f[R3_VY];
     const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
@@ -224,11 +214,10 @@ __device__ __forceinline__ void push(const DevParams& prm, double dt_min, double
     if (b < __DBL_EPSILON__) ib = 0.0; else ib = 1.0 / b;
     const double ib2 = ib * ib;
     const double ib3 = ib * ib2;
-    // vdp = 1.0/(3*pcharge) / sqrt(...); the first quotient is FP32 in the Fortran
-    const float pre = 1.0f / (float)(3 * prm.pcharge);
-    const double t1 = prm.drift1 * prm.p0 / s.p;
-    const double t2 = prm.drift2 * (prm.p0 * prm.p0) / (s.p * s.p);
-    const double vdp = (double)pre / sqrt(t1 * t1 + t2 * t2);
+    // vdp = 1.0/(3*pcharge) / sqrt((drift1*p0/p)**2 + (drift2*p0**2/p**2)**2)
+    const double t1 = m.d1p0 / s.p;
+    const double t2 = m.d2p02 / (s.p * s.p);
+    const double vdp = m.vdp_pre / sqrt(t1 * t1 + t2 * t2);
 
     double dx_dt, dy_dt, dz_dt, divv;
     double bxn = 0.0, byn = 0.0, bzn = 0.0, bxyn = 0.0, ibxyn = 0.0;
@@ -238,7 +227,7 @@ __device__ __forceinline__ void push(const DevParams& prm, double dt_min, double
         const double vdx = vdp * (dbz_dy * ib2 - 2.0 * bz * db_dy * ib3);
         const double vdy = vdp * (-(dbz_dx * ib2) + 2.0 * bz * db_dx * ib3);
         double vdz = 0.0;
-        if (prm.check_drift_2d) {
+        if (m.flags & MF_DRIFT_2D) {
             const double dbx_dy = f[R2_DBXDY], dby_dx = f[R2_DBYDX];
             vdz = vdp * ((dby_dx - dbx_dy) * ib2 - 2.0 * (by * db_dx - bx * db_dy) * ib3);
         }
@@ -266,22 +255,23 @@ __device__ __forceinline__ void push(const DevParams& prm, double dt_min, double
     const double dp_dt = -(s.p * divv / 3.0);
 
     if (!fixed_dt) {
+        // the zero test omits dz_dt in 3D (particle_module.f90:4792-4794)
         if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
             const double sk = k.skperp > 0.0 ? k.skperp : k.skpara;
-            double a = 0.5 * prm.dx / k.skpara;
-            double m = a * a;
-            a = 0.5 * prm.dy / k.skpara; m = fmin_seq(m, a * a);
-            if (NDIM == 3) { a = 0.5 * prm.dz / k.skpara; m = fmin_seq(m, a * a); }
-            a = sk / dx_dt; m = fmin_seq(m, a * a);
-            a = sk / dy_dt; m = fmin_seq(m, a * a);
-            if (NDIM == 3) { a = sk / dz_dt; m = fmin_seq(m, a * a); }
-            m = fmin_seq(m, (double)0.1f * s.p / fabs(dp_dt));
-            s.dt = m;
+            double a = m.half_dx / k.skpara;
+            double mm = a * a;
+            a = m.half_dy / k.skpara; mm = fmin_seq(mm, a * a);
+            if (NDIM == 3) { a = m.half_dz / k.skpara; mm = fmin_seq(mm, a * a); }
+            a = sk / dx_dt; mm = fmin_seq(mm, a * a);
+            a = sk / dy_dt; mm = fmin_seq(mm, a * a);
+            if (NDIM == 3) { a = sk / dz_dt; mm = fmin_seq(mm, a * a); }
+            mm = fmin_seq(mm, (double)0.1f * s.p / fabs(dp_dt));  // 0.1 is an FP32 literal
+            s.dt = mm;
         } else {
-            s.dt = dt_min;
+            s.dt = m.dt_min;
         }
-        if (s.dt < dt_min) s.dt = dt_min;
-        if (s.dt > dt_max) s.dt = dt_max;
+        if (s.dt < m.dt_min) s.dt = m.dt_min;
+        if (s.dt > m.dt_max) s.dt = m.dt_max;
     }
 
     const double sdt = sqrt(s.dt);
@@ -303,16 +293,16 @@ __device__ __forceinline__ void push(const DevParams& prm, double dt_min, double
     s.t = s.t + s.dt;
     const double dpp = 0.0;
     double dp = dp_dt * s.dt + r[3] * sqrt(2.0 * dpp) * sdt;
-    if (prm.acc_region_flag == 1) {
-        if (in_acc_region(prm, s.x, s.y, s.z)) s.p = s.p + dp;
+    if (ACC) {
+        if (in_acc_region<NDIM>(m, s.x, s.y, s.z)) s.p = s.p + dp;
         else dp = 0.0;
     } else {
         s.p = s.p + dp;
     }
-    if (s.p < 0.25 * prm.p0) {
+    if (s.p < m.p_floor) {
         s.p = s.p - dp;
-        dp = 0.25 * prm.p0 - s.p;
-        s.p = 0.25 * prm.p0;
+        dp = m.p_floor - s.p;
+        s.p = m.p_floor;
     }
     d[0] = deltax;
     d[1] = deltay;
@@ -322,9 +312,9 @@ __device__ __forceinline__ void push(const DevParams& prm, double dt_min, double
 
 // Steps' increments from one Philox block: (2u - 1) * sqrt(3) for each
 // of the four words (particle_module.f90:3548-3550, :3589).
-__device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i, uint32_t tag_s,
+__device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i, uint32_t tag_s,
                                            uint32_t step, double r[4]) {
-    const U4 u = philox4x32_10(tag_i, tag_s, step, prm.stream, prm.key0, prm.key1);
+    const U4 u = philox4x32_10(tag_i, tag_s, step, m.stream, m.key0, m.key1);
     const double sqrt3 = sqrt(3.0);
     r[0] = (2.0 * u01(u.a) - 1.0) * sqrt3;
     r[1] = (2.0 * u01(u.b) - 1.0) * sqrt3;
@@ -333,18 +323,16 @@ __device__ __forceinline__ void step_rands(const DevParams& prm, uint32_t tag_i,
 }
 
 // The full per-step chain: interpolation parameters, gather, kappa, push.
-template <int NDIM, bool TINTERP>
-__device__ __forceinline__ void gather_kappa_push(const DevParams& prm, const float* rec1,
-                                                  const float* rec2, double t0, double dtf,
-                                                  double dt_min, double dt_max, State& s,
-                                                  bool fixed_dt, const double r[4], double d[4]) {
+template <int NDIM, bool TINTERP, bool ACC>
+__device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
+                                                  const double r[4], double d[4]) {
     constexpr int NC = NDIM == 2 ? 15 : 21;
     double f[NC];
-    const Interp I = interp_params(prm, s.x, s.y, s.z);
-    const double rt = (s.t - t0) / dtf;
-    gather<NDIM, TINTERP>(prm, rec1, rec2, I, rt, f);
-    const Kappa k = kappa<NDIM>(prm, s.p, f);
-    push<NDIM>(prm, dt_min, dt_max, s, f, k, fixed_dt, r, d);
+    const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
+    const double rt = TINTERP ? (s.t - m.t0) / m.dtf : 0.0;
+    gather<NDIM, TINTERP>(m, I, rt, f);
+    const Kappa k = kappa<NDIM>(m, s.p, f);
+    push<NDIM, ACC>(m, s, f, k, fixed_dt, r, d);
 }
 
 }  // namespace f64

@@ -47,7 +47,7 @@ class SpConfig(C.Structure):
         ("check_drift_2d", C.c_int32), ("acc_region_flag", C.c_int32),
         ("acc_region", C.c_double * 6),
         ("time_interp", C.c_int32), ("precision", C.c_int32),
-        ("seed", C.c_uint64), ("stream", C.c_uint32), ("reserved0", C.c_int32),
+        ("seed", C.c_uint64), ("stream", C.c_uint32), ("sort_every", C.c_int32),
         ("nptl_max", C.c_int64),
     ]
 
@@ -110,11 +110,13 @@ EXPORTS = {
     "sp_particle_mover": [_P, C.c_double, C.c_double, C.c_int32, C.c_int32,
                           C.POINTER(SpMoverStats)],
     "sp_split_particle": [_P, C.c_double, C.c_double],
+    "sp_sort_particles": [_P],
     "sp_step_once": [_P, C.c_int64, C.c_double, C.c_double, _D, _D, C.POINTER(C.c_int8), _D],
     "sp_distributions_init": [_P, C.POINTER(SpDistConfig)],
     "sp_distributions": [_P, _D, _D, _D, _D, _D],
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],
     "sp_quick_check": [_P, _D],
+    "sp_calibration_copy": [_P, C.c_int64, _D],
 }
 
 _lib = None

@@ -35,7 +35,7 @@ CONFIG_DEFAULTS = dict(
     p0=1.0, pmin=1e-2, pmax=1e4, b0=1.0, gamma_turb=5.0 / 3.0, kpara0=1e-3, kret=0.01,
     dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
     acc_region_flag=0, acc_region=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), time_interp=0,
-    precision=_abi.SP_PREC_F64, seed=20261202, stream=0, nptl_max=1000,
+    precision=_abi.SP_PREC_F64, seed=20261202, stream=0, sort_every=1, nptl_max=1000,
 )
 
 
@@ -161,6 +161,9 @@ class ParkerPusher:
     def split_particle(self, split_ratio, pmin_split):
         self._call("sp_split_particle", float(split_ratio), float(pmin_split))
 
+    def sort_particles(self):
+        self._call("sp_sort_particles")
+
     def step_once(self, state: np.ndarray, rands: np.ndarray, fixed: np.ndarray, t0: float,
                   dtf: float):
         st = np.ascontiguousarray(state, dtype=np.float64).copy()
@@ -213,6 +216,11 @@ class ParkerPusher:
         for a, d in zip(outs, self._dist_dims):
             shaped.append(None if a is None else a.reshape(d[4], d[3], d[2], d[1], d[0]))
         return fg, shaped
+
+    def calibration_copy(self, nbytes: int) -> float:
+        ms = C.c_double()
+        self._call("sp_calibration_copy", int(nbytes), C.byref(ms))
+        return ms.value
 
     def quick_check(self) -> dict:
         out = np.zeros(9, dtype=np.float64)

@@ -250,3 +250,33 @@ def test_empty_population_and_state_errors():
     q = pz.quick_check()
     assert q["nptl_current"] == 0 and q["pdt_min"] == 1.0 and q["pdt_max"] == 0.0
     pz.close()
+
+
+def _morton2(bx, by):
+    k = np.zeros_like(bx, dtype=np.uint64)
+    for b in range(16):
+        k |= ((bx >> b) & 1).astype(np.uint64) << np.uint64(2 * b)
+        k |= ((by >> b) & 1).astype(np.uint64) << np.uint64(2 * b + 1)
+    return k
+
+
+def test_spatial_sort_is_a_stable_permutation():
+    c = cfg2d(128)
+    pz, _ = make(c, nodes_for(c), nptl_max=30000)
+    a = random_states(c, 25000, seed=50)
+    pz.set_particles(oracle.soa_from_particles(a))
+    pz.sort_particles()
+    g = pz.get_particles()
+    gs = by_id(g)
+    rs = by_id(oracle.soa_from_particles(a))
+    for key in rs:
+        np.testing.assert_array_equal(gs[key], rs[key], err_msg=key)
+    # non-decreasing Morton key of 2-cell blocks (128 cells -> at most 64 blocks per axis)
+    ix = np.clip(np.floor((g["x"] - c["xmin"]) / c["dx"]), 0, c["nx"] - 1).astype(np.int64) >> 1
+    iy = np.clip(np.floor((g["y"] - c["ymin"]) / c["dy"]), 0, c["ny"] - 1).astype(np.int64) >> 1
+    k = _morton2(ix, iy)
+    assert np.all(np.diff(k.astype(np.int64)) >= 0)
+    # stable: equal keys keep their previous (tag) order
+    same = np.diff(k.astype(np.int64)) == 0
+    assert np.all(np.diff(g["tag_injected"])[same] > 0)
+    pz.close()
