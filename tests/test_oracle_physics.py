@@ -287,3 +287,23 @@ def test_pure_diffusion_variance():
     assert np.all(out["p"] == 1.0)
     # every particle took dtf/dt steps (one of them possibly the redo)
     assert acc.steps == pytest.approx(n * 1000, rel=1e-3)
+
+
+def test_rng_modes_agree_in_distribution():
+    # the oracle's two generators (Philox keyed per particle; per-thread
+    # MT19937 as the reference draws) push one population through the 2D
+    # snapshot; final p, x, y agree in distribution (two-sample KS)
+    from scipy import stats
+    from sp_hip import workloads
+    n_grid, npart = 128, 20000
+    c = cfg2d(n_grid, pbcx=0, pbcy=1, dt_max_rel=2e-3)
+    prm = oracle.params(c)
+    fa = oracle.gradients(prm, workloads.fields_2d(n_grid, n_grid))
+    dtf = workloads.interval_2d(n_grid, 400)
+    a, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), npart, npart, 1e-7, 1, 1.0,
+                         0.0, dtf, (0, 0, 0, 1, 1, 0), 7.0)
+    mt, _ = oracle.mover(prm, fa, a, 0.0, dtf, rng_mode=1, nthreads=8)
+    ph, _ = oracle.mover(prm, fa, a, 0.0, dtf, rng_mode=0, nthreads=8)
+    for key in ("p", "x", "y"):
+        assert stats.ks_2samp(mt[key], ph[key]).pvalue > 1e-3, key
+    assert np.log(mt["p"]).std() > 1e-3
