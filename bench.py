@@ -1,10 +1,14 @@
 """Throughput of the Parker pusher on one MI355X.
 
-Workload (default): 2D snapshot on a 2048 x 2048 grid (FP32 fields), 1e8
-pseudo-particles injected uniformly with p = p0, MHD interval sized for
-2000 diffusion-limited steps per particle.  One bench "step" is one MHD
-interval of the hot path: the mover over every particle plus the f(p) and
-spatial-map histograms.  Inputs are resident in HBM before timing starts.
+Workloads:
+  2d (default)  2D snapshot on a 2048 x 2048 grid (FP32 fields), 1e8
+                pseudo-particles injected uniformly with p = p0, MHD interval
+                sized for 2000 diffusion-limited steps per particle.
+  3d            3D turbulence snapshot on a 512^3 grid with a compression
+                slab, 1e8 particles, 1000 diffusion-limited steps per interval.
+One bench "step" is one MHD interval of the hot path: the mover over every
+particle plus the f(p) and spatial-map histograms.  Inputs are resident in
+HBM before timing starts.
 
 Printed JSON line (one):
   value          particle-steps/s over the K timed intervals (FP32 gather path)
@@ -32,17 +36,25 @@ from sp_hip import ParkerPusher, workloads  # noqa: E402
 
 METRIC = "particle-steps/s (2D/3D snapshot, 1e8 ptcl) at f(p) L1 err ≤1% vs FP64 SDE"
 HBM_PEAK_GBS = 8000.0
-N_ARR_2D = 15          # node components the 2D step consumes
-N_CORNER_2D = 4
+# node components the step consumes x corners of the interpolation cell
+BYTES_PER_STEP = {2: 4 * 15 * 4, 3: 8 * 21 * 4}
 COPY_BYTES = 1 << 30
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mover_2d_2048_1e8_traffic.json")
+WORKLOADS = {
+    "2d": dict(grid=2048, nsteps=2000, cpu_nptl=1e6),
+    "3d": dict(grid=512, nsteps=1000, cpu_nptl=2e5),
+}
 
 
-def measured_traffic(workload: dict):
+def traffic_file(kind, grid, nptl):
+    return os.path.join(ROOT, "profiles", f"mover_{kind}_{grid}_{nptl:.0e}_traffic.json"
+                        .replace("+0", ""))
+
+
+def measured_traffic(path, workload: dict):
     """HBM bytes per mover launch from the committed rocprofv3 counter runs,
     when they were taken on this workload; None otherwise."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -55,8 +67,11 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+
+
 def run_intervals(pz, nptl, dtf, warmup, steps, inject=True):
-    box = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+    box = BOX
     if inject:
         pz.inject_particles_spatial_uniform(nptl, 1e-7, 1, 5.0, 0.0, dtf, box)
     for w in range(warmup):
@@ -109,7 +124,7 @@ def cpu_baseline(cfg, nodes, n, dtf, nthreads):
 
     prm = oracle.params(dict(cfg, seed=cfg.get("seed", 20261202)))
     fa = oracle.gradients(prm, nodes)
-    box = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+    box = BOX
     ptl, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), n, n, 1e-7, 1, 5.0, 0.0, dtf,
                            box, 7.0)
     # start every particle at the interval start so the sample does full intervals
@@ -125,12 +140,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed MHD intervals")
     ap.add_argument("--warmup", type=int, default=1, help="untimed MHD intervals")
-    ap.add_argument("--grid", type=int, default=2048)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d")
+    ap.add_argument("--grid", type=int, default=0, help="cells per axis (default per workload)")
     ap.add_argument("--nptl", type=float, default=1e8)
-    ap.add_argument("--nsteps", type=int, default=2000, help="diffusion-limited steps per interval")
+    ap.add_argument("--nsteps", type=int, default=0,
+                    help="diffusion-limited steps per interval (default per workload)")
     ap.add_argument("--precision", choices=("f32", "f64"), default="f32")
     ap.add_argument("--accuracy-nptl", type=float, default=1e7)
-    ap.add_argument("--cpu-nptl", type=float, default=3e5)
+    ap.add_argument("--cpu-nptl", type=float, default=0, help="CPU sample (default per workload)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -141,12 +158,24 @@ def main():
         sys.exit(2)
 
     n = int(a.nptl)
-    g = a.grid
-    cfg = workloads.config_2d(g)
-    dist = workloads.dist_config_2d(g)
-    dtf = workloads.interval_2d(g, a.nsteps)
-    log(f"generating {g}x{g} fields")
-    nodes = workloads.fields_2d(g, g)
+    wl = WORKLOADS[a.workload]
+    g = a.grid or wl["grid"]
+    nsteps = a.nsteps or wl["nsteps"]
+    ndim = 3 if a.workload == "3d" else 2
+    t0 = time.perf_counter()
+    if ndim == 2:
+        cfg = workloads.config_2d(g)
+        dist = workloads.dist_config_2d(g)
+        dtf = workloads.interval_2d(g, nsteps)
+        nodes = workloads.fields_2d(g, g)
+        desc = f"2D snapshot {g}x{g} FP32 fields"
+    else:
+        cfg = workloads.config_3d(g)
+        dist = workloads.dist_config_3d(g)
+        dtf = workloads.interval_3d(g, nsteps)
+        nodes = workloads.fields_3d(g)
+        desc = f"3D turbulence snapshot {g}^3 FP32 fields with a compression slab"
+    log(f"generated {desc} in {time.perf_counter() - t0:.1f} s")
     with ParkerPusher(**dict(cfg, precision=a.precision, nptl_max=n)) as pz:
         pz.upload_fields(nodes)
         pz.calc_fields_gradients()
@@ -158,7 +187,7 @@ def main():
         log(f"injecting {n} particles, dtf={dtf:.5g}")
         r = run_intervals(pz, n, dtf, a.warmup, a.steps)
     value = r["steps"] / r["seconds"]
-    bytes_per_step = N_CORNER_2D * N_ARR_2D * 4
+    bytes_per_step = BYTES_PER_STEP[ndim]
     achieved = bytes_per_step * r["steps"] / (r["kernel_ms"] * 1e-3) / 1e9
     result = {
         "metric": METRIC,
@@ -174,17 +203,18 @@ def main():
         "dtype": "f32" if a.precision == "f32" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"2D snapshot {g}x{g} FP32 fields, {n:.0e} particles, "
-                        f"{a.nsteps} diffusion-limited steps per MHD interval",
-            "grid": [g, g], "nptl": n, "precision": a.precision,
+            "workload": f"{desc}, {n:.0e} particles, "
+                        f"{nsteps} diffusion-limited steps per MHD interval",
+            "grid": [g] * ndim, "nptl": n, "precision": a.precision,
             "position_time_dtype": "f64", "particle_steps_timed": r["steps"],
             "mover_kernel_ms": r["kernel_ms"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dict(grid=g, nptl=n, precision=a.precision,
-                                             nsteps=a.nsteps)),
+            "traffic": measured_traffic(traffic_file(a.workload, g, n),
+                                        dict(grid=g, nptl=n, precision=a.precision,
+                                             nsteps=nsteps)),
             "kernel": "mover_kernel", "bytes_per_step": bytes_per_step,
         },
     }
@@ -198,12 +228,13 @@ def main():
                                   f32_steps_per_s=acc["f32_steps_per_s"])
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, os.cpu_count() or 1)
-        nc = int(a.cpu_nptl)
+        nc = int(a.cpu_nptl or wl["cpu_nptl"])
         log(f"cpu baseline: {nc} particles, {nt} threads")
         v, s, secs = cpu_baseline(cfg, nodes, nc, dtf, nt)
         result["cpu_baseline"] = {
             "value": v, "unit": "particle-steps/s", "cores": nt, "kind": "port",
-            "sample": f"{nc} particles x one full {g}x{g} interval ({s} steps, {secs:.1f} s), "
+            "sample": f"{nc} particles x one full interval of the same snapshot "
+                      f"({s} steps, {secs:.1f} s), "
                       "FP64 oracle, AoS particles, 32-component node records, MT19937 per thread",
         }
     print(json.dumps(result), flush=True)

@@ -108,3 +108,121 @@ def dist_config_2d(n: int, map_bins: int = 128, npbins1: int = 16) -> dict:
         local=[dict(enabled=1, npbins=npbins1, nmu=1, rx=r, ry=r, rz=1, pmin=1e-2, pmax=1e4),
                None, None, None],
     )
+
+
+# ---------------------------------------------------------------------------
+# 3D turbulence snapshot (512^3 configuration)
+#   B = B0 x + dB, dB solenoidal with E(k) ~ k^-5/3 on shells 2 <= |k| <= 64,
+#   rms |dB| = 0.5 B0;  V = compressive (curl-free) turbulence of the same
+#   spectrum with rms |V| = 0.05, plus a smooth step of Vx from U1 to U1/3
+#   across a central slab (divV < 0 there: a compression of ratio 3).  The
+#   turbulence is periodic on the unit cube; the Vx step is evaluated at every
+#   node, ghosts included (the box is open along x for particles).
+SLAB_U1 = 0.15
+SLAB_RATIO = 3.0
+SLAB_WIDTH = 0.02
+
+
+def _turbulence(n, rng, kmin, kmax, solenoidal, workers):
+    """Three real components on an n^3 periodic grid (index order z, y, x)
+    whose Fourier amplitudes follow |a_k|^2 ~ k^-11/3 (E(k) ~ k^-5/3) on
+    kmin <= |k| <= kmax, projected on the plane normal to k (solenoidal) or
+    on k (compressive).  Returned unscaled, FP64."""
+    from scipy import fft as sfft
+
+    kk = np.fft.fftfreq(n, 1.0 / n)
+    kr = np.fft.rfftfreq(n, 1.0 / n)
+    # only the band holds modes: draw them sparse, then scatter
+    lim = int(kmax)
+    ax = np.concatenate([np.arange(0, lim + 1), np.arange(-lim, 0)])
+    kz, ky, kx = np.meshgrid(ax, ax, np.arange(0, lim + 1), indexing="ij")
+    kabs = np.sqrt(kx * kx + ky * ky + kz * kz)
+    sel = (kabs >= kmin) & (kabs <= kmax)
+    kx, ky, kz, kabs = kx[sel], ky[sel], kz[sel], kabs[sel]
+    m = kabs.size
+    xi = rng.standard_normal((3, m)) + 1j * rng.standard_normal((3, m))
+    khat = np.stack([kx, ky, kz]) / kabs
+    kdotxi = (khat * xi).sum(axis=0)
+    if solenoidal:
+        vec = xi - khat * kdotxi
+    else:
+        vec = khat * kdotxi
+    vec *= kabs ** (-11.0 / 6.0)
+    iz = np.where(kz >= 0, kz, kz + n)
+    iy = np.where(ky >= 0, ky, ky + n)
+    out = []
+    for c in range(3):
+        spec = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+        spec[iz, iy, kx] = vec[c]
+        out.append(sfft.irfftn(spec, s=(n, n, n), workers=workers))
+        del spec
+    del kk, kr
+    return out
+
+
+def _ghosted(a, n):
+    """Periodic (n+4)^3 copy of an n^3 array (node i <-> grid index i - 2)."""
+    idx = (np.arange(n + 4) - 2) % n
+    return a[np.ix_(idx, idx, idx)]
+
+
+def fields_3d(n: int, seed: int = FIELD_SEED, kmin: float = 2.0, kmax: float = 64.0,
+              db_rms: float = 0.5, v_rms: float = 0.05, workers: int = -1) -> np.ndarray:
+    """(n+4, n+4, n+4, 8) FP32 node fields of the 3D snapshot on [0,1]^3."""
+    rng = np.random.default_rng(seed + 3)
+    kmax = min(kmax, n // 2 - 1)
+    out = np.empty((n + 4, n + 4, n + 4, 8), dtype=np.float32)
+    db = _turbulence(n, rng, kmin, kmax, True, workers)
+    s = db_rms / np.sqrt(sum(np.mean(c * c) for c in db))
+    btot2 = np.zeros((n + 4,) * 3, dtype=np.float64)
+    for c in range(3):
+        g = _ghosted(db[c], n) * s
+        if c == 0:
+            g += 1.0
+        out[..., 4 + c] = g
+        btot2 += g * g
+        db[c] = None
+    out[..., 7] = np.sqrt(btot2)
+    del btot2
+    v = _turbulence(n, rng, kmin, kmax, False, workers)
+    s = v_rms / np.sqrt(sum(np.mean(c * c) for c in v))
+    x = node_coords(n, 1.0 / n, 0.0)
+    step = SLAB_U1 - 0.5 * SLAB_U1 * (1.0 - 1.0 / SLAB_RATIO) * (1.0 + np.tanh((x - 0.5) / SLAB_WIDTH))
+    for c in range(3):
+        g = _ghosted(v[c], n) * s
+        if c == 0:
+            g += step[None, None, :]
+        out[..., c] = g
+        v[c] = None
+    out[..., 3] = 1.0
+    return out
+
+
+def config_3d(n: int, **over) -> dict:
+    """sp_config keys of the 3D snapshot on an n^3 grid over [0,1]^3: open
+    along x (the compression slab's normal), periodic along y and z."""
+    c = dict(
+        ndim=3, nx=n, ny=n, nz=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n,
+        xmin=0.0, ymin=0.0, zmin=0.0, xmax=1.0, ymax=1.0, zmax=1.0, lx=1.0, ly=1.0, lz=1.0,
+        pbcx=1, pbcy=0, pbcz=0, momentum_dependency=1, mag_dependency=0, charge=-1,
+        p0=1.0, pmin=1e-2, pmax=1e4, b0=1.0, gamma_turb=5.0 / 3.0, kpara0=1e-3, kret=0.01,
+        dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
+        acc_region_flag=0, time_interp=0,
+    )
+    c.update(over)
+    return c
+
+
+def interval_3d(n: int, nsteps: int, kpara0: float = 1e-3) -> float:
+    """dtf holding `nsteps` diffusion-limited steps at p0 on an n^3 grid."""
+    return interval_2d(n, nsteps, kpara0)
+
+
+def dist_config_3d(n: int, map_bins: int = 64, npbins1: int = 16) -> dict:
+    """Global f(p) as in 2D; local set 1: a map_bins^3 spatial map."""
+    r = max(1, n // map_bins)
+    return dict(
+        npp_global=256, nmu_global=1, local_dist=1,
+        local=[dict(enabled=1, npbins=npbins1, nmu=1, rx=r, ry=r, rz=r, pmin=1e-2, pmax=1e4),
+               None, None, None],
+    )
