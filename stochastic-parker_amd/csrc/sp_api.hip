@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -34,7 +35,10 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr double kInvFx = 1.0 / sp::kFxScale;
-constexpr int kMoverBlocksPerCU = 2;
+// Resident 256-lane mover blocks per CU: 2D records (64 B) of a die's window
+// fit its L2 with two blocks, 3D records (96 B, 8 corners) with one.
+constexpr int kMoverBlocksPerCU2D = 2;
+constexpr int kMoverBlocksPerCU3D = 1;
 
 struct Ctx {
     sp_config cfg{};
@@ -171,11 +175,13 @@ Ctx* live_ctx(void* p, int& rc) {
 // Occupancy-sized persistent grid for the mover.
 int mover_grid(Ctx* c, long long n) {
     const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
-    // Resident 256-lane blocks per CU.  Particles in flight are a window of
-    // consecutive (spatially sorted) indices; a smaller window keeps its field
-    // records in L2.  Two blocks (8 waves) per CU measured best on MI355X for
-    // both precisions on the 2048^2 snapshot; SP_MOVER_BLOCKS_PER_CU overrides.
-    int per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), kMoverBlocksPerCU);
+    // Resident 256-lane blocks per CU.  Particles in flight are, on each XCD,
+    // a window of consecutive (spatially sorted) indices; a smaller window
+    // keeps its field records in that XCD's L2.  Measured on MI355X: two
+    // blocks per CU on the 2048^2 snapshot, one on the 512^3 snapshot, for
+    // both precisions; SP_MOVER_BLOCKS_PER_CU overrides.
+    const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D : kMoverBlocksPerCU2D;
+    int per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), per_cu_default);
     if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {
         const int v = std::atoi(e);
         if (v > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), v);
@@ -228,6 +234,7 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     sp::MoverParams m{};
     m.P = sp::PtlBase{static_cast<char*>(c->pbuf[c->cur]), particle_cap(c->nmax)};
     m.n = c->nptl;
+    m.part_len = (c->nptl + sp::kParts - 1) / sp::kParts;
     m.rec1 = c->rec[0];
     m.rec2 = f.time_interp ? c->rec[1] : c->rec[0];
     m.ctr = c->ctr;
@@ -519,8 +526,8 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     ++c->mover_calls;
     const sp::MoverParams m = mover_params(c, t0, dtf, num_fine_steps, nsteps_interval);
     const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
-    // clear the work and step counters (leak counters are cumulative)
-    SP_HIP(hipMemsetAsync(&c->ctr->work, 0, 2 * sizeof(unsigned long long), c->st));
+    // clear the work, step and pause counters (leak counters are cumulative)
+    SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
     SP_HIP(hipEventRecord(c->ev[0], c->st));
     if (c->nptl > 0) {
         const int grid = mover_grid(c, c->nptl);

@@ -36,6 +36,13 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 }
 
 // genrand_double1 mapping: closed [0, 1]
+// Index of the XCD (accelerator die) the calling wave runs on.
+__device__ __forceinline__ unsigned xcc_id() {
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xFu;
+}
+
 __device__ __forceinline__ double u01(uint32_t r) { return (double)r * (1.0 / 4294967295.0); }
 
 // Particle boundary condition for one rank (particle_module.f90:1984-2129):

@@ -59,10 +59,18 @@ struct Ptl {
     uint32_t *rng_step;
 };
 
+// The mover splits its particle range into one contiguous part per XCD (the
+// chip's eight accelerator dies, each with its own L2); a block takes work
+// from the part of the die it runs on first, so each L2 serves a compact
+// window of the sorted particles.  Part counters sit 128 B apart.
+constexpr int kParts = 8;
+constexpr int kPartStride = 16;
+
 // Device-side counters of one context.
 struct Counters {
-    unsigned long long work;       // next particle index handed out by the mover
-    unsigned long long steps;      // particle-steps of the current mover call
+    unsigned long long part[kParts * kPartStride];  // next index inside each part
+    unsigned long long steps;      // particle-steps of the current mover launch
+    unsigned long long paused;     // particles that stopped at the step budget
     unsigned long long leak_fx;    // leak in units of 2^-32 weight
     unsigned long long negp_fx;    // leak_negp in units of 2^-32 weight
     unsigned long long nkeep;      // compaction / split results
@@ -103,6 +111,7 @@ enum MoverFlags {
 struct MoverParams {
     PtlBase P;
     long long n;
+    long long part_len;           // particles per XCD part (the last part may be shorter)
     const float* rec1;
     const float* rec2;
     Counters* ctr;

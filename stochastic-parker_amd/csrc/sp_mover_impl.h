@@ -45,6 +45,8 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
 
     long long idx = -1;
     bool exhausted = false;
+    // start on this die's part of the range, then help the other parts
+    int part = (int)(xcc_id() % (unsigned)kParts), parts_done = 0;
     State s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double w = 0.0, dtarget = 0.0, dt_old = 0.0;
     double dl[4] = {0.0, 0.0, 0.0, 0.0};
@@ -58,16 +60,22 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
         const unsigned long long need_mask = __ballot(need);
         if (need_mask) {
             const int leader = __ffsll((unsigned long long)need_mask) - 1;
+            const unsigned long long cnt = (unsigned long long)__popcll(need_mask);
             unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&m.ctr->work, (unsigned long long)__popcll(need_mask));
+            if (lane == leader) base = atomicAdd(&m.ctr->part[part * kPartStride], cnt);
             base = __shfl(base, leader);
+            const long long pb = (long long)part * m.part_len;
+            const long long pe = pb + m.part_len < m.n ? pb + m.part_len : m.n;
+            if (pb + (long long)(base + cnt) >= pe) {
+                // this part is used up: move on; after all parts, stop fetching
+                part = (part + 1) % kParts;
+                if (++parts_done == kParts) exhausted = true;
+            }
             if (need) {
-                const unsigned long long j =
-                    base + (unsigned long long)__popcll(need_mask & ((1ull << lane) - 1ull));
-                if (j >= (unsigned long long)m.n) {
-                    exhausted = true;
-                } else {
-                    idx = (long long)j;
+                const long long j =
+                    pb + (long long)(base + (unsigned long long)__popcll(need_mask & ((1ull << lane) - 1ull)));
+                if (j < pe) {
+                    idx = j;
                     s.x = px[idx];
                     s.y = py[idx];
                     s.z = pz[idx];
