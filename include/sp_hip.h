@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SP_HIP_ABI_VERSION 1
+#define SP_HIP_ABI_VERSION 2
 
 typedef enum sp_status {
     SP_OK = 0,
@@ -83,6 +83,10 @@ typedef struct sp_config {
     uint64_t seed;              /* Philox key */
     uint32_t stream;            /* Philox stream word (run / variant id) */
     int32_t sort_every;         /* order particles spatially before every k-th mover call (0: never) */
+    int32_t sort_steps;         /* with sort_every > 0: re-sort the particles still moving after
+                                   every sort_steps steps of each (0: once per call); results
+                                   do not depend on it */
+    int32_t reserved0;          /* must be 0 */
     int64_t nptl_max;           /* -nm */
 } sp_config;
 

@@ -89,18 +89,19 @@ void free_dev(void* p) {
 
 // One allocation per particle buffer set: capacity rounded up to 256 so every
 // array starts 256-byte aligned; array order and offsets are those of
-// sp::PtlBase (9 FP64 arrays, 2 int8 arrays, 6 int32 arrays).
+// sp::PtlBase (10 FP64 arrays, 2 int8 arrays, 6 int32 arrays).
 long long particle_cap(long long nmax) { return (nmax + 255) / 256 * 256; }
 
 int alloc_particles(long long nmax, sp::Ptl& P, void*& base) {
     const long long cap = particle_cap(nmax);
-    const size_t total = (size_t)98 * (size_t)cap;
+    const size_t total = (size_t)sp::kPtlBytes * (size_t)cap;
     if (hipMalloc(&base, total) != hipSuccess)
         return fail(SP_ERR_NOMEM, "device allocation of " + std::to_string(total) + " bytes for particles failed");
     const sp::PtlBase B{static_cast<char*>(base), cap};
     P.x = B.d(sp::PD_X); P.y = B.d(sp::PD_Y); P.z = B.d(sp::PD_Z); P.p = B.d(sp::PD_P);
     P.v = B.d(sp::PD_V); P.mu = B.d(sp::PD_MU); P.w = B.d(sp::PD_W); P.t = B.d(sp::PD_T);
     P.dt = B.d(sp::PD_DT);
+    P.rdt = B.d(sp::PD_RDT);
     P.split_times = B.b(sp::PB_SPLIT);
     P.count_flag = B.b(sp::PB_FLAG);
     P.origin = B.i(sp::PI_ORIGIN);
@@ -128,6 +129,7 @@ int validate(const sp_config* c) {
     if (c->pbcx < 0 || c->pbcx > 1 || c->pbcy < 0 || c->pbcy > 1 || c->pbcz < 0 || c->pbcz > 1) return fail(SP_ERR_ARG, "pbc must be 0 or 1");
     if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32) return fail(SP_ERR_ARG, "precision must be SP_PREC_F64 or SP_PREC_F32");
     if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
+    if (c->sort_steps < 0 || c->reserved0 != 0) return fail(SP_ERR_ARG, "need sort_steps >= 0 and reserved0 == 0");
     if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
     return SP_OK;
 }
@@ -192,9 +194,12 @@ int mover_grid(Ctx* c, long long n) {
 }
 
 // Orders the current particle set by the Morton key of 2^shift-cell blocks
-// (at most 64 blocks per axis) into the spare buffer set and swaps.
-int spatial_sort(Ctx* c) {
-    if (c->nptl <= 1) return SP_OK;
+// (at most 64 blocks per axis) into the spare buffer set and swaps.  With
+// paused_first, only the first `n` particles are ordered, in place: those the
+// mover paused (rdt > 0) first, in key order, then the others.
+int spatial_sort(Ctx* c, bool paused_first = false, long long n = -1) {
+    if (n < 0) n = c->nptl;
+    if (n <= 1) return SP_OK;
     if (c->nptl >= (1ll << 32)) return fail(SP_ERR_UNSUPPORTED, "sorting needs fewer than 2^32 particles");
     const long long cap = particle_cap(c->nmax);
     if (!c->skey[0]) {
@@ -216,14 +221,17 @@ int spatial_sort(Ctx* c) {
     sa.shift = std::max(0, bits - 6);
     if (const char* e = std::getenv("SP_SORT_SHIFT")) sa.shift = std::max(0, std::atoi(e));
     const int axis_bits = std::max(1, bits - sa.shift);
-    if (axis_bits > (f.ndim == 3 ? 10 : 16)) return fail(SP_ERR_UNSUPPORTED, "sort key too wide");
-    sa.passes = (f.ndim * axis_bits + 3) / 4;
+    if (axis_bits > (f.ndim == 3 ? 10 : 15)) return fail(SP_ERR_UNSUPPORTED, "sort key too wide");
+    const int kbits = f.ndim * axis_bits + (paused_first ? 1 : 0);
+    sa.done_bit = paused_first ? f.ndim * axis_bits : -1;
+    sa.passes = (kbits + 3) / 4;
     sa.xmin = f.xmin; sa.ymin = f.ymin; sa.zmin = f.zmin;
     sa.idx = 1.0 / f.dx; sa.idy = 1.0 / f.dy; sa.idz = 1.0 / f.dz;
-    SP_HIP(sp::launch_spatial_sort(c->P[c->cur], c->P[1 - c->cur], c->nptl, sa, c->skey[0],
+    const bool prefix = n < c->nptl;
+    SP_HIP(sp::launch_spatial_sort(c->P[c->cur], c->P[1 - c->cur], n, sa, c->skey[0],
                                    c->sidx[0], c->skey[1], c->sidx[1], c->shist, c->soff,
-                                   c->totals, c->st));
-    c->cur = 1 - c->cur;
+                                   c->totals, prefix, c->st));
+    if (!prefix) c->cur = 1 - c->cur;
     return SP_OK;
 }
 
@@ -519,22 +527,54 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
     if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
     if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
-    if (c->cfg.sort_every > 0 && c->mover_calls % c->cfg.sort_every == 0) {
+    const bool sorting = c->cfg.sort_every > 0;
+    SP_HIP(hipEventRecord(c->ev[0], c->st));
+    if (sorting && c->mover_calls % c->cfg.sort_every == 0) {
         rc = spatial_sort(c);
         if (rc != SP_OK) return rc;
     }
     ++c->mover_calls;
-    const sp::MoverParams m = mover_params(c, t0, dtf, num_fine_steps, nsteps_interval);
     const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
-    // clear the work, step and pause counters (leak counters are cumulative)
-    SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
-    SP_HIP(hipEventRecord(c->ev[0], c->st));
-    if (c->nptl > 0) {
-        const int grid = mover_grid(c, c->nptl);
+    // With sort_steps, the interval runs as launches of at most sort_steps
+    // steps per particle; between launches the paused particles are sorted
+    // to the front in spatial order and the next launch takes only them.
+    // Once fewer than 1/8 of the particles are still moving, the last launch
+    // runs them to the end of the interval: re-sorting a sparse remainder
+    // costs more than its locality gains.
+    // SP_SORT_STEPS overrides sort_steps (tuning).
+    unsigned budget = (unsigned)c->cfg.sort_steps;
+    if (const char* e = std::getenv("SP_SORT_STEPS")) budget = (unsigned)std::max(0, std::atoi(e));
+    if (!sorting) budget = 0u;
+    unsigned long long steps = 0;
+    float ms_push = 0.f;
+    long long nrun = c->nptl;
+    if (nrun > 0) SP_HIP(hipMemsetAsync(c->P[c->cur].rdt, 0, sizeof(double) * nrun, c->st));
+    while (nrun > 0) {
+        sp::MoverParams m = mover_params(c, t0, dtf, num_fine_steps, nsteps_interval);
+        m.n = nrun;
+        m.part_len = (nrun + sp::kParts - 1) / sp::kParts;
+        m.step_budget = nrun * 8 < c->nptl ? 0u : budget;
+        // clear the work, step and pause counters (leak counters are cumulative)
+        SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
+        SP_HIP(hipEventRecord(c->ev[1], c->st));
+        const int grid = mover_grid(c, nrun);
         if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
         else SP_HIP(sp::launch_mover_f64(m, ti, acc, c->cfg.ndim, grid, c->st));
+        SP_HIP(hipEventRecord(c->ev[3], c->st));
+        unsigned long long sp_[2] = {0, 0};
+        SP_HIP(hipMemcpyAsync(sp_, &c->ctr->steps, sizeof(sp_), hipMemcpyDeviceToHost, c->st));
+        SP_HIP(hipStreamSynchronize(c->st));
+        float ms = 0.f;
+        SP_HIP(hipEventElapsedTime(&ms, c->ev[1], c->ev[3]));
+        ms_push += ms;
+        steps += sp_[0];
+        const long long paused = (long long)sp_[1];
+        if (paused == 0) break;
+        if (paused > nrun) return fail(SP_ERR_STATE, "mover pause count exceeds the particles run");
+        rc = spatial_sort(c, true, nrun);
+        if (rc != SP_OK) return rc;
+        nrun = paused;
     }
-    SP_HIP(hipEventRecord(c->ev[1], c->st));
     // remove_particles: stable compaction into the other buffer set
     long long kept = 0, esc = 0;
     if (c->nptl > 0) {
@@ -557,11 +597,10 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
         c->nptl = kept;
         c->nesc += esc;
     }
-    float ms_push = 0.f, ms_all = 0.f;
-    SP_HIP(hipEventElapsedTime(&ms_push, c->ev[0], c->ev[1]));
+    float ms_all = 0.f;
     SP_HIP(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
     if (stats) {
-        stats->steps = (int64_t)hc.steps;
+        stats->steps = (int64_t)steps;
         stats->nptl_current = c->nptl;
         stats->nptl_escaped = c->nesc;
         stats->leak = (double)hc.leak_fx * kInvFx;

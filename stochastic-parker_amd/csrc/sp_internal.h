@@ -51,9 +51,11 @@ struct DevParams {
     uint32_t key0, key1, stream;
 };
 
-// Device particle arrays (structure of arrays).
+// Device particle arrays (structure of arrays).  `rdt` is internal to the
+// mover: the interval target time of a particle paused at the step budget
+// (0 = not paused); it is not part of the exchanged particle record.
 struct Ptl {
-    double *x, *y, *z, *p, *v, *mu, *w, *t, *dt;
+    double *x, *y, *z, *p, *v, *mu, *w, *t, *dt, *rdt;
     int8_t *split_times, *count_flag;
     int32_t *origin, *nsteps_tracked, *nsteps_pushed, *tag_injected, *tag_splitted;
     uint32_t *rng_step;
@@ -81,21 +83,22 @@ struct Counters {
 constexpr double kFxScale = 4294967296.0;  // 2^32 weight units per unit weight
 
 // Particle arrays of one buffer set live in one allocation with a common
-// capacity `cap` (a multiple of 256): the 9 FP64 arrays first, then the two
-// int8 arrays, then the six 32-bit arrays, in the order of Ptl.  The mover
-// addresses them from (base, cap) alone, which keeps its kernel arguments
-// small enough to stay in scalar registers.
-enum PtlD { PD_X, PD_Y, PD_Z, PD_P, PD_V, PD_MU, PD_W, PD_T, PD_DT };
+// capacity `cap` (a multiple of 256): the 10 FP64 arrays first, then the two
+// int8 arrays, then the six 32-bit arrays, in the order of Ptl (106 bytes per
+// particle).  The mover addresses them from (base, cap) alone, which keeps
+// its kernel arguments small enough to stay in scalar registers.
+enum PtlD { PD_X, PD_Y, PD_Z, PD_P, PD_V, PD_MU, PD_W, PD_T, PD_DT, PD_RDT };
 enum PtlB { PB_SPLIT, PB_FLAG };
 enum PtlI { PI_ORIGIN, PI_TRACKED, PI_PUSHED, PI_TAGI, PI_TAGS, PI_RSTEP };
+constexpr int kPtlBytes = 106;
 
 struct PtlBase {
     char* base;
     long long cap;
     __host__ __device__ double* d(int k) const { return reinterpret_cast<double*>(base + 8 * cap * k); }
-    __host__ __device__ int8_t* b(int k) const { return reinterpret_cast<int8_t*>(base + 72 * cap + cap * k); }
+    __host__ __device__ int8_t* b(int k) const { return reinterpret_cast<int8_t*>(base + 80 * cap + cap * k); }
     __host__ __device__ int32_t* i(int k) const {
-        return reinterpret_cast<int32_t*>(base + 74 * cap + 4 * cap * k);
+        return reinterpret_cast<int32_t*>(base + 82 * cap + 4 * cap * k);
     }
 };
 
@@ -112,6 +115,7 @@ struct MoverParams {
     PtlBase P;
     long long n;
     long long part_len;           // particles per XCD part (the last part may be shorter)
+    unsigned step_budget;         // steps per particle per launch before it pauses (0 = none)
     const float* rec1;
     const float* rec2;
     Counters* ctr;

@@ -21,6 +21,12 @@
 //                 the redo's boundary check and the advance of the target run
 //                 at the top of the next iteration (pending REDO)
 //   otherwise     advance the target (next fine step, or done)
+//
+// With a step budget, a particle that has taken that many steps in this
+// launch stops at the top of its inner loop (before the boundary check of its
+// next step): its state and interval target are stored and the launch counts
+// it as paused.  The next launch resumes it at the same point; the increments
+// depend only on the particle's counter, so pausing never changes a result.
 #pragma once
 
 #include "sp_device.h"
@@ -28,10 +34,46 @@
 namespace sp {
 
 enum : int { PEND_NONE = 0, PEND_PRE = 1, PEND_REDO = 2 };
-enum : int { ACT_NONE = 0, ACT_PRE = 1, ACT_PUSH = 2, ACT_PUSH_FIXED = 3, ACT_ADVANCE = 4 };
+enum : int { ACT_NONE = 0, ACT_PRE = 1, ACT_PUSH = 2, ACT_PUSH_FIXED = 3, ACT_ADVANCE = 4,
+              ACT_PAUSE = 5 };
+
+// Keeps a uniform floating-point value in vector registers: the hot loop
+// reads dozens of FP64/FP32 constants, more than the scalar register file
+// holds, and constants spilled from scalar registers cost one v_readlane per
+// 32-bit half at every use.  An operand already in a VGPR costs nothing.
+template <class T>
+__device__ __forceinline__ void pin_vgpr(T& v) {
+    asm volatile("" : "+v"(v));
+}
+
+__device__ __forceinline__ void pin_constants(MoverParams& m) {
+    pin_vgpr(m.t0); pin_vgpr(m.dtf); pin_vgpr(m.dt_fine); pin_vgpr(m.dt_limit);
+    pin_vgpr(m.xmin); pin_vgpr(m.ymin); pin_vgpr(m.zmin);
+    pin_vgpr(m.dx); pin_vgpr(m.dy); pin_vgpr(m.dz);
+    pin_vgpr(m.idx); pin_vgpr(m.idy); pin_vgpr(m.idz);
+    for (int k = 0; k < 3; ++k) {
+        pin_vgpr(m.slo[k]); pin_vgpr(m.shi[k]); pin_vgpr(m.elo[k]); pin_vgpr(m.ehi[k]);
+    }
+    pin_vgpr(m.p0); pin_vgpr(m.p_floor);
+    pin_vgpr(m.kpara0); pin_vgpr(m.kret); pin_vgpr(m.pindex); pin_vgpr(m.gamma_m2);
+    pin_vgpr(m.vdp_pre); pin_vgpr(m.d1p0); pin_vgpr(m.d2p02);
+    pin_vgpr(m.half_dx); pin_vgpr(m.half_dy); pin_vgpr(m.half_dz);
+    pin_vgpr(m.dt_min); pin_vgpr(m.dt_max);
+    pin_vgpr(m.f_kpara0); pin_vgpr(m.f_kret); pin_vgpr(m.f_pindex); pin_vgpr(m.f_gm2);
+    pin_vgpr(m.f_vdp_pre); pin_vgpr(m.f_d1p0); pin_vgpr(m.f_d2p02);
+    pin_vgpr(m.f_half_dx); pin_vgpr(m.f_half_dy); pin_vgpr(m.f_half_dz);
+    pin_vgpr(m.f_dt_min); pin_vgpr(m.f_dt_max);
+    pin_vgpr(m.inv_p0);
+    for (int k = 0; k < 6; ++k) pin_vgpr(m.acc_region[k]);
+    pin_vgpr(m.lx); pin_vgpr(m.ly); pin_vgpr(m.lz);
+}
 
 template <class Pol, int NDIM, bool TINTERP, bool ACC>
-__global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
+__global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
+    MoverParams m = m0;
+    // 2D steps are instruction-bound and gain from this; 3D steps are
+    // memory-bound and lose occupancy headroom to the extra VGPRs
+    if (NDIM == 2) pin_constants(m);
     const int lane = threadIdx.x & (kWave - 1);
     double* const px = m.P.d(PD_X);
     double* const py = m.P.d(PD_Y);
@@ -42,6 +84,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
     int8_t* const pflag = m.P.b(PB_FLAG);
     int32_t* const ppushed = m.P.i(PI_PUSHED);
     uint32_t* const prstep = reinterpret_cast<uint32_t*>(m.P.i(PI_RSTEP));
+    double* const prdt = m.P.d(PD_RDT);
 
     long long idx = -1;
     bool exhausted = false;
@@ -52,7 +95,9 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
     double dl[4] = {0.0, 0.0, 0.0, 0.0};
     int cf = 0, nsp = 0, pending = PEND_NONE;
     uint32_t tagi = 0, tags = 0, rstep = 0;
-    unsigned long long my_steps = 0;
+    unsigned ksteps = 0;          // steps of the current particle in this launch
+    bool resumed = false;
+    unsigned long long my_steps = 0, my_paused = 0;
 
     while (true) {
         // ---- refill idle lanes (wave-aggregated work fetch) ----
@@ -89,11 +134,20 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
                     tags = (uint32_t)m.P.i(PI_TAGS)[idx];
                     rstep = prstep[idx];
                     dl[0] = dl[1] = dl[2] = dl[3] = 0.0;
-                    // target time from the particle's own time stamp (:1570-1578)
-                    const double stp = ceil((s.t - m.t0) / m.dt_fine);
-                    if (stp <= 0.0) dtarget = m.dt_fine; else dtarget = stp * m.dt_fine;
-                    if (dtarget > m.dtf) dtarget = m.dtf;
-                    pending = PEND_PRE;
+                    ksteps = 0;
+                    const double rd = prdt[idx];
+                    resumed = rd > 0.0;
+                    if (resumed) {
+                        // paused at the top of the inner loop by an earlier launch
+                        dtarget = rd;
+                        pending = PEND_NONE;
+                    } else {
+                        // target time from the particle's own time stamp (:1570-1578)
+                        const double stp = ceil((s.t - m.t0) / m.dt_fine);
+                        if (stp <= 0.0) dtarget = m.dt_fine; else dtarget = stp * m.dt_fine;
+                        if (dtarget > m.dtf) dtarget = m.dtf;
+                        pending = PEND_PRE;
+                    }
                 }
             }
         }
@@ -111,8 +165,12 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
                 need_bc = true;
                 act = ACT_ADVANCE;
             } else if ((s.t - m.t0) < dtarget && cf == SP_COUNT_FLAG_INBOX) {
-                need_bc = true;  // top of the inner loop (:1600-1612)
-                act = ACT_PUSH;
+                if (m.step_budget != 0u && ksteps >= m.step_budget) {
+                    act = ACT_PAUSE;
+                } else {
+                    need_bc = true;  // top of the inner loop (:1600-1612)
+                    act = ACT_PUSH;
+                }
             } else if ((s.t - m.t0) > dtarget && cf == SP_COUNT_FLAG_INBOX) {
                 // land exactly on the target time (:1707-1720)
                 s.x = s.x - dl[0];
@@ -162,12 +220,19 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
         } else if (act == ACT_PRE) {
             fin = cf != SP_COUNT_FLAG_INBOX || !(dtarget < m.dt_limit);
         }
-        if (fin) {
-            if (cf == SP_COUNT_FLAG_INBOX) {
+        const bool pause = act == ACT_PAUSE;
+        if (fin || pause) {
+            if (fin && cf == SP_COUNT_FLAG_INBOX) {
                 const double lk = boundary_condition<NDIM>(m.flags, s.x, s.y, s.z, cf, w, m.elo[0],
                                                            m.elo[1], m.elo[2], m.ehi[0], m.ehi[1],
                                                            m.ehi[2]);
                 if (lk != 0.0) atomicAdd(&m.ctr->leak_fx, weight_fx(lk));
+            }
+            if (pause) {
+                prdt[idx] = dtarget;
+                ++my_paused;
+            } else if (resumed) {
+                prdt[idx] = 0.0;
             }
             px[idx] = s.x;
             py[idx] = s.y;
@@ -190,14 +255,19 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m) {
             if ((unsigned)v >= (unsigned)m.nsteps_interval) v %= m.nsteps_interval;
             nsp = v;
             ++my_steps;
+            ++ksteps;
             if (fixed) pending = PEND_REDO;
         }
     }
 
-    // ---- per-wave step total: one atomic per wave ----
+    // ---- per-wave step and pause totals: one atomic each per wave ----
 #pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) my_steps += __shfl_xor(my_steps, off);
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        my_steps += __shfl_xor(my_steps, off);
+        my_paused += __shfl_xor(my_paused, off);
+    }
     if (lane == 0 && my_steps) atomicAdd(&m.ctr->steps, my_steps);
+    if (lane == 0 && my_paused) atomicAdd(&m.ctr->paused, my_paused);
 }
 
 // Single-step kernel for parity checks: caller-supplied states and increments.

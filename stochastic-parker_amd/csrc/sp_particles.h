@@ -36,6 +36,8 @@ struct SortArgs {
     int ndim, nx, ny, nz;       // interior cells per axis
     int shift;                  // cells per sort block = 2^shift along each axis
     int passes;                 // 4-bit radix passes over the Morton key
+    int done_bit;               // >= 0: particles not paused in the mover (rdt == 0)
+                                // get this key bit and sort after the paused ones
     double xmin, ymin, zmin, idx, idy, idz;
 };
 
@@ -44,7 +46,7 @@ hipError_t launch_copy(const void* a, void* b, long long n16, hipStream_t s);
 hipError_t launch_spatial_sort(const Ptl& S, const Ptl& D, long long n, const SortArgs& sa,
                                uint32_t* keyA, uint32_t* idxA, uint32_t* keyB, uint32_t* idxB,
                                unsigned* hist, unsigned long long* off, unsigned long long* totals,
-                               hipStream_t s);
+                               bool copy_back, hipStream_t s);
 hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, long long nptl,
                          long long nmax, int tag0, const InjectArgs& ia, hipStream_t s);
 hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,

@@ -59,6 +59,7 @@ __global__ void __launch_bounds__(kBlock) key_kernel(Ptl P, long long n, SortArg
     } else {
         k = spread2(bx) | (spread2(by) << 1);
     }
+    if (sa.done_bit >= 0 && !(P.rdt[i] > 0.0)) k |= 1u << sa.done_bit;
     key[i] = k;
     idx[i] = (uint32_t)i;
 }
@@ -120,10 +121,11 @@ __global__ void __launch_bounds__(kBlock) digit_scatter_kernel(const uint32_t* k
     }
 }
 
+// D[j] = S[perm[j]] for j < n (perm == nullptr: the identity).
 __global__ void __launch_bounds__(kBlock) permute_kernel(Ptl S, Ptl D, long long n, const uint32_t* perm) {
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const long long i = perm[j];
+    const long long i = perm ? (long long)perm[j] : j;
     D.x[j] = S.x[i];
     D.y[j] = S.y[i];
     D.z[j] = S.z[i];
@@ -133,6 +135,7 @@ __global__ void __launch_bounds__(kBlock) permute_kernel(Ptl S, Ptl D, long long
     D.w[j] = S.w[i];
     D.t[j] = S.t[i];
     D.dt[j] = S.dt[i];
+    D.rdt[j] = S.rdt[i];
     D.split_times[j] = S.split_times[i];
     D.count_flag[j] = S.count_flag[i];
     D.origin[j] = S.origin[i];
@@ -148,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) permute_kernel(Ptl S, Ptl D, long long
 hipError_t launch_spatial_sort(const Ptl& S, const Ptl& D, long long n, const SortArgs& sa,
                                uint32_t* keyA, uint32_t* idxA, uint32_t* keyB, uint32_t* idxB,
                                unsigned* hist, unsigned long long* off, unsigned long long* totals,
-                               hipStream_t s) {
+                               bool copy_back, hipStream_t s) {
     if (n <= 1) return hipSuccess;
     const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
     const long long ntiles = num_tiles(n);
@@ -165,6 +168,11 @@ hipError_t launch_spatial_sort(const Ptl& S, const Ptl& D, long long n, const So
         t = idxA; idxA = idxB; idxB = t;
     }
     hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(kBlock), 0, s, S, D, n, idxA);
+    if (copy_back) {
+        // a sorted prefix goes back over the prefix of S; the rest of S stays
+        hipLaunchKernelGGL(permute_kernel, dim3(grid), dim3(kBlock), 0, s, D, S, n,
+                           (const uint32_t*)nullptr);
+    }
     return hipGetLastError();
 }
 

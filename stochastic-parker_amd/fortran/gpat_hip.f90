@@ -18,7 +18,7 @@ module gpat_hip
     public mhd_configuration, mhd_config, tstamps_mhd, get_variable, load_mhd_config, &
         init_tstamps_mhd, calc_tstamps_mhd, free_tstamps_mhd, &
         read_particle_boundary_conditions, read_particle_params, read_diagnostics_params, &
-        set_drift_parameters, set_flag_check_drift_2d, set_precision, &
+        set_drift_parameters, set_flag_check_drift_2d, set_precision, set_sort_steps, &
         init_particles, free_particles, init_field_data, free_field_data, &
         read_field_data_parallel, calc_fields_gradients, copy_fields, &
         inject_particles_spatial_uniform, particle_mover, split_particle, &
@@ -266,6 +266,12 @@ contains
         cfg%seed = seed
     end subroutine set_precision
 
+    !< spatial sort cadence inside one mover call (steps per particle; 0: once)
+    subroutine set_sort_steps(nsteps)
+        integer, intent(in) :: nsteps
+        cfg%sort_steps = nsteps
+    end subroutine set_sort_steps
+
     !< init_field_data (mhd_data_parallel.f90:66-101); the context is created
     !< here, after the grid and the particle parameters are known
     subroutine init_field_data(time_interp_flag, ndim)
@@ -280,6 +286,7 @@ contains
         cfg%nptl_max = nptl_max_allowed
         cfg%stream = 0
         cfg%sort_every = 1
+        cfg%reserved0 = 0
         call check_status(sp_ctx_create(cfg, ctx), "sp_ctx_create")
         nptl_current = 0
         leak = 0.0_dp

@@ -38,7 +38,7 @@ module sp_hip_binding
         real(c_double) :: acc_region(6)
         integer(c_int32_t) :: time_interp, precision
         integer(c_int64_t) :: seed
-        integer(c_int32_t) :: stream, sort_every
+        integer(c_int32_t) :: stream, sort_every, sort_steps, reserved0
         integer(c_int64_t) :: nptl_max
     end type sp_config
 

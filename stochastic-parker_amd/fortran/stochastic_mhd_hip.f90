@@ -6,8 +6,9 @@
 !< particle_module / mhd_data_parallel / diagnostics procedures.
 !<
 !< GPU-only flags: -gp/--gpu_precision (f64 = reference arithmetic, default;
-!< f32 = FP32 gather/coefficients with FP64 positions and time) and
-!< -rs/--rng_seed (Philox key).
+!< f32 = FP32 gather/coefficients with FP64 positions and time),
+!< -rs/--rng_seed (Philox key) and -ss/--sort_steps (spatial re-sort of the
+!< particles every that many steps inside an interval; 0 = once per interval).
 !*******************************************************************************
 program stochastic_hip
     use iso_c_binding
@@ -27,6 +28,7 @@ program stochastic_hip
     integer :: t_start = 0, t_end = 200, tmax_mhd = 100000, time_interp_flag = 0
     integer :: single_time_frame = 0, dist_flag = 0, split_flag = 1, nsteps_interval = 10
     integer :: num_fine_steps = 1, tmax_to_inject = 100000, ndim_field = 2, charge = -1
+    integer :: sort_steps = 0
     integer :: uniform_grid = 0, spherical_coord = 0, check_drift_2d = 0
     integer :: dpp_wave = 0, dpp_shear = 0, deltab_flag = 0, correlation_flag = 0
     integer :: include_3rd_dim = 0, acc_by_surface = 0, size_mpi_sub = 1
@@ -54,6 +56,7 @@ program stochastic_hip
     call set_drift_parameters(drift_param1, drift_param2, charge)
     if (ndim_field == 2 .and. check_drift_2d == 1) call set_flag_check_drift_2d(check_drift_2d)
     call set_precision(merge(1, 0, trim(gpu_precision) == 'f32'), rng_seed)
+    call set_sort_steps(sort_steps)
     call init_particles(nptl_max)
     call init_particle_distributions(local_dist, dump_escaped_dist)
     tmin = t_start
@@ -230,6 +233,7 @@ contains
             case ('-ded', '--dump_escaped_dist'); dump_escaped_dist = to_logical(val)
             case ('-gp', '--gpu_precision'); gpu_precision = val
             case ('-rs', '--rng_seed'); rng_seed = int(to_real(val), c_int64_t)
+            case ('-ss', '--sort_steps'); sort_steps = to_int(val)
             case ('-sn', '-ib', '-iv', '-ir', '-iaj', '-jz', '-nn', '-db2', '-nb', '-dv', '-nv', &
                   '-rm', '-nr', '-ajm', '-naj', '-t0', '-ws', '-pd', '-ptf', '-de', '-du', &
                   '-sf1', '-sn1', '-s2e', '-ii', '-sf2', '-sn2')

@@ -48,6 +48,7 @@ class SpConfig(C.Structure):
         ("acc_region", C.c_double * 6),
         ("time_interp", C.c_int32), ("precision", C.c_int32),
         ("seed", C.c_uint64), ("stream", C.c_uint32), ("sort_every", C.c_int32),
+        ("sort_steps", C.c_int32), ("reserved0", C.c_int32),
         ("nptl_max", C.c_int64),
     ]
 

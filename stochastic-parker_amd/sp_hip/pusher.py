@@ -35,7 +35,8 @@ CONFIG_DEFAULTS = dict(
     p0=1.0, pmin=1e-2, pmax=1e4, b0=1.0, gamma_turb=5.0 / 3.0, kpara0=1e-3, kret=0.01,
     dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
     acc_region_flag=0, acc_region=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), time_interp=0,
-    precision=_abi.SP_PREC_F64, seed=20261202, stream=0, sort_every=1, nptl_max=1000,
+    precision=_abi.SP_PREC_F64, seed=20261202, stream=0, sort_every=1, sort_steps=0, reserved0=0,
+    nptl_max=1000,
 )
 
 
