@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SP_HIP_ABI_VERSION 2
+#define SP_HIP_ABI_VERSION 3
 
 typedef enum sp_status {
     SP_OK = 0,
@@ -125,6 +125,15 @@ typedef struct sp_dist_config {
     sp_local_dist local[4];
 } sp_dist_config;
 
+/* One parameter variant of a batched run: the diffusion parameters of
+ * read_particle_params (particle_module.f90:2778-2878) that a sweep varies. */
+typedef struct sp_variant {
+    double kpara0;          /* kappa_parallel at p0, B0 */
+    double kret;            /* kappa_perp / kappa_parallel */
+    double gamma_turb;      /* turbulence index: kappa ~ p^(3 - gamma_turb) */
+    double reserved0;       /* must be 0 */
+} sp_variant;
+
 /* ---- context --------------------------------------------------------- */
 /* Replaces the module initialisation chain: set_mhd_config / set_field_configuration
  * (mhd_config.f90:43-74, simulation_setup.f90:171-253), init_field_data
@@ -183,6 +192,25 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
  * (dx, dy, dz, dp) per particle.  fixed_dt is per particle (0/1). */
 int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
                  const double* rands, const int8_t* fixed_dt, double* deltas);
+
+/* ---- batched parameter sweeps (no reference counterpart) ----------------
+ * The reference runs one parameter set per job.  Here one context can push
+ * many variants at once: each particle's `origin` holds its variant index and
+ * the mover takes that variant's kpara0, kret and gamma_turb; every other
+ * setting is the context's.  Each particle's result is the one a
+ * single-variant run with that variant's parameters gives.
+ * sp_set_variants installs nvar variants (nvar = 0 returns to the
+ * configuration's values). */
+int sp_set_variants(void* ctx, int32_t nvar, const sp_variant* variants);
+/* inject_particles_spatial_uniform for each variant v in turn: nptl[v]
+ * particles in part_boxes[6 v .. 6 v + 5], origin = v. */
+int sp_inject_variants(void* ctx, int32_t nvar, const int64_t* nptl, double dt, int32_t dist_flag,
+                       double particle_v0, double t0, double dtf, const double* part_boxes,
+                       double power_index);
+/* f(p) per variant: fglobal is (npp_global, nvar) in Fortran order (the
+ * momentum bins of sp_distributions_init, summed over mu); counts[v] is the
+ * number of particles of variant v (NULL skips).  Needs sp_distributions_init. */
+int sp_variant_distributions(void* ctx, int32_t nvar, double* fglobal, int64_t* counts);
 
 /* ---- diagnostics ------------------------------------------------------ */
 /* init_particle_distributions (diagnostics.f90:178-226) */

@@ -70,6 +70,11 @@ struct Ctx {
     unsigned* shist = nullptr;
     unsigned long long* soff = nullptr;
     long long mover_calls = 0;
+    // batched parameter variants
+    sp::KappaVar* dvar = nullptr;
+    int nvar = 0, nvar_cap = 0;
+    unsigned long long* vhist = nullptr;   // per-variant f(p) bins, then per-variant counts
+    size_t vhist_len = 0;
     int mover_blocks_per_cu[2] = {0, 0};
     // distributions
     bool dist_ready = false;
@@ -235,6 +240,22 @@ int spatial_sort(Ctx* c, bool paused_first = false, long long n = -1) {
     return SP_OK;
 }
 
+// Diffusion parameters of one variant in the forms both step paths use, by
+// the same expressions as the configuration's (pindex = 3 - gamma_turb,
+// particle_module.f90:2798).
+sp::KappaVar kappa_var(double kpara0, double kret, double gamma_turb) {
+    sp::KappaVar k{};
+    k.kpara0 = kpara0;
+    k.kret = kret;
+    k.pindex = 3.0 - gamma_turb;
+    k.gamma_m2 = gamma_turb - 2.0;
+    k.f_kpara0 = (float)kpara0;
+    k.f_kret = (float)kret;
+    k.f_pindex = (float)k.pindex;
+    k.f_gm2 = (float)k.gamma_m2;
+    return k;
+}
+
 // Kernel arguments of the mover / single-step kernels for one interval.
 sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_interval) {
     const sp_config& f = c->cfg;
@@ -268,20 +289,23 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.ehi[0] = f.xmax; m.ehi[1] = f.ymax; m.ehi[2] = f.zmax;
     m.p0 = f.p0;
     m.p_floor = 0.25 * f.p0;
-    m.kpara0 = f.kpara0;
-    m.kret = f.kret;
-    m.pindex = d.pindex;
-    m.gamma_m2 = f.gamma_turb - 2.0;
+    const sp::KappaVar kv = kappa_var(f.kpara0, f.kret, f.gamma_turb);
+    m.kpara0 = kv.kpara0;
+    m.kret = kv.kret;
+    m.pindex = kv.pindex;
+    m.gamma_m2 = kv.gamma_m2;
+    m.var = c->dvar;
+    m.nvar = c->nvar;
     m.vdp_pre = (double)(1.0f / (float)(3 * f.charge));
     m.d1p0 = f.drift1 * f.p0;
     m.d2p02 = f.drift2 * (f.p0 * f.p0);
     m.half_dx = 0.5 * f.dx; m.half_dy = 0.5 * f.dy; m.half_dz = 0.5 * f.dz;
     m.dt_min = f.dt_min_rel * dtf;  // set_dt_min_max (:5519-5524)
     m.dt_max = f.dt_max_rel * dtf;
-    m.f_kpara0 = (float)f.kpara0;
-    m.f_kret = (float)f.kret;
-    m.f_pindex = (float)d.pindex;
-    m.f_gm2 = (float)(f.gamma_turb - 2.0);
+    m.f_kpara0 = kv.f_kpara0;
+    m.f_kret = kv.f_kret;
+    m.f_pindex = kv.f_pindex;
+    m.f_gm2 = kv.f_gm2;
     m.f_vdp_pre = 1.0f / (float)(3 * f.charge);
     m.f_d1p0 = (float)m.d1p0;
     m.f_d2p02 = (float)m.d2p02;
@@ -316,6 +340,8 @@ void destroy(Ctx* c) {
     free_dev(c->soff);
     free_dev(c->dglob);
     for (auto* q : c->dloc) free_dev(q);
+    free_dev(c->dvar);
+    free_dev(c->vhist);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -778,6 +804,96 @@ int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal
         SP_HIP(hipMemcpy(hl.data(), c->dloc[k], sizeof(unsigned long long) * c->loc_size[k], hipMemcpyDeviceToHost));
         for (long long i = 0; i < c->loc_size[k]; ++i) outs[k][i] = (double)hl[i] * kInvFx;
     }
+    return SP_OK;
+}
+
+int sp_set_variants(void* ctx, int32_t nvar, const sp_variant* variants) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (nvar < 0 || nvar > (1 << 24)) return fail(SP_ERR_ARG, "nvar must be in [0, 2^24]");
+    if (nvar > 0 && !variants) return fail(SP_ERR_ARG, "variants is NULL");
+    std::vector<sp::KappaVar> hv((size_t)nvar);
+    for (int v = 0; v < nvar; ++v) {
+        const sp_variant& q = variants[v];
+        if (!(q.kpara0 > 0)) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": kpara0 must be > 0");
+        if (!(q.kret >= 0) || q.kret > 1) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": kret must be in [0, 1]");
+        if (!std::isfinite(q.gamma_turb)) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": gamma_turb must be finite");
+        if (q.reserved0 != 0) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": reserved0 must be 0");
+        hv[(size_t)v] = kappa_var(q.kpara0, q.kret, q.gamma_turb);
+    }
+    if (nvar > c->nvar_cap) {
+        free_dev(c->dvar);
+        c->dvar = nullptr;
+        c->nvar_cap = 0;
+        SP_HIP(hipMalloc(&c->dvar, sizeof(sp::KappaVar) * (size_t)nvar));
+        c->nvar_cap = nvar;
+    }
+    if (nvar > 0)
+        SP_HIP(hipMemcpyAsync(c->dvar, hv.data(), sizeof(sp::KappaVar) * (size_t)nvar,
+                              hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->nvar = nvar;
+    return SP_OK;
+}
+
+int sp_inject_variants(void* ctx, int32_t nvar, const int64_t* nptl, double dt, int32_t dist_flag,
+                       double particle_v0, double t0, double dtf, const double* part_boxes,
+                       double power_index) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (nvar < 1 || nvar > c->nvar) return fail(SP_ERR_ARG, "nvar must be in [1, number of installed variants]");
+    if (!nptl || !part_boxes) return fail(SP_ERR_ARG, "nptl or part_boxes is NULL");
+    if (dist_flag < 0 || dist_flag > 2) return fail(SP_ERR_ARG, "dist_flag must be 0, 1 or 2");
+    long long total = 0;
+    for (int v = 0; v < nvar; ++v) {
+        if (nptl[v] < 0) return fail(SP_ERR_ARG, "nptl must be >= 0");
+        total += nptl[v];
+    }
+    if ((long long)c->tag_max + total > 2147483647ll) return fail(SP_ERR_CAPACITY, "tag_max would overflow int32");
+    if (c->nptl + total > c->nmax) return fail(SP_ERR_CAPACITY, "the variants' particles exceed nptl_max");
+    for (int v = 0; v < nvar; ++v) {
+        if (nptl[v] == 0) continue;
+        sp::InjectArgs ia{};
+        for (int i = 0; i < 6; ++i) ia.box[i] = part_boxes[6 * v + i];
+        ia.dt = dt; ia.v0 = particle_v0; ia.t0 = t0; ia.dtf = dtf; ia.power_index = power_index;
+        ia.dist_flag = dist_flag;
+        ia.origin = v;
+        SP_HIP(sp::launch_inject(c->dp, c->P[c->cur], c->nptl, nptl[v], c->nmax, c->tag_max, ia, c->st));
+        c->nptl += nptl[v];
+        c->tag_max += (int)nptl[v];
+    }
+    SP_HIP(hipStreamSynchronize(c->st));
+    return SP_OK;
+}
+
+int sp_variant_distributions(void* ctx, int32_t nvar, double* fglobal, int64_t* counts) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->dist_ready) return fail(SP_ERR_STATE, "distributions are not initialised");
+    if (nvar < 1 || nvar > c->nvar) return fail(SP_ERR_ARG, "nvar must be in [1, number of installed variants]");
+    const long long npp = c->da.npp;
+    const size_t nbins = (size_t)npp * (size_t)nvar;
+    if (c->vhist_len < nbins + (size_t)nvar) {
+        free_dev(c->vhist);
+        c->vhist = nullptr;
+        c->vhist_len = 0;
+        SP_HIP(hipMalloc(&c->vhist, sizeof(unsigned long long) * (nbins + (size_t)nvar)));
+        c->vhist_len = nbins + (size_t)nvar;
+    }
+    unsigned long long* hist = c->vhist;
+    unsigned long long* cnt = c->vhist + nbins;
+    SP_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * (nbins + (size_t)nvar), c->st));
+    SP_HIP(sp::launch_variant_dist(c->P[c->cur], c->nptl, c->da, nvar, hist, cnt, c->st));
+    std::vector<unsigned long long> hb(nbins + (size_t)nvar);
+    SP_HIP(hipMemcpyAsync(hb.data(), hist, sizeof(unsigned long long) * hb.size(), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    if (fglobal)
+        for (size_t i = 0; i < nbins; ++i) fglobal[i] = (double)hb[i] * kInvFx;
+    if (counts)
+        for (int v = 0; v < nvar; ++v) counts[v] = (int64_t)hb[nbins + (size_t)v];
     return SP_OK;
 }
 

@@ -102,6 +102,12 @@ struct PtlBase {
     }
 };
 
+// Diffusion parameters of one variant, in the forms the two step paths use.
+struct KappaVar {
+    double kpara0, kret, pindex, gamma_m2;
+    float f_kpara0, f_kret, f_pindex, f_gm2;
+};
+
 // Bit flags of MoverParams::flags
 enum MoverFlags {
     MF_OPEN_X = 1, MF_OPEN_Y = 2, MF_OPEN_Z = 4,
@@ -116,6 +122,8 @@ struct MoverParams {
     long long n;
     long long part_len;           // particles per XCD part (the last part may be shorter)
     unsigned step_budget;         // steps per particle per launch before it pauses (0 = none)
+    const KappaVar* var;          // per-variant diffusion parameters (nvar > 0)
+    int nvar;                     // 0: every particle uses the values below
     const float* rec1;
     const float* rec2;
     Counters* ctr;

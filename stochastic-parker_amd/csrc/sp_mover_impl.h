@@ -130,6 +130,16 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
                     w = m.P.d(PD_W)[idx];
                     cf = pflag[idx];
                     nsp = ppushed[idx];
+                    if (m0.nvar > 0) {
+                        // this particle's variant: its diffusion parameters
+                        int v = m.P.i(PI_ORIGIN)[idx];
+                        v = v < 0 ? 0 : (v >= m0.nvar ? m0.nvar - 1 : v);
+                        const KappaVar kv = m0.var[v];
+                        m.kpara0 = kv.kpara0; m.kret = kv.kret;
+                        m.pindex = kv.pindex; m.gamma_m2 = kv.gamma_m2;
+                        m.f_kpara0 = kv.f_kpara0; m.f_kret = kv.f_kret;
+                        m.f_pindex = kv.f_pindex; m.f_gm2 = kv.f_gm2;
+                    }
                     tagi = (uint32_t)m.P.i(PI_TAGI)[idx];
                     tags = (uint32_t)m.P.i(PI_TAGS)[idx];
                     rstep = prstep[idx];

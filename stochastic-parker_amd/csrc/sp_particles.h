@@ -12,6 +12,7 @@ struct InjectArgs {
     double box[6];
     double dt, v0, t0, dtf, power_index;
     int dist_flag;
+    int origin;                 // stored in each injected particle (variant index)
 };
 
 struct SplitArgs {
@@ -59,6 +60,8 @@ hipError_t launch_split_apply(const Ptl& P, long long n, long long nmax, const S
                               const unsigned long long* tile_off, hipStream_t s);
 hipError_t launch_dist(const Ptl& P, long long n, const DistArgs& da, unsigned long long* gglob,
                        unsigned long long* const gloc[4], int grid, hipStream_t s);
+hipError_t launch_variant_dist(const Ptl& P, long long n, const DistArgs& da, int nvar,
+                               unsigned long long* hist, unsigned long long* counts, hipStream_t s);
 hipError_t launch_stats(const Ptl& P, long long n, double* part, int grid, hipStream_t s);
 
 }  // namespace sp

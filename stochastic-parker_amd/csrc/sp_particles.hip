@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, lo
     P.dt[slot] = ia.dt;
     P.split_times[slot] = 0;
     P.count_flag[slot] = SP_COUNT_FLAG_INBOX;
-    P.origin[slot] = 0;
+    P.origin[slot] = ia.origin;
     P.nsteps_tracked[slot] = 0;
     P.nsteps_pushed[slot] = 0;
     P.tag_injected[slot] = tag0 + (int)i;
@@ -341,6 +341,26 @@ __global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, DistAr
 
 // quick_check / get_pmax_global partials, one row per block:
 // {sum w (fixed point, as double bits), sum dt, min dt, max dt, max p}
+// f(p) per parameter variant: the global momentum bins of dist_kernel
+// (summed over mu) for the particles whose origin is each variant, plus the
+// particle count of each variant.  Fixed-point weights, so the sums do not
+// depend on the order of the atomics.
+__global__ void __launch_bounds__(kBlock) variant_dist_kernel(Ptl P, long long n, DistArgs da,
+                                                              int nvar,
+                                                              unsigned long long* hist,
+                                                              unsigned long long* counts) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = P.origin[i];
+    if (v < 0 || v >= nvar) return;
+    atomicAdd(&counts[v], 1ull);
+    const double p = P.p[i];
+    if (!(p > da.pmin && p <= da.pmax)) return;
+    const long long ip = floor_ll((log10(p) - da.pmin_log) / da.dp_log) + 1;
+    if (ip >= 1 && ip <= da.npp)
+        atomicAdd(&hist[(ip - 1) + (long long)da.npp * v], weight_fx(P.w[i]));
+}
+
 __global__ void __launch_bounds__(kBlock) stats_kernel(Ptl P, long long n, double* part) {
     __shared__ double s_sum[kBlock / kWave], s_min[kBlock / kWave], s_max[kBlock / kWave],
         s_pmx[kBlock / kWave];
@@ -450,6 +470,15 @@ hipError_t launch_dist(const Ptl& P, long long n, const DistArgs& da, unsigned l
     const size_t shm = sizeof(unsigned long long) * (size_t)da.npp * da.nmu;
     hipLaunchKernelGGL(dist_kernel, dim3(grid), dim3(kBlock), shm, s, P, n, da, gglob, gloc[0],
                        gloc[1], gloc[2], gloc[3]);
+    return hipGetLastError();
+}
+
+hipError_t launch_variant_dist(const Ptl& P, long long n, const DistArgs& da, int nvar,
+                               unsigned long long* hist, unsigned long long* counts, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(variant_dist_kernel, dim3(grid), dim3(kBlock), 0, s, P, n, da, nvar, hist,
+                       counts);
     return hipGetLastError();
 }
 

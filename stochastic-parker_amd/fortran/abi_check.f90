@@ -9,5 +9,6 @@ program abi_check
     type(sp_local_dist) :: l
     type(sp_dist_config) :: d
     type(sp_particles) :: p
-    print "(5I6)", c_sizeof(c), c_sizeof(m), c_sizeof(l), c_sizeof(d), c_sizeof(p)
+    type(sp_variant) :: v
+    print "(6I6)", c_sizeof(c), c_sizeof(m), c_sizeof(l), c_sizeof(d), c_sizeof(p), c_sizeof(v)
 end program abi_check

@@ -8,13 +8,14 @@ module sp_hip_binding
     use iso_c_binding
     implicit none
     private
-    public sp_config, sp_particles, sp_mover_stats, sp_local_dist, sp_dist_config
+    public sp_config, sp_particles, sp_mover_stats, sp_local_dist, sp_dist_config, sp_variant
     public sp_ctx_create, sp_ctx_destroy, sp_last_error, sp_abi_version, &
         sp_fields_upload, sp_fields_gradients, sp_fields_copy, &
         sp_particles_upload, sp_particles_download, sp_particles_counts, &
         sp_inject_particles_spatial_uniform, sp_particle_mover, sp_sort_particles, &
         sp_split_particle, sp_step_once, sp_distributions_init, sp_distributions, &
-        sp_local_dist_dims, sp_quick_check, sp_calibration_copy, sp_error_message
+        sp_local_dist_dims, sp_quick_check, sp_calibration_copy, sp_error_message, &
+        sp_set_variants, sp_inject_variants, sp_variant_distributions
     public SP_OK, SP_PREC_F64, SP_PREC_F32
 
     integer(c_int), parameter :: SP_OK = 0
@@ -58,6 +59,10 @@ module sp_hip_binding
         integer(c_int32_t) :: enabled, npbins, nmu, rx, ry, rz
         real(c_double) :: pmin, pmax
     end type sp_local_dist
+
+    type, bind(C) :: sp_variant
+        real(c_double) :: kpara0, kret, gamma_turb, reserved0
+    end type sp_variant
 
     type, bind(C) :: sp_dist_config
         integer(c_int32_t) :: npp_global, nmu_global, local_dist, reserved0
@@ -211,6 +216,36 @@ module sp_hip_binding
             integer(c_int64_t), value :: nbytes
             real(c_double), intent(out) :: ms
         end function sp_calibration_copy
+
+        integer(c_int) function sp_set_variants(ctx, nvar, variants) &
+                bind(C, name="sp_set_variants")
+            import :: c_int, c_int32_t, c_ptr, sp_variant
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nvar
+            type(sp_variant), intent(in) :: variants(*)
+        end function sp_set_variants
+
+        integer(c_int) function sp_inject_variants(ctx, nvar, nptl, dt, dist_flag, particle_v0, &
+                t0, dtf, part_boxes, power_index) bind(C, name="sp_inject_variants")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nvar
+            integer(c_int64_t), intent(in) :: nptl(*)
+            real(c_double), value :: dt
+            integer(c_int32_t), value :: dist_flag
+            real(c_double), value :: particle_v0, t0, dtf
+            real(c_double), intent(in) :: part_boxes(*)
+            real(c_double), value :: power_index
+        end function sp_inject_variants
+
+        integer(c_int) function sp_variant_distributions(ctx, nvar, fglobal, counts) &
+                bind(C, name="sp_variant_distributions")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nvar
+            real(c_double), intent(out) :: fglobal(*)
+            integer(c_int64_t), intent(out) :: counts(*)
+        end function sp_variant_distributions
 
         integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
             import :: c_size_t, c_ptr

@@ -81,6 +81,11 @@ class SpDistConfig(C.Structure):
                 ("local", SpLocalDist * 4)]
 
 
+class SpVariant(C.Structure):
+    _fields_ = [("kpara0", C.c_double), ("kret", C.c_double), ("gamma_turb", C.c_double),
+                ("reserved0", C.c_double)]
+
+
 # Particle record members: name -> numpy dtype (order of sp_particles)
 PARTICLE_FIELDS = [
     ("x", np.float64), ("y", np.float64), ("z", np.float64), ("p", np.float64),
@@ -118,6 +123,10 @@ EXPORTS = {
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],
     "sp_quick_check": [_P, _D],
     "sp_calibration_copy": [_P, C.c_int64, _D],
+    "sp_set_variants": [_P, C.c_int32, C.POINTER(SpVariant)],
+    "sp_inject_variants": [_P, C.c_int32, C.POINTER(C.c_int64), C.c_double, C.c_int32,
+                           C.c_double, C.c_double, C.c_double, _D, C.c_double],
+    "sp_variant_distributions": [_P, C.c_int32, _D, C.POINTER(C.c_int64)],
 }
 
 _lib = None

@@ -26,7 +26,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import PARTICLE_FIELDS, SpConfig, SpDistConfig, SpMoverStats, SpParticles, check, dptr
+from ._abi import (PARTICLE_FIELDS, SpConfig, SpDistConfig, SpMoverStats, SpParticles, SpVariant,
+                   check, dptr)
 
 CONFIG_DEFAULTS = dict(
     ndim=2, nx=64, ny=64, nz=1, dx=1.0 / 64, dy=1.0 / 64, dz=1.0,
@@ -84,6 +85,7 @@ class ParkerPusher:
         nzg = self.cfg.nz + 4 if self.cfg.ndim == 3 else self.cfg.nz
         self.shape = (nzg, nyg, nxg)
         self._dist_dims = None
+        self._nvar = 0
 
     # -- lifetime ---------------------------------------------------------
     def close(self):
@@ -190,6 +192,7 @@ class ParkerPusher:
                 setattr(dc.local[k], key, L[key])
         self._call("sp_distributions_init", C.byref(dc))
         self._npp = npp_global * nmu_global
+        self._npp_p = npp_global
         dims = []
         for k in range(4):
             d = (C.c_int64 * 5)()
@@ -217,6 +220,35 @@ class ParkerPusher:
         for a, d in zip(outs, self._dist_dims):
             shaped.append(None if a is None else a.reshape(d[4], d[3], d[2], d[1], d[0]))
         return fg, shaped
+
+    # -- batched parameter variants (no reference counterpart) -----------
+    def set_variants(self, variants):
+        """variants: sequence of (kpara0, kret, gamma_turb); [] returns to the
+        configuration's values."""
+        n = len(variants)
+        arr = (SpVariant * max(n, 1))()
+        for i, (k0, kr, gt) in enumerate(variants):
+            arr[i].kpara0, arr[i].kret, arr[i].gamma_turb, arr[i].reserved0 = k0, kr, gt, 0.0
+        self._call("sp_set_variants", n, arr)
+        self._nvar = n
+
+    def inject_variants(self, nptl, dt, dist_flag, particle_v0, t0, dtf, boxes, power_index=7.0):
+        """nptl[v] particles of variant v uniformly in boxes[v] (6 values each)."""
+        counts = np.ascontiguousarray(nptl, dtype=np.int64)
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        if len(b) != len(counts):
+            raise ValueError("one box per variant")
+        self._call("sp_inject_variants", len(counts), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                   float(dt), int(dist_flag), float(particle_v0), float(t0), float(dtf), dptr(b),
+                   float(power_index))
+
+    def variant_distributions(self, nvar=None):
+        """(f, counts): f[v] is variant v's f(p) over the npp_global bins."""
+        nvar = self._nvar if nvar is None else nvar
+        f = np.zeros((nvar, self._npp_p), dtype=np.float64)
+        cnt = np.zeros(nvar, dtype=np.int64)
+        self._call("sp_variant_distributions", nvar, dptr(f), cnt.ctypes.data_as(C.POINTER(C.c_int64)))
+        return f, cnt
 
     def calibration_copy(self, nbytes: int) -> float:
         ms = C.c_double()

@@ -35,11 +35,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _abi.load().sp_abi_version() == 2
+    assert _abi.load().sp_abi_version() == 3
 
 
 def test_struct_sizes_match_header_layout():
     assert C.sizeof(_abi.SpConfig) == 320
+    assert C.sizeof(_abi.SpVariant) == 32
     assert C.sizeof(_abi.SpMoverStats) == 56
     assert C.sizeof(_abi.SpLocalDist) == 40
     assert C.sizeof(_abi.SpDistConfig) == 176
