@@ -5,7 +5,11 @@ Workloads:
                 pseudo-particles injected uniformly with p = p0, MHD interval
                 sized for 2000 diffusion-limited steps per particle.
   3d            3D turbulence snapshot on a 512^3 grid with a compression
-                slab, 1e8 particles, 1000 diffusion-limited steps per interval.
+                slab, 1e8 particles, 1000 diffusion-limited steps per interval,
+                particles re-sorted every 200 steps.
+  batch         64 diffusion-parameter variants over the 512^2 2D snapshot in
+                one mover launch: kpara0, kret, momentum exponent and injection
+                region per variant, 1e5 .. 1e7 particles each (1.39e8 total).
 One bench "step" is one MHD interval of the hot path: the mover over every
 particle plus the f(p) and spatial-map histograms.  Inputs are resident in
 HBM before timing starts.
@@ -36,61 +40,98 @@ from sp_hip import ParkerPusher, workloads  # noqa: E402
 
 METRIC = "particle-steps/s (2D/3D snapshot, 1e8 ptcl) at f(p) L1 err ≤1% vs FP64 SDE"
 HBM_PEAK_GBS = 8000.0
-# node components the step consumes x corners of the interpolation cell
+# node components the step consumes x corners of the interpolation cell x 4 B
 BYTES_PER_STEP = {2: 4 * 15 * 4, 3: 8 * 21 * 4}
 COPY_BYTES = 1 << 30
-WORKLOADS = {
-    "2d": dict(grid=2048, nsteps=2000, cpu_nptl=1e6),
-    "3d": dict(grid=512, nsteps=1000, cpu_nptl=2e5),
+DEFAULTS = {
+    "2d": dict(grid=2048, nsteps=2000, nptl=1e8, cpu_nptl=1e6),
+    "3d": dict(grid=512, nsteps=1000, nptl=1e8, cpu_nptl=2e5),
+    "batch": dict(grid=512, nsteps=1000, nptl=0, cpu_nptl=1e6),
 }
-
-
-def traffic_file(kind, grid, nptl):
-    return os.path.join(ROOT, "profiles", f"mover_{kind}_{grid}_{nptl:.0e}_traffic.json"
-                        .replace("+0", ""))
-
-
-def measured_traffic(path, workload: dict):
-    """HBM bytes per mover launch from the committed rocprofv3 counter runs,
-    when they were taken on this workload; None otherwise."""
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if any(t.get("workload", {}).get(k) != v for k, v in workload.items()):
-        return None
-    return t.get("traffic_bytes_per_launch")
+BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+class Workload:
+    """Snapshot, configuration, interval and injection of one workload."""
+
+    def __init__(self, kind, grid, nsteps, nptl):
+        self.kind, self.grid, self.nsteps = kind, grid, nsteps
+        self.variants = None
+        t0 = time.perf_counter()
+        if kind == "3d":
+            self.ndim = 3
+            self.cfg = workloads.config_3d(grid)
+            self.dist = workloads.dist_config_3d(grid)
+            self.dtf = workloads.interval_3d(grid, nsteps)
+            self.nodes = workloads.fields_3d(grid)
+            self.desc = f"3D turbulence snapshot {grid}^3 FP32 fields with a compression slab"
+        else:
+            self.ndim = 2
+            self.cfg = workloads.config_2d(grid)
+            self.dist = workloads.dist_config_2d(grid)
+            self.nodes = workloads.fields_2d(grid, grid)
+            self.desc = f"2D snapshot {grid}x{grid} FP32 fields"
+            if kind == "batch":
+                v = workloads.sweep_variants()
+                if nptl:
+                    v["nptl"] = np.maximum(1, np.round(v["nptl"] * nptl / v["nptl"].sum())).astype(np.int64)
+                self.variants = v
+                self.dtf = workloads.sweep_interval(grid, float(v["kpara0"].max()), nsteps)
+                self.desc = (f"{len(v['nptl'])} diffusion-parameter variants in one launch over the "
+                             f"{self.desc}")
+            else:
+                self.dtf = workloads.interval_2d(grid, nsteps)
+        self.nptl = int(self.variants["nptl"].sum()) if self.variants is not None else int(nptl)
+        log(f"generated {self.desc} in {time.perf_counter() - t0:.1f} s")
+
+    def scaled_counts(self, n):
+        c = self.variants["nptl"].astype(np.float64) * (n / self.variants["nptl"].sum())
+        return np.maximum(1, np.round(c)).astype(np.int64)
+
+    def setup(self, pz, n):
+        """Upload fields, set diagnostics and inject n particles (all variants)."""
+        pz.upload_fields(self.nodes)
+        pz.calc_fields_gradients()
+        pz.init_particle_distributions(**self.dist)
+        if self.variants is not None:
+            v = self.variants
+            pz.set_variants(list(zip(v["kpara0"], v["kret"], v["gamma_turb"])))
+            pz.inject_variants(self.scaled_counts(n), 1e-7, 1, 5.0, 0.0, self.dtf, v["boxes"])
+        else:
+            pz.inject_particles_spatial_uniform(n, 1e-7, 1, 5.0, 0.0, self.dtf, BOX)
+
+    def histograms(self, pz):
+        """(f(p) per variant or global, spatial map or None)."""
+        if self.variants is not None:
+            f, _ = pz.variant_distributions()
+            return f, None
+        fg, maps = pz.calc_particle_distributions()
+        return fg, maps[0].sum(axis=(3, 4))
 
 
-def run_intervals(pz, nptl, dtf, warmup, steps, inject=True):
-    box = BOX
-    if inject:
-        pz.inject_particles_spatial_uniform(nptl, 1e-7, 1, 5.0, 0.0, dtf, box)
+def run_intervals(wl, pz, warmup, steps):
     for w in range(warmup):
-        st = pz.particle_mover(w * dtf, dtf, 1, 10)
-        pz.calc_particle_distributions()
+        st = pz.particle_mover(w * wl.dtf, wl.dtf, 1, 10)
+        wl.histograms(pz)
         log(f"warmup interval {w}: {st['steps']:.3e} steps, kernel {st['kernel_ms']:.1f} ms")
     tot_steps = 0
     kern_ms = 0.0
+    fp = fmap = None
     t0 = time.perf_counter()
     for k in range(steps):
         it = warmup + k
-        st = pz.particle_mover(it * dtf, dtf, 1, 10)
-        fg, maps = pz.calc_particle_distributions()
+        st = pz.particle_mover(it * wl.dtf, wl.dtf, 1, 10)
+        fp, fmap = wl.histograms(pz)
         tot_steps += st["steps"]
         kern_ms += st["kernel_ms"]
         log(f"timed interval {it}: {st['steps']:.3e} steps, kernel {st['kernel_ms']:.1f} ms, "
             f"nptl {st['nptl_current']}")
     t1 = time.perf_counter()
-    return dict(steps=tot_steps, seconds=t1 - t0, kernel_ms=kern_ms, fglobal=fg, maps=maps)
+    return dict(steps=tot_steps, seconds=t1 - t0, kernel_ms=kern_ms, fp=fp, fmap=fmap)
 
 
 def l1(a, b):
@@ -99,40 +140,74 @@ def l1(a, b):
     return float(np.abs(a / a.sum() - b / b.sum()).sum())
 
 
-def accuracy(cfg, nodes, dist, n, dtf, intervals):
+def accuracy(wl, n, intervals):
     """f(p) and map L1 of FP32 vs FP64 on the same particles and increments."""
     out = {}
     res = {}
     for prec in ("f64", "f32"):
-        with ParkerPusher(**dict(cfg, precision=prec, nptl_max=n)) as pz:
-            pz.upload_fields(nodes)
-            pz.calc_fields_gradients()
-            pz.init_particle_distributions(**dist)
-            r = run_intervals(pz, n, dtf, intervals - 1, 1)
+        with ParkerPusher(**dict(wl.cfg, precision=prec, nptl_max=n)) as pz:
+            wl.setup(pz, n)
+            r = run_intervals(wl, pz, intervals - 1, 1)
             res[prec] = r
             out[f"{prec}_steps_per_s"] = r["steps"] / r["seconds"]
-    out["fp_l1"] = l1(res["f32"]["fglobal"], res["f64"]["fglobal"])
-    m32 = res["f32"]["maps"][0].sum(axis=(3, 4))
-    m64 = res["f64"]["maps"][0].sum(axis=(3, 4))
-    out["map_l1"] = l1(m32, m64)
+    f32, f64 = res["f32"]["fp"], res["f64"]["fp"]
+    if wl.variants is not None:
+        per = [l1(a, b) for a, b in zip(f32, f64) if b.sum() > 0]
+        out["fp_l1"] = max(per)
+        out["fp_l1_mean"] = float(np.mean(per))
+        out["map_l1"] = None
+    else:
+        out["fp_l1"] = l1(f32, f64)
+        out["map_l1"] = l1(res["f32"]["fmap"], res["f64"]["fmap"])
     return out
 
 
-def cpu_baseline(cfg, nodes, n, dtf, nthreads):
+def cpu_baseline(wl, n, nthreads):
+    """The FP64 oracle over n particles of the workload for one full interval."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    prm = oracle.params(dict(cfg, seed=cfg.get("seed", 20261202)))
-    fa = oracle.gradients(prm, nodes)
-    box = BOX
-    ptl, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), n, n, 1e-7, 1, 5.0, 0.0, dtf,
-                           box, 7.0)
-    # start every particle at the interval start so the sample does full intervals
-    ptl["t"] = 0.0
+    cfg = dict(wl.cfg, seed=wl.cfg.get("seed", 20261202))
+    prm = oracle.params(cfg)
+    fa = oracle.gradients(prm, wl.nodes)
+    jobs = []
+    if wl.variants is not None:
+        v = wl.variants
+        for k, cnt in enumerate(wl.scaled_counts(n)):
+            pk = oracle.params(dict(cfg, kpara0=v["kpara0"][k], kret=v["kret"][k],
+                                    gamma_turb=v["gamma_turb"][k]))
+            ptl, _ = oracle.inject(pk, np.zeros(0, oracle.PARTICLE_DTYPE), int(cnt), int(cnt),
+                                   1e-7, 1, 5.0, 0.0, wl.dtf, v["boxes"][k], 7.0)
+            jobs.append((pk, ptl))
+    else:
+        ptl, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), n, n, 1e-7, 1, 5.0, 0.0,
+                               wl.dtf, BOX, 7.0)
+        jobs.append((prm, ptl))
+    steps = 0
     t0 = time.perf_counter()
-    _, acc = oracle.mover(prm, fa, ptl, 0.0, dtf, rng_mode=1, nthreads=nthreads)
+    for pk, ptl in jobs:
+        # every particle starts at the interval start: the sample does full intervals
+        ptl["t"] = 0.0
+        _, acc = oracle.mover(pk, fa, ptl, 0.0, wl.dtf, rng_mode=1, nthreads=nthreads)
+        steps += acc.steps
     dt = time.perf_counter() - t0
-    return acc.steps / dt, acc.steps, dt
+    return steps / dt, steps, dt
+
+
+def measured_traffic(wl, n, precision):
+    """HBM bytes per mover launch from the committed rocprofv3 counter runs,
+    when they were taken on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", f"mover_{wl.kind}_{wl.grid}_{n:.0e}_traffic.json"
+                        .replace("+0", ""))
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = dict(grid=wl.grid, nptl=n, precision=precision, nsteps=wl.nsteps)
+    if any(t.get("workload", {}).get(k) != v for k, v in want.items()):
+        return None
+    return t.get("traffic_bytes_per_launch")
 
 
 def main():
@@ -140,9 +215,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed MHD intervals")
     ap.add_argument("--warmup", type=int, default=1, help="untimed MHD intervals")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="2d")
+    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="2d")
     ap.add_argument("--grid", type=int, default=0, help="cells per axis (default per workload)")
-    ap.add_argument("--nptl", type=float, default=1e8)
+    ap.add_argument("--nptl", type=float, default=-1, help="particles (default per workload)")
     ap.add_argument("--nsteps", type=int, default=0,
                     help="diffusion-limited steps per interval (default per workload)")
     ap.add_argument("--precision", choices=("f32", "f64"), default="f32")
@@ -157,37 +232,20 @@ def main():
               "particles and this build measures one GPU's share", file=sys.stderr)
         sys.exit(2)
 
-    n = int(a.nptl)
-    wl = WORKLOADS[a.workload]
-    g = a.grid or wl["grid"]
-    nsteps = a.nsteps or wl["nsteps"]
-    ndim = 3 if a.workload == "3d" else 2
-    t0 = time.perf_counter()
-    if ndim == 2:
-        cfg = workloads.config_2d(g)
-        dist = workloads.dist_config_2d(g)
-        dtf = workloads.interval_2d(g, nsteps)
-        nodes = workloads.fields_2d(g, g)
-        desc = f"2D snapshot {g}x{g} FP32 fields"
-    else:
-        cfg = workloads.config_3d(g)
-        dist = workloads.dist_config_3d(g)
-        dtf = workloads.interval_3d(g, nsteps)
-        nodes = workloads.fields_3d(g)
-        desc = f"3D turbulence snapshot {g}^3 FP32 fields with a compression slab"
-    log(f"generated {desc} in {time.perf_counter() - t0:.1f} s")
-    with ParkerPusher(**dict(cfg, precision=a.precision, nptl_max=n)) as pz:
-        pz.upload_fields(nodes)
-        pz.calc_fields_gradients()
-        pz.init_particle_distributions(**dist)
+    d = DEFAULTS[a.workload]
+    wl = Workload(a.workload, a.grid or d["grid"], a.nsteps or d["nsteps"],
+                  d["nptl"] if a.nptl < 0 else a.nptl)
+    n = wl.nptl
+    with ParkerPusher(**dict(wl.cfg, precision=a.precision, nptl_max=n)) as pz:
+        wl.setup(pz, n)
         # a streaming copy of known size, outside the timed region: the same
         # run then carries the reference point for the memory counters
         copy_ms = pz.calibration_copy(COPY_BYTES)
         log(f"calibration copy {COPY_BYTES} B: {copy_ms:.3f} ms")
-        log(f"injecting {n} particles, dtf={dtf:.5g}")
-        r = run_intervals(pz, n, dtf, a.warmup, a.steps)
+        log(f"{n} particles, dtf={wl.dtf:.5g}")
+        r = run_intervals(wl, pz, a.warmup, a.steps)
     value = r["steps"] / r["seconds"]
-    bytes_per_step = BYTES_PER_STEP[ndim]
+    bytes_per_step = BYTES_PER_STEP[wl.ndim]
     achieved = bytes_per_step * r["steps"] / (r["kernel_ms"] * 1e-3) / 1e9
     result = {
         "metric": METRIC,
@@ -200,40 +258,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if a.precision == "f32" else "f64",
+        "dtype": a.precision,
         "data": "synthetic",
         "config": {
-            "workload": f"{desc}, {n:.0e} particles, "
-                        f"{nsteps} diffusion-limited steps per MHD interval",
-            "grid": [g] * ndim, "nptl": n, "precision": a.precision,
+            "workload": f"{wl.desc}, {n:.3g} particles, "
+                        f"{wl.nsteps} diffusion-limited steps per MHD interval",
+            "grid": [wl.grid] * wl.ndim, "nptl": n, "precision": a.precision,
             "position_time_dtype": "f64", "particle_steps_timed": r["steps"],
             "mover_kernel_ms": r["kernel_ms"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(traffic_file(a.workload, g, n),
-                                        dict(grid=g, nptl=n, precision=a.precision,
-                                             nsteps=nsteps)),
+            "traffic": measured_traffic(wl, n, a.precision),
             "kernel": "mover_kernel", "bytes_per_step": bytes_per_step,
         },
     }
     if not a.no_accuracy:
         na = int(a.accuracy_nptl)
         log(f"accuracy sample: {na} particles, FP32 vs FP64")
-        acc = accuracy(cfg, nodes, dist, na, dtf, a.warmup + 1)
+        acc = accuracy(wl, na, a.warmup + 1)
         result["accuracy"] = dict(sample_nptl=na, fp_l1_vs_f64=acc["fp_l1"],
                                   map_l1_vs_f64=acc["map_l1"],
                                   f64_steps_per_s=acc["f64_steps_per_s"],
                                   f32_steps_per_s=acc["f32_steps_per_s"])
+        if "fp_l1_mean" in acc:
+            result["accuracy"]["fp_l1_mean_over_variants"] = acc["fp_l1_mean"]
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, os.cpu_count() or 1)
-        nc = int(a.cpu_nptl or wl["cpu_nptl"])
+        nc = int(a.cpu_nptl or d["cpu_nptl"])
         log(f"cpu baseline: {nc} particles, {nt} threads")
-        v, s, secs = cpu_baseline(cfg, nodes, nc, dtf, nt)
+        v, s, secs = cpu_baseline(wl, nc, nt)
         result["cpu_baseline"] = {
             "value": v, "unit": "particle-steps/s", "cores": nt, "kind": "port",
-            "sample": f"{nc} particles x one full interval of the same snapshot "
+            "sample": f"{nc} particles x one full interval of the same workload "
                       f"({s} steps, {secs:.1f} s), "
                       "FP64 oracle, AoS particles, 32-component node records, MT19937 per thread",
         }

@@ -193,6 +193,24 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
 int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
                  const double* rands, const int8_t* fixed_dt, double* deltas);
 
+/* ---- launch autotuning (no reference counterpart) -------------------------
+ * Times the mover over [t0, t0 + dtf] from the current particles for every
+ * pair of resident blocks per CU (1 .. occupancy, at most 4) and step budget
+ * between re-sorts (0, 100, 200, 400 when sort_every > 0), keeps the fastest
+ * for this context's later mover calls, and restores the particles and
+ * counters, so the call changes no result.  Pick dtf so that a trial takes a
+ * fraction of a second.  Launch settings never change a particle's result. */
+typedef struct sp_tune_result {
+    int32_t blocks_per_cu;        /* chosen resident mover blocks per CU */
+    int32_t sort_steps;           /* chosen step budget between re-sorts */
+    int32_t ntrials;
+    int32_t reserved0;
+    double steps_per_s;           /* mover throughput of the chosen setting */
+    double default_steps_per_s;   /* mover throughput of the untuned setting */
+} sp_tune_result;
+int sp_autotune_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
+                      int32_t nsteps_interval, sp_tune_result* out);
+
 /* ---- batched parameter sweeps (no reference counterpart) ----------------
  * The reference runs one parameter set per job.  Here one context can push
  * many variants at once: each particle's `origin` holds its variant index and

@@ -70,6 +70,9 @@ struct Ctx {
     unsigned* shist = nullptr;
     unsigned long long* soff = nullptr;
     long long mover_calls = 0;
+    // autotuned launch settings (0 / -1: not tuned) and the tuner's state copy
+    int tuned_bpc = 0, tuned_sort_steps = -1;
+    void* pbuf_save = nullptr;
     // batched parameter variants
     sp::KappaVar* dvar = nullptr;
     int nvar = 0, nvar_cap = 0;
@@ -179,8 +182,9 @@ Ctx* live_ctx(void* p, int& rc) {
     return as_ctx(p);
 }
 
-// Occupancy-sized persistent grid for the mover.
-int mover_grid(Ctx* c, long long n) {
+// Occupancy-sized persistent grid for the mover (bpc > 0: that many blocks
+// per CU, within the occupancy limit).
+int mover_grid(Ctx* c, long long n, int bpc = 0) {
     const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
     // Resident 256-lane blocks per CU.  Particles in flight are, on each XCD,
     // a window of consecutive (spatially sorted) indices; a smaller window
@@ -189,10 +193,12 @@ int mover_grid(Ctx* c, long long n) {
     // both precisions; SP_MOVER_BLOCKS_PER_CU overrides.
     const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D : kMoverBlocksPerCU2D;
     int per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), per_cu_default);
+    if (c->tuned_bpc > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), c->tuned_bpc);
     if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {
         const int v = std::atoi(e);
         if (v > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), v);
     }
+    if (bpc > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), bpc);
     const long long need = (n + 255) / 256;
     const long long cap = (long long)c->ncu * per_cu;
     return (int)std::max<long long>(1, std::min(need, cap));
@@ -342,6 +348,7 @@ void destroy(Ctx* c) {
     for (auto* q : c->dloc) free_dev(q);
     free_dev(c->dvar);
     free_dev(c->vhist);
+    free_dev(c->pbuf_save);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -544,22 +551,12 @@ int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int3
     return SP_OK;
 }
 
-int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
-                      int32_t nsteps_interval, sp_mover_stats* stats) {
-    int rc;
-    Ctx* c = live_ctx(ctx, rc);
-    if (!c) return rc;
-    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
-    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
-    if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
-    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
-    const bool sorting = c->cfg.sort_every > 0;
-    SP_HIP(hipEventRecord(c->ev[0], c->st));
-    if (sorting && c->mover_calls % c->cfg.sort_every == 0) {
-        rc = spatial_sort(c);
-        if (rc != SP_OK) return rc;
-    }
-    ++c->mover_calls;
+// The push of one interval: launches of the persistent mover over the
+// current particles, with re-sorts between launches when a step budget is set.
+// bpc > 0 / sort_steps >= 0 override the launch settings (autotuner trials).
+int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_interval, int bpc,
+              int sort_steps, unsigned long long& steps, float& ms_push) {
+    int rc = SP_OK;
     const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
     // With sort_steps, the interval runs as launches of at most sort_steps
     // steps per particle; between launches the paused particles are sorted
@@ -567,12 +564,15 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     // Once fewer than 1/8 of the particles are still moving, the last launch
     // runs them to the end of the interval: re-sorting a sparse remainder
     // costs more than its locality gains.
-    // SP_SORT_STEPS overrides sort_steps (tuning).
+    // Budget precedence, last wins: the configuration's sort_steps, the
+    // autotuner's choice, SP_SORT_STEPS, an explicit trial value.
     unsigned budget = (unsigned)c->cfg.sort_steps;
+    if (c->tuned_sort_steps >= 0) budget = (unsigned)c->tuned_sort_steps;
     if (const char* e = std::getenv("SP_SORT_STEPS")) budget = (unsigned)std::max(0, std::atoi(e));
-    if (!sorting) budget = 0u;
-    unsigned long long steps = 0;
-    float ms_push = 0.f;
+    if (sort_steps >= 0) budget = (unsigned)sort_steps;
+    if (c->cfg.sort_every <= 0) budget = 0u;
+    steps = 0;
+    ms_push = 0.f;
     long long nrun = c->nptl;
     if (nrun > 0) SP_HIP(hipMemsetAsync(c->P[c->cur].rdt, 0, sizeof(double) * nrun, c->st));
     while (nrun > 0) {
@@ -583,7 +583,7 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
         // clear the work, step and pause counters (leak counters are cumulative)
         SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
         SP_HIP(hipEventRecord(c->ev[1], c->st));
-        const int grid = mover_grid(c, nrun);
+        const int grid = mover_grid(c, nrun, bpc);
         if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
         else SP_HIP(sp::launch_mover_f64(m, ti, acc, c->cfg.ndim, grid, c->st));
         SP_HIP(hipEventRecord(c->ev[3], c->st));
@@ -601,6 +601,29 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
         if (rc != SP_OK) return rc;
         nrun = paused;
     }
+    return SP_OK;
+}
+
+int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
+                      int32_t nsteps_interval, sp_mover_stats* stats) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
+    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
+    if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
+    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    const bool sorting = c->cfg.sort_every > 0;
+    SP_HIP(hipEventRecord(c->ev[0], c->st));
+    if (sorting && c->mover_calls % c->cfg.sort_every == 0) {
+        rc = spatial_sort(c);
+        if (rc != SP_OK) return rc;
+    }
+    ++c->mover_calls;
+    unsigned long long steps = 0;
+    float ms_push = 0.f;
+    rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, 0, -1, steps, ms_push);
+    if (rc != SP_OK) return rc;
     // remove_particles: stable compaction into the other buffer set
     long long kept = 0, esc = 0;
     if (c->nptl > 0) {
@@ -803,6 +826,85 @@ int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal
         std::vector<unsigned long long> hl((size_t)c->loc_size[k]);
         SP_HIP(hipMemcpy(hl.data(), c->dloc[k], sizeof(unsigned long long) * c->loc_size[k], hipMemcpyDeviceToHost));
         for (long long i = 0; i < c->loc_size[k]; ++i) outs[k][i] = (double)hl[i] * kInvFx;
+    }
+    return SP_OK;
+}
+
+int sp_autotune_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
+                      int32_t nsteps_interval, sp_tune_result* out) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
+    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
+    if (num_fine_steps < 1 || nsteps_interval < 1) return fail(SP_ERR_ARG, "num_fine_steps and nsteps_interval must be >= 1");
+    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    if (c->nptl < 1) return fail(SP_ERR_STATE, "no particles to tune on");
+    const long long cap = particle_cap(c->nmax);
+    const size_t bytes = (size_t)sp::kPtlBytes * (size_t)cap;
+    if (!c->pbuf_save) SP_HIP(hipMalloc(&c->pbuf_save, bytes));
+    // trials start from the order the next mover call would push
+    if (c->cfg.sort_every > 0) {
+        rc = spatial_sort(c);
+        if (rc != SP_OK) return rc;
+    }
+    const int cur0 = c->cur;
+    sp::Counters saved{};
+    SP_HIP(hipMemcpyAsync(&saved, c->ctr, sizeof(saved), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipMemcpyAsync(c->pbuf_save, c->pbuf[cur0], bytes, hipMemcpyDeviceToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
+    const int max_bpc = std::max(1, std::min(4, c->mover_blocks_per_cu[prec]));
+    std::vector<int> budgets = {0};
+    if (c->cfg.sort_every > 0) budgets = {0, 100, 200, 400};
+    const int saved_bpc = c->tuned_bpc, saved_ss = c->tuned_sort_steps;
+    c->tuned_bpc = 0;
+    c->tuned_sort_steps = -1;
+    double best = -1.0, dflt = -1.0;
+    int best_bpc = 1, best_ss = 0, ntrials = 0;
+    for (int trial = -1; trial < max_bpc * (int)budgets.size(); ++trial) {
+        // trial -1: the untuned defaults; then every (bpc, budget) pair
+        const int bpc = trial < 0 ? 0 : 1 + trial / (int)budgets.size();
+        const int ss = trial < 0 ? -1 : budgets[(size_t)(trial % (int)budgets.size())];
+        if (trial >= 0) {
+            SP_HIP(hipMemcpyAsync(c->pbuf[cur0], c->pbuf_save, bytes, hipMemcpyDeviceToDevice, c->st));
+            c->cur = cur0;
+        }
+        unsigned long long steps = 0;
+        float ms = 0.f;
+        rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, bpc, ss, steps, ms);
+        if (rc != SP_OK) break;
+        const double rate = ms > 0.f ? (double)steps / (1e-3 * (double)ms) : 0.0;
+        if (trial < 0) {
+            dflt = rate;
+            continue;
+        }
+        ++ntrials;
+        if (rate > best) {
+            best = rate;
+            best_bpc = bpc;
+            best_ss = ss;
+        }
+    }
+    // restore the particles and counters whatever the trials did
+    SP_HIP(hipMemcpyAsync(c->pbuf[cur0], c->pbuf_save, bytes, hipMemcpyDeviceToDevice, c->st));
+    SP_HIP(hipMemcpyAsync(c->ctr, &saved, sizeof(saved), hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    c->cur = cur0;
+    if (rc != SP_OK) {
+        c->tuned_bpc = saved_bpc;
+        c->tuned_sort_steps = saved_ss;
+        return rc;
+    }
+    c->tuned_bpc = best_bpc;
+    c->tuned_sort_steps = c->cfg.sort_every > 0 ? best_ss : -1;
+    if (out) {
+        out->blocks_per_cu = best_bpc;
+        out->sort_steps = best_ss;
+        out->ntrials = ntrials;
+        out->reserved0 = 0;
+        out->steps_per_s = best;
+        out->default_steps_per_s = dflt;
     }
     return SP_OK;
 }

@@ -121,10 +121,13 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     if (m.flags & MF_MOMENTUM_DEP) knorm *= __powf((float)(s.p * m.inv_p0), m.f_pindex);
     const float kpara = m.f_kpara0 * knorm;
     const float kperp = kpara * m.f_kret;
+    // kpara - kperp as a product of non-negative factors: a contracted
+    // kpara - kpara * kret can come out as a negative rounding error when
+    // kret = 1, and its square root as NaN
+    const float kpp = kpara * (1.0f - m.f_kret);
     const float skpara = fsqrt(2.0f * kpara);
     const float skperp = fsqrt(2.0f * kperp);
-    const float skpp = fsqrt(2.0f * (kpara - kperp));
-    const float kpp = kpara - kperp;
+    const float skpp = fsqrt(2.0f * kpp);
 
     float dkxx_dx, dkyy_dy, dkxy_dx, dkxy_dy;
     float dkzz_dz = 0.f, dkxz_dx = 0.f, dkxz_dz = 0.f, dkyz_dy = 0.f, dkyz_dz = 0.f;

@@ -10,5 +10,7 @@ program abi_check
     type(sp_dist_config) :: d
     type(sp_particles) :: p
     type(sp_variant) :: v
-    print "(6I6)", c_sizeof(c), c_sizeof(m), c_sizeof(l), c_sizeof(d), c_sizeof(p), c_sizeof(v)
+    type(sp_tune_result) :: t
+    print "(7I6)", c_sizeof(c), c_sizeof(m), c_sizeof(l), c_sizeof(d), c_sizeof(p), c_sizeof(v), &
+        c_sizeof(t)
 end program abi_check

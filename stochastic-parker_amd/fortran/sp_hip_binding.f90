@@ -8,14 +8,15 @@ module sp_hip_binding
     use iso_c_binding
     implicit none
     private
-    public sp_config, sp_particles, sp_mover_stats, sp_local_dist, sp_dist_config, sp_variant
+    public sp_config, sp_particles, sp_mover_stats, sp_local_dist, sp_dist_config, sp_variant, &
+        sp_tune_result
     public sp_ctx_create, sp_ctx_destroy, sp_last_error, sp_abi_version, &
         sp_fields_upload, sp_fields_gradients, sp_fields_copy, &
         sp_particles_upload, sp_particles_download, sp_particles_counts, &
         sp_inject_particles_spatial_uniform, sp_particle_mover, sp_sort_particles, &
         sp_split_particle, sp_step_once, sp_distributions_init, sp_distributions, &
         sp_local_dist_dims, sp_quick_check, sp_calibration_copy, sp_error_message, &
-        sp_set_variants, sp_inject_variants, sp_variant_distributions
+        sp_set_variants, sp_inject_variants, sp_variant_distributions, sp_autotune_mover
     public SP_OK, SP_PREC_F64, SP_PREC_F32
 
     integer(c_int), parameter :: SP_OK = 0
@@ -63,6 +64,11 @@ module sp_hip_binding
     type, bind(C) :: sp_variant
         real(c_double) :: kpara0, kret, gamma_turb, reserved0
     end type sp_variant
+
+    type, bind(C) :: sp_tune_result
+        integer(c_int32_t) :: blocks_per_cu, sort_steps, ntrials, reserved0
+        real(c_double) :: steps_per_s, default_steps_per_s
+    end type sp_tune_result
 
     type, bind(C) :: sp_dist_config
         integer(c_int32_t) :: npp_global, nmu_global, local_dist, reserved0
@@ -216,6 +222,15 @@ module sp_hip_binding
             integer(c_int64_t), value :: nbytes
             real(c_double), intent(out) :: ms
         end function sp_calibration_copy
+
+        integer(c_int) function sp_autotune_mover(ctx, t0, dtf, num_fine_steps, &
+                nsteps_interval, res) bind(C, name="sp_autotune_mover")
+            import :: c_int, c_int32_t, c_double, c_ptr, sp_tune_result
+            type(c_ptr), value :: ctx
+            real(c_double), value :: t0, dtf
+            integer(c_int32_t), value :: num_fine_steps, nsteps_interval
+            type(sp_tune_result), intent(out) :: res
+        end function sp_autotune_mover
 
         integer(c_int) function sp_set_variants(ctx, nvar, variants) &
                 bind(C, name="sp_set_variants")

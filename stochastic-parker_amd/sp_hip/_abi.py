@@ -86,6 +86,12 @@ class SpVariant(C.Structure):
                 ("reserved0", C.c_double)]
 
 
+class SpTuneResult(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("sort_steps", C.c_int32), ("ntrials", C.c_int32),
+                ("reserved0", C.c_int32), ("steps_per_s", C.c_double),
+                ("default_steps_per_s", C.c_double)]
+
+
 # Particle record members: name -> numpy dtype (order of sp_particles)
 PARTICLE_FIELDS = [
     ("x", np.float64), ("y", np.float64), ("z", np.float64), ("p", np.float64),
@@ -123,6 +129,8 @@ EXPORTS = {
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],
     "sp_quick_check": [_P, _D],
     "sp_calibration_copy": [_P, C.c_int64, _D],
+    "sp_autotune_mover": [_P, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                          C.POINTER(SpTuneResult)],
     "sp_set_variants": [_P, C.c_int32, C.POINTER(SpVariant)],
     "sp_inject_variants": [_P, C.c_int32, C.POINTER(C.c_int64), C.c_double, C.c_int32,
                            C.c_double, C.c_double, C.c_double, _D, C.c_double],

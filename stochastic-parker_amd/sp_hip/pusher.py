@@ -26,8 +26,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (PARTICLE_FIELDS, SpConfig, SpDistConfig, SpMoverStats, SpParticles, SpVariant,
-                   check, dptr)
+from ._abi import (PARTICLE_FIELDS, SpConfig, SpDistConfig, SpMoverStats, SpParticles,
+                   SpTuneResult, SpVariant, check, dptr)
 
 CONFIG_DEFAULTS = dict(
     ndim=2, nx=64, ny=64, nz=1, dx=1.0 / 64, dy=1.0 / 64, dz=1.0,
@@ -160,6 +160,14 @@ class ParkerPusher:
         self._call("sp_particle_mover", float(t0), float(dtf), int(num_fine_steps),
                    int(nsteps_interval), C.byref(st))
         return {k: getattr(st, k) for k, _ in SpMoverStats._fields_}
+
+    def autotune_mover(self, t0, dtf, num_fine_steps=1, nsteps_interval=10) -> dict:
+        """Times mover launch settings over [t0, t0 + dtf] and keeps the
+        fastest; the particles are left as they were."""
+        r = SpTuneResult()
+        self._call("sp_autotune_mover", float(t0), float(dtf), int(num_fine_steps),
+                   int(nsteps_interval), C.byref(r))
+        return {k: getattr(r, k) for k, _ in SpTuneResult._fields_ if k != "reserved0"}
 
     def split_particle(self, split_ratio, pmin_split):
         self._call("sp_split_particle", float(split_ratio), float(pmin_split))

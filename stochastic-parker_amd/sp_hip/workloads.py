@@ -228,3 +228,38 @@ def dist_config_3d(n: int, map_bins: int = 64, npbins1: int = 16) -> dict:
         local=[dict(enabled=1, npbins=npbins1, nmu=1, rx=r, ry=r, rz=r, pmin=1e-2, pmax=1e4),
                None, None, None],
     )
+
+
+# ---------------------------------------------------------------------------
+# Batched parameter sweep over the 512^2 2D snapshot
+#   kpara0 log-uniform in [1e-4, 1e-1], kret log-uniform in [1e-3, 1], the
+#   momentum exponent 3 - gamma_turb uniform in [1, 2], injection in one of
+#   four regions, particle counts log-uniform in [1e5, 1e7] per variant.
+SWEEP_SEED = 20261203
+SWEEP_REGIONS = (
+    (0.0, 0.20, 0.0, 1.0, 0.30, 0.0),   # band around the current sheet at y = 0.25
+    (0.0, 0.70, 0.0, 1.0, 0.80, 0.0),   # band around the current sheet at y = 0.75
+    (0.0, 0.45, 0.0, 1.0, 0.55, 0.0),   # band between the sheets
+    (0.0, 0.00, 0.0, 1.0, 1.00, 0.0),   # the whole box
+)
+
+
+def sweep_variants(nvar: int = 64, nmin: float = 1e5, nmax: float = 1e7,
+                   seed: int = SWEEP_SEED) -> dict:
+    """Parameters of a batched sweep: arrays kpara0, kret, pindex, gamma_turb,
+    region (0..3), nptl, and boxes (nvar, 6)."""
+    rng = np.random.default_rng(seed)
+    kpara0 = 10.0 ** rng.uniform(-4.0, -1.0, nvar)
+    kret = 10.0 ** rng.uniform(-3.0, 0.0, nvar)
+    pindex = rng.uniform(1.0, 2.0, nvar)
+    region = rng.integers(0, len(SWEEP_REGIONS), nvar)
+    nptl = np.round(10.0 ** rng.uniform(np.log10(nmin), np.log10(nmax), nvar)).astype(np.int64)
+    boxes = np.array([SWEEP_REGIONS[r] for r in region], dtype=np.float64)
+    return dict(kpara0=kpara0, kret=kret, pindex=pindex, gamma_turb=3.0 - pindex, region=region,
+                nptl=nptl, boxes=boxes)
+
+
+def sweep_interval(n: int, kpara_max: float, nsteps: int = 1000) -> float:
+    """dtf holding `nsteps` diffusion-limited steps at p0 for the most
+    diffusive variant (the others take dt_max = dt_max_rel dtf steps)."""
+    return interval_2d(n, nsteps, kpara_max)

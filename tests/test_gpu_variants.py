@@ -119,3 +119,48 @@ def test_variant_argument_errors():
     with pytest.raises(SpError):
         pz.inject_variants([30000, 1], 1e-7, 1, 5.0, 0.0, DTF, BOXES[:2])  # capacity
     pz.close()
+
+
+def test_autotune_keeps_results_and_state():
+    """The tuner's trials leave the particles as they were, and the tuned
+    launch settings give the same per-particle results as the defaults."""
+    c = cfg2d(32, pbcy=1, dt_max_rel=1e-2)
+    nodes = nodes_for(c)
+    runs = {}
+    for tune in (False, True):
+        pz, start, _ = _batched(c, nodes, "f64", intervals=0)
+        if tune:
+            before = by_id(pz.get_particles())
+            q0 = pz.quick_check()
+            r = pz.autotune_mover(0.0, 0.05)
+            assert r["ntrials"] >= 4 and 1 <= r["blocks_per_cu"] <= 4
+            assert r["sort_steps"] in (0, 100, 200, 400)
+            assert r["steps_per_s"] > 0 and r["steps_per_s"] >= 0.5 * r["default_steps_per_s"]
+            after = by_id(pz.get_particles())
+            for k in before:
+                np.testing.assert_array_equal(before[k], after[k], err_msg=k)
+            q1 = pz.quick_check()
+            # pdt_avg sums over the particle order, which the tuner's sort changes
+            assert q1.pop("pdt_avg") == pytest.approx(q0.pop("pdt_avg"), rel=1e-14)
+            assert q1 == q0
+        st = pz.particle_mover(0.0, DTF, 1, 10)
+        runs[tune] = (by_id(pz.get_particles()), st["steps"], st["leak"])
+        pz.close()
+    (g0, s0, l0), (g1, s1, l1) = runs[False], runs[True]
+    assert s0 == s1 and l0 == l1
+    for k in g0:
+        np.testing.assert_array_equal(g0[k], g1[k], err_msg=k)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_isotropic_diffusion_kret_one(precision):
+    """kret = 1 (kperp = kpara): the parallel-minus-perpendicular noise term is
+    exactly zero and every momentum stays finite."""
+    c = cfg2d(32, pbcy=1, dt_max_rel=1e-2, kpara0=5e-4, kret=1.0, gamma_turb=2.0)
+    pz = _ctx(c, nodes_for(c), precision, nmax=4000)
+    pz.inject_particles_spatial_uniform(1000, 1e-7, 1, 5.0, 0.0, DTF, (0, 0, 0, 1, 1, 0))
+    st = pz.particle_mover(0.0, DTF, 1, 10)
+    g = pz.get_particles()
+    pz.close()
+    assert np.all(np.isfinite(g["p"])) and np.all(np.isfinite(g["x"]))
+    assert st["steps"] < 200 * 1000  # dt stays near dt_max = dtf / 100
