@@ -230,6 +230,44 @@ int sp_inject_variants(void* ctx, int32_t nvar, const int64_t* nptl, double dt, 
  * number of particles of variant v (NULL skips).  Needs sp_distributions_init. */
 int sp_variant_distributions(void* ctx, int32_t nvar, double* fglobal, int64_t* counts);
 
+/* ---- f(p) surrogate network (no reference counterpart) -------------------
+ * A dense network nin -> nhidden (x nlayers, SiLU) -> nout, trained and run on
+ * the GPU with BF16 matrix-core GEMMs that accumulate in FP32; AdamW keeps
+ * FP32 master weights.  Parameters are one FP32 vector: for each linear layer
+ * l its weights W_l (K_l x N_l, row-major, Z = X W + b) then its bias b_l.
+ * Initial values: U(-1/sqrt(K_l), 1/sqrt(K_l)) from a splitmix64 stream of
+ * `seed`.  Training minimises the mean squared error over all outputs. */
+typedef struct sp_mlp_config {
+    int32_t nin, nhidden, nlayers, nout;
+    int32_t batch_max;          /* rows per device pass (forward splits larger inputs) */
+    int32_t reserved0;          /* must be 0 */
+    uint64_t seed;
+    double beta1, beta2, eps, weight_decay;   /* AdamW (decoupled decay, bias correction) */
+} sp_mlp_config;
+int sp_mlp_create(const sp_mlp_config* cfg, void** out_mlp);
+int sp_mlp_destroy(void* mlp);
+int sp_mlp_num_params(void* mlp, int64_t* nparams);
+int sp_mlp_get_params(void* mlp, float* params);
+/* Sets the parameters and clears the AdamW moments and step count. */
+int sp_mlp_set_params(void* mlp, const float* params);
+/* y (rows x nout) = network(x (rows x nin)), row-major host arrays. */
+int sp_mlp_forward(void* mlp, const float* x, int64_t rows, float* y);
+/* Mean squared error of network(x) against t and its gradient with respect
+ * to every parameter (rows <= batch_max); no update.  grads may be NULL. */
+int sp_mlp_gradients(void* mlp, const float* x, const float* t, int64_t rows, float* grads,
+                     double* loss);
+/* One AdamW step at learning rate lr on the rows given. */
+int sp_mlp_train_step(void* mlp, const float* x, const float* t, int64_t rows, double lr,
+                      double* loss);
+/* Copies a training set (rows x nin inputs, rows x nout targets) to the device. */
+int sp_mlp_set_data(void* mlp, const float* x, const float* t, int64_t rows);
+/* nsteps AdamW steps on device-drawn mini-batches of `batch` rows (with
+ * replacement, Philox keyed on seed and step), learning rate decaying from
+ * lr0 to lr_min on a cosine; losses[s] receives each step's batch loss (NULL
+ * skips).  The loop runs without host synchronisation. */
+int sp_mlp_train(void* mlp, int64_t nsteps, int32_t batch, double lr0, double lr_min,
+                 uint64_t seed, float* losses);
+
 /* ---- diagnostics ------------------------------------------------------ */
 /* init_particle_distributions (diagnostics.f90:178-226) */
 int sp_distributions_init(void* ctx, const sp_dist_config* dc);

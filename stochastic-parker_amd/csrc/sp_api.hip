@@ -27,6 +27,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+int sp::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define SP_HIP(expr)                                                                         \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/sp_hip.h"
 
 namespace sp {
@@ -151,6 +153,9 @@ struct MoverParams {
     double acc_region[6];
     double lx, ly, lz;
 };
+
+// Records the calling thread's error message (sp_last_error) and returns code.
+int set_error(int code, const std::string& msg);
 
 // Launchers (defined in the .hip translation units)
 hipError_t launch_gradients(const DevParams& prm, const float* nodes8, float* rec, float* farray32,

@@ -1,0 +1,623 @@
+// sp_mlp.hip -- the f(p) surrogate: a dense SiLU network trained and run
+// with BF16 matrix-core (MFMA) GEMMs accumulating in FP32, AdamW in FP32.
+//
+// Layer l maps K_l inputs to N_l outputs: Z_l = X_l W_l + b_l, with W_l
+// (K_l x N_l, row-major) and b_l in one FP32 master vector (W_0, b_0, W_1,
+// b_1, ...) and a BF16 copy that the GEMMs read.  Hidden layers store Z_l
+// (FP32) for the backward pass and X_{l+1} = bf16(silu(Z_l)); the last layer
+// is linear.  Rounding points (all round-to-nearest-even to BF16): the
+// network input, every weight, every hidden activation, the output gradient
+// and every back-propagated pre-activation gradient; all products accumulate
+// in FP32.
+//
+// GEMM: 64 x 64 output tiles per 256-lane block, four waves of 32 x 32, K in
+// steps of 32 staged through LDS as k-contiguous BF16 rows, two
+// v_mfma_f32_32x32x16_bf16 per wave per step.  Operand lane layout (checked
+// on gfx950): lane l holds row/column l % 32, k = 8 (l / 32) .. + 7; result
+// register j of lane l is row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sp_device.h"
+#include "sp_internal.h"
+
+namespace sp {
+namespace mlp {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int kTile = 64;
+constexpr int kKT = 32;
+constexpr int kLdsPad = 8;   // row stride 40 halves = 80 B
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+enum Epilogue : int {
+    EPI_HIDDEN = 0,   // z = acc + b: store z (FP32) and bf16(silu(z))
+    EPI_LINEAR = 1,   // y = acc + b: store y (FP32)
+    EPI_BACK = 2,     // g = acc * silu'(zprev): store bf16(g)
+    EPI_PLAIN = 3,    // store acc (FP32)
+};
+
+struct GemmArgs {
+    const uint16_t* A;
+    const uint16_t* B;
+    int M, N, K;
+    int lda, ldb;
+    int transA, transB;      // A(m,k) = transA ? A[k lda + m] : A[m lda + k]; same for B(k,n)
+    const float* bias;
+    const float* zprev;      // EPI_BACK: pre-activations, M x N
+    float* outF;
+    uint16_t* outH;
+};
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+    __shared__ uint16_t As[kTile][kKT + kLdsPad];
+    __shared__ uint16_t Bs[kTile][kKT + kLdsPad];
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    v16f acc = {};
+    for (int k0 = 0; k0 < g.K; k0 += kKT) {
+        // stage A (64 x 32) and B (32 x 64, stored n-major) in LDS; the index
+        // mapping follows the contiguous axis of each operand in memory
+        for (int e = t; e < kTile * kKT; e += 256) {
+            int r, kk;
+            if (g.transA) { r = e % kTile; kk = e / kTile; } else { r = e / kKT; kk = e % kKT; }
+            const int gm = m0 + r, gk = k0 + kk;
+            uint16_t v = 0;
+            if (gm < g.M && gk < g.K)
+                v = g.transA ? g.A[(long long)gk * g.lda + gm] : g.A[(long long)gm * g.lda + gk];
+            As[r][kk] = v;
+        }
+        for (int e = t; e < kTile * kKT; e += 256) {
+            int c, kk;
+            if (g.transB) { c = e / kKT; kk = e % kKT; } else { c = e % kTile; kk = e / kTile; }
+            const int gn = n0 + c, gk = k0 + kk;
+            uint16_t v = 0;
+            if (gn < g.N && gk < g.K)
+                v = g.transB ? g.B[(long long)gn * g.ldb + gk] : g.B[(long long)gk * g.ldb + gn];
+            Bs[c][kk] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kc = 0; kc < kKT; kc += 16) {
+            v8bf a, b;
+            const uint16_t* ap = &As[wm * 32 + (lane & 31)][kc + 8 * (lane >> 5)];
+            const uint16_t* bp = &Bs[wn * 32 + (lane & 31)][kc + 8 * (lane >> 5)];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                a[i] = __builtin_bit_cast(__bf16, ap[i]);
+                b[i] = __builtin_bit_cast(__bf16, bp[i]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int col = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int row = m0 + wm * 32 + (j >> 2) * 8 + (lane >> 5) * 4 + (j & 3);
+        if (row >= g.M || col >= g.N) continue;
+        const long long o = (long long)row * g.N + col;
+        const float v = acc[j];
+        if (EPI == EPI_HIDDEN) {
+            const float z = v + g.bias[col];
+            g.outF[o] = z;
+            g.outH[o] = bf16_rne(z * sigmoid(z));
+        } else if (EPI == EPI_LINEAR) {
+            g.outF[o] = v + g.bias[col];
+        } else if (EPI == EPI_BACK) {
+            const float z = g.zprev[o];
+            const float s = sigmoid(z);
+            g.outH[o] = bf16_rne(v * (s * (1.0f + z * (1.0f - s))));
+        } else {
+            g.outF[o] = v;
+        }
+    }
+}
+
+template <int EPI>
+hipError_t gemm(const GemmArgs& g, hipStream_t s) {
+    const dim3 grid((unsigned)((g.N + kTile - 1) / kTile), (unsigned)((g.M + kTile - 1) / kTile));
+    hipLaunchKernelGGL(gemm_bf16_kernel<EPI>, grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+// ---- elementwise kernels ---------------------------------------------------
+__global__ void to_bf16_kernel(const float* in, uint16_t* out, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = bf16_rne(in[i]);
+}
+
+// Rows idx[r] of the data set (nin and nout columns) into the batch buffers.
+__global__ void gather_kernel(const float* xd, const float* td, const uint32_t* idx, int nrow,
+                              int nin, int nout, uint16_t* xb, float* tb) {
+    const int r = blockIdx.x;
+    if (r >= nrow) return;
+    const long long s = idx[r];
+    for (int c = threadIdx.x; c < nin; c += blockDim.x) xb[(long long)r * nin + c] = bf16_rne(xd[s * nin + c]);
+    for (int c = threadIdx.x; c < nout; c += blockDim.x) tb[(long long)r * nout + c] = td[s * nout + c];
+}
+
+// Mini-batch rows drawn with replacement: Philox(step, r) mod ndata.
+__global__ void batch_index_kernel(uint32_t* idx, int nrow, uint32_t ndata, uint32_t step,
+                                   uint32_t k0, uint32_t k1) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const U4 u = philox4x32_10((uint32_t)r, step, 0x6d6c70u, 0u, k0, k1);
+    idx[r] = (uint32_t)(((unsigned long long)u.a * ndata) >> 32);
+}
+
+// Mean squared error over n x nout: per-block partial sums of e^2 and the
+// output gradient 2 e / (n nout) rounded to BF16.
+__global__ void __launch_bounds__(256) mse_kernel(const float* y, const float* t, long long count,
+                                                  float scale, uint16_t* g, float* part) {
+    __shared__ float red[256];
+    float acc = 0.0f;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < count; i += (long long)gridDim.x * 256) {
+        const float e = y[i] - t[i];
+        acc += e * e;
+        g[i] = bf16_rne(2.0f * e * scale);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// out[0] = scale * sum(part[0 .. np)), one block.
+__global__ void __launch_bounds__(256) sum_scale_kernel(const float* part, int np, float scale,
+                                                        float* out) {
+    __shared__ float red[256];
+    float acc = 0.0f;
+    for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+__global__ void zero_kernel(float* a, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0f;
+}
+
+// Column sums of a BF16 M x N matrix (bias gradients), one block per column.
+__global__ void __launch_bounds__(256) colsum_kernel(const uint16_t* gmat, int M, int N, float* out) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float acc = 0.0f;
+    for (int r = threadIdx.x; r < M; r += 256) acc += bf16_to_f(gmat[(long long)r * N + c]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// AdamW (decoupled weight decay) with bias correction; refreshes the BF16 copy.
+__global__ void adamw_kernel(float* p, const float* gr, float* m, float* v, uint16_t* pb,
+                             long long n, float lr, float b1, float b2, float eps, float wd,
+                             float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float pi = p[i] - lr * (upd + wd * p[i]);
+    p[i] = pi;
+    pb[i] = bf16_rne(pi);
+}
+
+}  // namespace mlp
+}  // namespace sp
+
+// ---------------------------------------------------------------------------
+// Host side: the network context and the extern "C" entry points.
+namespace {
+
+struct Mlp {
+    sp_mlp_config cfg{};
+    hipStream_t st = nullptr;
+    int L = 0;                          // linear layers
+    std::vector<int> K, N;              // inputs / outputs of layer l
+    std::vector<long long> woff, boff;  // offsets of W_l, b_l in the parameter vector
+    long long P = 0;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t* pb = nullptr;
+    int bmax = 0, nmax = 0;
+    std::vector<uint16_t*> xh;          // X_l (bf16), l = 0 .. L-1
+    std::vector<float*> z;              // Z_l (fp32), l = 0 .. L-2
+    float* y = nullptr;                 // network output (fp32)
+    float* t = nullptr;                 // targets of the batch (fp32)
+    uint16_t* gh[2] = {nullptr, nullptr};
+    float* part = nullptr;
+    float* loss_hist = nullptr;
+    long long loss_cap = 0;
+    float* xin = nullptr;               // host-supplied input rows (fp32)
+    float* xd = nullptr;                // training data set on the device
+    float* td = nullptr;
+    long long nd = 0;
+    uint32_t* bidx = nullptr;
+    long long step = 0;                 // AdamW step count
+};
+
+constexpr int kPartBlocks = 256;
+
+void dfree(void* q) {
+    if (q) (void)hipFree(q);
+}
+
+void destroy(Mlp* n) {
+    if (!n) return;
+    if (n->st) (void)hipStreamSynchronize(n->st);
+    dfree(n->p); dfree(n->g); dfree(n->m); dfree(n->v); dfree(n->pb);
+    for (auto* q : n->xh) dfree(q);
+    for (auto* q : n->z) dfree(q);
+    dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
+    dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
+    if (n->st) (void)hipStreamDestroy(n->st);
+    delete n;
+}
+
+#define MLP_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return sp::set_error(SP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Deterministic initial parameters: U(-1/sqrt(K), 1/sqrt(K)) for W_l and b_l
+// from a splitmix64 stream of the configured seed.
+void init_params(const Mlp* n, std::vector<float>& h) {
+    uint64_t s = n->cfg.seed;
+    auto next = [&s]() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    h.assign((size_t)n->P, 0.0f);
+    for (int l = 0; l < n->L; ++l) {
+        const double a = 1.0 / std::sqrt((double)n->K[l]);
+        const long long nw = (long long)n->K[l] * n->N[l];
+        for (long long i = 0; i < nw; ++i) h[(size_t)(n->woff[l] + i)] = (float)((2.0 * next() - 1.0) * a);
+        for (int i = 0; i < n->N[l]; ++i) h[(size_t)(n->boff[l] + i)] = (float)((2.0 * next() - 1.0) * a);
+    }
+}
+
+unsigned blocks(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// Forward pass of rows [0, rows) of X_0 (already in xh[0]).
+int forward(Mlp* n, int rows) {
+    for (int l = 0; l < n->L; ++l) {
+        sp::mlp::GemmArgs g{};
+        g.A = n->xh[l];
+        g.lda = n->K[l];
+        g.B = n->pb + n->woff[l];
+        g.ldb = n->N[l];
+        g.M = rows; g.N = n->N[l]; g.K = n->K[l];
+        g.bias = n->p + n->boff[l];
+        if (l + 1 < n->L) {
+            g.outF = n->z[l];
+            g.outH = n->xh[l + 1];
+            MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_HIDDEN>(g, n->st));
+        } else {
+            g.outF = n->y;
+            MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_LINEAR>(g, n->st));
+        }
+    }
+    return SP_OK;
+}
+
+// Loss, gradients of all parameters into n->g; loss_out (device) receives the
+// mean squared error.
+int backward(Mlp* n, int rows, float* loss_out) {
+    const long long cnt = (long long)rows * n->cfg.nout;
+    const float scale = 1.0f / (float)cnt;
+    hipLaunchKernelGGL(sp::mlp::mse_kernel, dim3(kPartBlocks), dim3(256), 0, n->st, n->y, n->t, cnt,
+                       scale, n->gh[0], n->part);
+    MLP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sp::mlp::sum_scale_kernel, dim3(1), dim3(256), 0, n->st, n->part,
+                       kPartBlocks, scale, loss_out);
+    MLP_HIP(hipGetLastError());
+    int cur = 0;
+    for (int l = n->L - 1; l >= 0; --l) {
+        sp::mlp::GemmArgs w{};
+        w.A = n->xh[l]; w.lda = n->K[l]; w.transA = 1;
+        w.B = n->gh[cur]; w.ldb = n->N[l];
+        w.M = n->K[l]; w.N = n->N[l]; w.K = rows;
+        w.outF = n->g + n->woff[l];
+        MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_PLAIN>(w, n->st));
+        hipLaunchKernelGGL(sp::mlp::colsum_kernel, dim3((unsigned)n->N[l]), dim3(256), 0, n->st,
+                           n->gh[cur], rows, n->N[l], n->g + n->boff[l]);
+        MLP_HIP(hipGetLastError());
+        if (l > 0) {
+            sp::mlp::GemmArgs d{};
+            d.A = n->gh[cur]; d.lda = n->N[l];
+            d.B = n->pb + n->woff[l]; d.ldb = n->N[l]; d.transB = 1;
+            d.M = rows; d.N = n->K[l]; d.K = n->N[l];
+            d.zprev = n->z[l - 1];
+            d.outH = n->gh[1 - cur];
+            MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_BACK>(d, n->st));
+            cur = 1 - cur;
+        }
+    }
+    return SP_OK;
+}
+
+int adamw(Mlp* n, double lr) {
+    n->step += 1;
+    const double c1 = 1.0 - std::pow(n->cfg.beta1, (double)n->step);
+    const double c2 = 1.0 - std::pow(n->cfg.beta2, (double)n->step);
+    hipLaunchKernelGGL(sp::mlp::adamw_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->p, n->g,
+                       n->m, n->v, n->pb, n->P, (float)lr, (float)n->cfg.beta1,
+                       (float)n->cfg.beta2, (float)n->cfg.eps, (float)n->cfg.weight_decay,
+                       (float)c1, (float)c2);
+    MLP_HIP(hipGetLastError());
+    return SP_OK;
+}
+
+// Host rows -> xh[0] (bf16) and, when given, targets -> n->t.
+int load_rows(Mlp* n, const float* x, const float* t, int rows) {
+    const long long nx = (long long)rows * n->cfg.nin;
+    MLP_HIP(hipMemcpyAsync(n->xin, x, sizeof(float) * (size_t)nx, hipMemcpyHostToDevice, n->st));
+    hipLaunchKernelGGL(sp::mlp::to_bf16_kernel, dim3(blocks(nx)), dim3(256), 0, n->st, n->xin,
+                       n->xh[0], nx);
+    MLP_HIP(hipGetLastError());
+    if (t)
+        MLP_HIP(hipMemcpyAsync(n->t, t, sizeof(float) * (size_t)rows * n->cfg.nout,
+                               hipMemcpyHostToDevice, n->st));
+    return SP_OK;
+}
+
+Mlp* live(void* q, int& rc) {
+    if (!q) {
+        rc = sp::set_error(SP_ERR_ARG, "network is NULL");
+        return nullptr;
+    }
+    rc = SP_OK;
+    return static_cast<Mlp*>(q);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sp_mlp_create(const sp_mlp_config* cfg, void** out) {
+    if (!cfg || !out) return sp::set_error(SP_ERR_ARG, "cfg or out is NULL");
+    *out = nullptr;
+    if (cfg->nin < 1 || cfg->nin > 4096 || cfg->nout < 1 || cfg->nout > 4096 || cfg->nhidden < 1 ||
+        cfg->nhidden > 4096 || cfg->nlayers < 0 || cfg->nlayers > 64)
+        return sp::set_error(SP_ERR_ARG, "layer sizes out of range");
+    if (cfg->batch_max < 1 || cfg->batch_max > (1 << 21))
+        return sp::set_error(SP_ERR_ARG, "batch_max must be in [1, 2^21]");
+    if (cfg->reserved0 != 0) return sp::set_error(SP_ERR_ARG, "reserved0 must be 0");
+    if (!(cfg->beta1 >= 0 && cfg->beta1 < 1 && cfg->beta2 >= 0 && cfg->beta2 < 1 && cfg->eps > 0 &&
+          cfg->weight_decay >= 0))
+        return sp::set_error(SP_ERR_ARG, "AdamW parameters out of range");
+    Mlp* n = new Mlp();
+    n->cfg = *cfg;
+    n->L = cfg->nlayers + 1;
+    for (int l = 0; l < n->L; ++l) {
+        n->K.push_back(l == 0 ? cfg->nin : cfg->nhidden);
+        n->N.push_back(l + 1 == n->L ? cfg->nout : cfg->nhidden);
+    }
+    for (int l = 0; l < n->L; ++l) {
+        n->woff.push_back(n->P);
+        n->P += (long long)n->K[l] * n->N[l];
+        n->boff.push_back(n->P);
+        n->P += n->N[l];
+    }
+    n->bmax = cfg->batch_max;
+    int wmax = std::max(cfg->nin, std::max(cfg->nhidden, cfg->nout));
+    int rc = SP_OK;
+    auto fail_all = [&](hipError_t e, const char* what) {
+        rc = sp::set_error(SP_ERR_NOMEM, std::string(what) + ": " + hipGetErrorString(e));
+        destroy(n);
+        return rc;
+    };
+    hipError_t e = hipStreamCreate(&n->st);
+    if (e != hipSuccess) return fail_all(e, "hipStreamCreate");
+    const size_t P = (size_t)n->P, B = (size_t)n->bmax;
+    e = hipMalloc(&n->p, 4 * P);
+    if (e == hipSuccess) e = hipMalloc(&n->g, 4 * P);
+    if (e == hipSuccess) e = hipMalloc(&n->m, 4 * P);
+    if (e == hipSuccess) e = hipMalloc(&n->v, 4 * P);
+    if (e == hipSuccess) e = hipMalloc(&n->pb, 2 * P);
+    n->xh.assign((size_t)n->L, nullptr);
+    n->z.assign((size_t)std::max(0, n->L - 1), nullptr);
+    for (int l = 0; l < n->L && e == hipSuccess; ++l) e = hipMalloc(&n->xh[(size_t)l], 2 * B * (size_t)n->K[l]);
+    for (int l = 0; l + 1 < n->L && e == hipSuccess; ++l) e = hipMalloc(&n->z[(size_t)l], 4 * B * (size_t)n->N[l]);
+    if (e == hipSuccess) e = hipMalloc(&n->y, 4 * B * (size_t)cfg->nout);
+    if (e == hipSuccess) e = hipMalloc(&n->t, 4 * B * (size_t)cfg->nout);
+    if (e == hipSuccess) e = hipMalloc(&n->gh[0], 2 * B * (size_t)wmax);
+    if (e == hipSuccess) e = hipMalloc(&n->gh[1], 2 * B * (size_t)wmax);
+    if (e == hipSuccess) e = hipMalloc(&n->part, 4 * (size_t)kPartBlocks);
+    if (e == hipSuccess) e = hipMalloc(&n->xin, 4 * B * (size_t)cfg->nin);
+    if (e == hipSuccess) e = hipMalloc(&n->bidx, 4 * B);
+    if (e == hipSuccess) e = hipMalloc(&n->loss_hist, 4 * 16);
+    if (e != hipSuccess) return fail_all(e, "device allocation for the network");
+    n->loss_cap = 16;
+    std::vector<float> h;
+    init_params(n, h);
+    rc = sp_mlp_set_params(n, h.data());
+    if (rc != SP_OK) {
+        destroy(n);
+        return rc;
+    }
+    *out = n;
+    return SP_OK;
+}
+
+int sp_mlp_destroy(void* q) {
+    destroy(static_cast<Mlp*>(q));
+    return SP_OK;
+}
+
+int sp_mlp_num_params(void* q, int64_t* np) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (!np) return sp::set_error(SP_ERR_ARG, "np is NULL");
+    *np = n->P;
+    return SP_OK;
+}
+
+int sp_mlp_get_params(void* q, float* out) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (!out) return sp::set_error(SP_ERR_ARG, "out is NULL");
+    MLP_HIP(hipMemcpyAsync(out, n->p, 4 * (size_t)n->P, hipMemcpyDeviceToHost, n->st));
+    MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+int sp_mlp_set_params(void* q, const float* in) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (!in) return sp::set_error(SP_ERR_ARG, "in is NULL");
+    MLP_HIP(hipMemcpyAsync(n->p, in, 4 * (size_t)n->P, hipMemcpyHostToDevice, n->st));
+    hipLaunchKernelGGL(sp::mlp::to_bf16_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->p, n->pb,
+                       n->P);
+    MLP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sp::mlp::zero_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->m, n->P);
+    hipLaunchKernelGGL(sp::mlp::zero_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->v, n->P);
+    MLP_HIP(hipGetLastError());
+    MLP_HIP(hipStreamSynchronize(n->st));
+    n->step = 0;
+    return SP_OK;
+}
+
+int sp_mlp_forward(void* q, const float* x, int64_t rows, float* y) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (rows < 0 || (rows > 0 && (!x || !y))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
+    for (int64_t r0 = 0; r0 < rows; r0 += n->bmax) {
+        const int r = (int)std::min<int64_t>(n->bmax, rows - r0);
+        rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
+        if (rc != SP_OK) return rc;
+        rc = forward(n, r);
+        if (rc != SP_OK) return rc;
+        MLP_HIP(hipMemcpyAsync(y + r0 * n->cfg.nout, n->y, 4 * (size_t)r * n->cfg.nout,
+                               hipMemcpyDeviceToHost, n->st));
+    }
+    MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+int sp_mlp_gradients(void* q, const float* x, const float* t, int64_t rows, float* grads, double* loss) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (rows < 1 || rows > n->bmax || !x || !t) return sp::set_error(SP_ERR_ARG, "rows must be in [1, batch_max] with x and t given");
+    rc = load_rows(n, x, t, (int)rows);
+    if (rc == SP_OK) rc = forward(n, (int)rows);
+    if (rc == SP_OK) rc = backward(n, (int)rows, n->loss_hist);
+    if (rc != SP_OK) return rc;
+    float l = 0.0f;
+    MLP_HIP(hipMemcpyAsync(&l, n->loss_hist, 4, hipMemcpyDeviceToHost, n->st));
+    if (grads) MLP_HIP(hipMemcpyAsync(grads, n->g, 4 * (size_t)n->P, hipMemcpyDeviceToHost, n->st));
+    MLP_HIP(hipStreamSynchronize(n->st));
+    if (loss) *loss = l;
+    return SP_OK;
+}
+
+int sp_mlp_train_step(void* q, const float* x, const float* t, int64_t rows, double lr, double* loss) {
+    int rc = sp_mlp_gradients(q, x, t, rows, nullptr, loss);
+    if (rc != SP_OK) return rc;
+    Mlp* n = static_cast<Mlp*>(q);
+    rc = adamw(n, lr);
+    if (rc != SP_OK) return rc;
+    MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+int sp_mlp_set_data(void* q, const float* x, const float* t, int64_t rows) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (rows < 1 || rows > 0xFFFFFFFFll || !x || !t) return sp::set_error(SP_ERR_ARG, "bad rows or NULL data");
+    dfree(n->xd);
+    dfree(n->td);
+    n->xd = nullptr;
+    n->td = nullptr;
+    n->nd = 0;
+    MLP_HIP(hipMalloc(&n->xd, 4 * (size_t)rows * n->cfg.nin));
+    MLP_HIP(hipMalloc(&n->td, 4 * (size_t)rows * n->cfg.nout));
+    MLP_HIP(hipMemcpyAsync(n->xd, x, 4 * (size_t)rows * n->cfg.nin, hipMemcpyHostToDevice, n->st));
+    MLP_HIP(hipMemcpyAsync(n->td, t, 4 * (size_t)rows * n->cfg.nout, hipMemcpyHostToDevice, n->st));
+    MLP_HIP(hipStreamSynchronize(n->st));
+    n->nd = rows;
+    return SP_OK;
+}
+
+int sp_mlp_train(void* q, int64_t nsteps, int32_t batch, double lr0, double lr_min, uint64_t seed,
+                 float* losses) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (n->nd < 1) return sp::set_error(SP_ERR_STATE, "no training data (sp_mlp_set_data)");
+    if (nsteps < 0 || batch < 1 || batch > n->bmax) return sp::set_error(SP_ERR_ARG, "need nsteps >= 0 and batch in [1, batch_max]");
+    if (!(lr0 > 0) || !(lr_min >= 0)) return sp::set_error(SP_ERR_ARG, "learning rates must be positive");
+    if (nsteps > n->loss_cap) {
+        dfree(n->loss_hist);
+        n->loss_hist = nullptr;
+        n->loss_cap = 0;
+        MLP_HIP(hipMalloc(&n->loss_hist, 4 * (size_t)nsteps));
+        n->loss_cap = nsteps;
+    }
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        hipLaunchKernelGGL(sp::mlp::batch_index_kernel, dim3(blocks(batch)), dim3(256), 0, n->st,
+                           n->bidx, batch, (uint32_t)n->nd, (uint32_t)s, k0, k1);
+        hipLaunchKernelGGL(sp::mlp::gather_kernel, dim3((unsigned)batch), dim3(64), 0, n->st, n->xd,
+                           n->td, n->bidx, batch, n->cfg.nin, n->cfg.nout, n->xh[0], n->t);
+        MLP_HIP(hipGetLastError());
+        rc = forward(n, batch);
+        if (rc == SP_OK) rc = backward(n, batch, n->loss_hist + s);
+        // cosine decay from lr0 to lr_min
+        const double f = nsteps > 1 ? (double)s / (double)(nsteps - 1) : 0.0;
+        const double lr = lr_min + 0.5 * (lr0 - lr_min) * (1.0 + std::cos(3.141592653589793 * f));
+        if (rc == SP_OK) rc = adamw(n, lr);
+        if (rc != SP_OK) return rc;
+    }
+    if (losses && nsteps > 0)
+        MLP_HIP(hipMemcpyAsync(losses, n->loss_hist, 4 * (size_t)nsteps, hipMemcpyDeviceToHost, n->st));
+    MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+}  // extern "C"

@@ -92,6 +92,13 @@ class SpTuneResult(C.Structure):
                 ("default_steps_per_s", C.c_double)]
 
 
+class SpMlpConfig(C.Structure):
+    _fields_ = [("nin", C.c_int32), ("nhidden", C.c_int32), ("nlayers", C.c_int32),
+                ("nout", C.c_int32), ("batch_max", C.c_int32), ("reserved0", C.c_int32),
+                ("seed", C.c_uint64), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("weight_decay", C.c_double)]
+
+
 # Particle record members: name -> numpy dtype (order of sp_particles)
 PARTICLE_FIELDS = [
     ("x", np.float64), ("y", np.float64), ("z", np.float64), ("p", np.float64),
@@ -103,6 +110,7 @@ PARTICLE_FIELDS = [
 
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
 
 # name -> (argtypes)
 EXPORTS = {
@@ -135,6 +143,16 @@ EXPORTS = {
     "sp_inject_variants": [_P, C.c_int32, C.POINTER(C.c_int64), C.c_double, C.c_int32,
                            C.c_double, C.c_double, C.c_double, _D, C.c_double],
     "sp_variant_distributions": [_P, C.c_int32, _D, C.POINTER(C.c_int64)],
+    "sp_mlp_create": [C.POINTER(SpMlpConfig), C.POINTER(C.c_void_p)],
+    "sp_mlp_destroy": [_P],
+    "sp_mlp_num_params": [_P, C.POINTER(C.c_int64)],
+    "sp_mlp_get_params": [_P, _F],
+    "sp_mlp_set_params": [_P, _F],
+    "sp_mlp_forward": [_P, _F, C.c_int64, _F],
+    "sp_mlp_gradients": [_P, _F, _F, C.c_int64, _F, _D],
+    "sp_mlp_train_step": [_P, _F, _F, C.c_int64, C.c_double, _D],
+    "sp_mlp_set_data": [_P, _F, _F, C.c_int64],
+    "sp_mlp_train": [_P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_uint64, _F],
 }
 
 _lib = None

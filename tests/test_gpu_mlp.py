@@ -1,0 +1,79 @@
+"""The surrogate network's BF16 MFMA kernels against the CPU oracle
+(oracle/mlp_oracle.py, which rounds to BF16 at the same points)."""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+from sp_hip.surrogate import Mlp
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (6, 256, 4, 64)
+
+
+def _data(n, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.0, 1.0, (n, SHAPE[0])).astype(np.float32)
+    t = (np.sin(x @ rng.standard_normal((SHAPE[0], SHAPE[3]))) * 0.5).astype(np.float32)
+    return x, t
+
+
+def _net(**kw):
+    return Mlp(*SHAPE, **kw)
+
+
+def test_forward_matches_oracle():
+    x, _ = _data(300)
+    with _net(seed=3) as net:
+        y = net.forward(x)
+        p = net.get_params()
+    yr, _ = mo.forward(p, x, SHAPE, bf=True)
+    err = np.abs(y - yr)
+    assert err.max() < 2e-3 * np.abs(yr).max()
+    assert np.median(err) < 1e-4 * np.abs(yr).max()
+
+
+def test_forward_splits_rows_beyond_batch_max():
+    x, _ = _data(1000, seed=1)
+    with _net(seed=4, batch_max=128) as a, _net(seed=4, batch_max=4096) as b:
+        np.testing.assert_array_equal(a.forward(x), b.forward(x))
+
+
+def test_gradients_match_oracle():
+    x, t = _data(500, seed=2)
+    with _net(seed=5) as net:
+        p = net.get_params()
+        loss, g = net.gradients(x, t)
+    lr, gr = mo.loss_and_grads(p, x, t, SHAPE, bf=True)
+    assert loss == pytest.approx(lr, rel=1e-4)
+    layers = mo.unpack(np.arange(len(p)), *SHAPE)
+    for w, b in layers:
+        for idx in (w.ravel(), b):
+            num = np.linalg.norm(g[idx] - gr[idx])
+            assert num < 1e-3 * np.linalg.norm(gr[idx]) + 1e-9
+    # and the rounded network's gradient is the exact one to BF16 precision
+    _, g_exact = mo.loss_and_grads(p, x, t, SHAPE, bf=False)
+    assert np.dot(g, g_exact) / (np.linalg.norm(g) * np.linalg.norm(g_exact)) > 0.999
+
+
+def test_adamw_step_matches_oracle():
+    x, t = _data(256, seed=3)
+    with _net(seed=6, weight_decay=0.01) as net:
+        p0 = net.get_params().astype(np.float64)
+        _, g = net.gradients(x, t)
+        net.train_step(x, t, 1e-3)
+        p1 = net.get_params()
+    ref, _, _ = mo.adamw(p0, g.astype(np.float64), 0.0, 0.0, 1, 1e-3, 0.9, 0.999, 1e-8, 0.01)
+    np.testing.assert_allclose(p1, ref, rtol=1e-6, atol=1e-8)
+
+
+def test_training_fits_and_is_deterministic():
+    x, t = _data(20000, seed=4)
+    finals = []
+    for _ in range(2):
+        with _net(seed=8) as net:
+            net.set_data(x, t)
+            losses = net.train(400, batch=2048, lr0=2e-3, lr_min=1e-4, seed=11)
+            finals.append(net.get_params())
+    assert losses[-20:].mean() < 0.1 * losses[:5].mean()
+    np.testing.assert_array_equal(finals[0], finals[1])
