@@ -259,6 +259,11 @@ int sp_mlp_gradients(void* mlp, const float* x, const float* t, int64_t rows, fl
 /* One AdamW step at learning rate lr on the rows given. */
 int sp_mlp_train_step(void* mlp, const float* x, const float* t, int64_t rows, double lr,
                       double* loss);
+/* Ensemble of networks with equal nin, nout whose outputs are log10(f + floor):
+ * spec (rows x nout) = mean over members of max(10^y - floor, 0); spread
+ * (rows) = mean over members of the L1 distance of their spectrum from spec. */
+int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, int64_t rows,
+                            double floor, float* spec, float* spread);
 /* Copies a training set (rows x nin inputs, rows x nout targets) to the device. */
 int sp_mlp_set_data(void* mlp, const float* x, const float* t, int64_t rows);
 /* nsteps AdamW steps on device-drawn mini-batches of `batch` rows (with

@@ -56,6 +56,7 @@ struct GemmArgs {
     const uint16_t* A;
     const uint16_t* B;
     int M, N, K;
+    int kchunk;              // K range of one blockIdx.z (split K); 0: all of K
     int lda, ldb;
     int transA, transB;      // A(m,k) = transA ? A[k lda + m] : A[m lda + k]; same for B(k,n)
     const float* bias;
@@ -72,8 +73,11 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     const int lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    // split K: block z covers [kb, ke) and writes its own partial (EPI_PLAIN)
+    const int kb = g.kchunk ? (int)blockIdx.z * g.kchunk : 0;
+    const int ke = g.kchunk ? min(g.K, kb + g.kchunk) : g.K;
     v16f acc = {};
-    for (int k0 = 0; k0 < g.K; k0 += kKT) {
+    for (int k0 = kb; k0 < ke; k0 += kKT) {
         // stage A (64 x 32) and B (32 x 64, stored n-major) in LDS; the index
         // mapping follows the contiguous axis of each operand in memory
         for (int e = t; e < kTile * kKT; e += 256) {
@@ -81,7 +85,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
             if (g.transA) { r = e % kTile; kk = e / kTile; } else { r = e / kKT; kk = e % kKT; }
             const int gm = m0 + r, gk = k0 + kk;
             uint16_t v = 0;
-            if (gm < g.M && gk < g.K)
+            if (gm < g.M && gk < ke)
                 v = g.transA ? g.A[(long long)gk * g.lda + gm] : g.A[(long long)gm * g.lda + gk];
             As[r][kk] = v;
         }
@@ -90,7 +94,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
             if (g.transB) { c = e / kKT; kk = e % kKT; } else { c = e % kTile; kk = e / kTile; }
             const int gn = n0 + c, gk = k0 + kk;
             uint16_t v = 0;
-            if (gn < g.N && gk < g.K)
+            if (gn < g.N && gk < ke)
                 v = g.transB ? g.B[(long long)gn * g.ldb + gk] : g.B[(long long)gk * g.ldb + gn];
             Bs[c][kk] = v;
         }
@@ -127,16 +131,26 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
             const float s = sigmoid(z);
             g.outH[o] = bf16_rne(v * (s * (1.0f + z * (1.0f - s))));
         } else {
-            g.outF[o] = v;
+            g.outF[(long long)blockIdx.z * g.M * g.N + o] = v;
         }
     }
 }
 
 template <int EPI>
-hipError_t gemm(const GemmArgs& g, hipStream_t s) {
-    const dim3 grid((unsigned)((g.N + kTile - 1) / kTile), (unsigned)((g.M + kTile - 1) / kTile));
+hipError_t gemm(const GemmArgs& g, hipStream_t s, int splits = 1) {
+    const dim3 grid((unsigned)((g.N + kTile - 1) / kTile), (unsigned)((g.M + kTile - 1) / kTile),
+                    (unsigned)splits);
     hipLaunchKernelGGL(gemm_bf16_kernel<EPI>, grid, dim3(256), 0, s, g);
     return hipGetLastError();
+}
+
+// out[i] = sum over s < splits of part[s n + i], in order of s (deterministic).
+__global__ void split_sum_kernel(const float* part, int splits, long long n, float* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int s = 0; s < splits; ++s) acc += part[(long long)s * n + i];
+    out[i] = acc;
 }
 
 // ---- elementwise kernels ---------------------------------------------------
@@ -182,6 +196,36 @@ __global__ void __launch_bounds__(256) mse_kernel(const float* y, const float* t
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Ensemble output transform, one thread per row: each member's spectrum
+// s_i = max(10^y_i - floor, 0), their mean, and the mean L1 distance of the
+// members from it.
+struct EnsembleArgs {
+    const float* y[8];
+    int members, nout;
+    float floor;
+    float* spec;
+    float* spread;
+};
+
+__global__ void ensemble_kernel(EnsembleArgs e, int rows) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float inv = 1.0f / (float)e.members;
+    float dist = 0.0f;
+    for (int c = 0; c < e.nout; ++c) {
+        float s[8];
+        float mean = 0.0f;
+        for (int i = 0; i < e.members; ++i) {
+            s[i] = fmaxf(exp10f(e.y[i][(long long)r * e.nout + c]) - e.floor, 0.0f);
+            mean += s[i];
+        }
+        mean *= inv;
+        for (int i = 0; i < e.members; ++i) dist += fabsf(s[i] - mean);
+        e.spec[(long long)r * e.nout + c] = mean;
+    }
+    e.spread[r] = dist * inv;
 }
 
 // out[0] = scale * sum(part[0 .. np)), one block.
@@ -266,10 +310,14 @@ struct Mlp {
     float* td = nullptr;
     long long nd = 0;
     uint32_t* bidx = nullptr;
+    float* wpart = nullptr;             // split-K partial weight gradients
+    float* ens_spec = nullptr;          // ensemble spectra and spreads (batch_max rows)
+    float* ens_spread = nullptr;
     long long step = 0;                 // AdamW step count
 };
 
 constexpr int kPartBlocks = 256;
+constexpr int kMaxSplits = 32;
 
 void dfree(void* q) {
     if (q) (void)hipFree(q);
@@ -283,6 +331,7 @@ void destroy(Mlp* n) {
     for (auto* q : n->z) dfree(q);
     dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
     dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
+    dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread);
     if (n->st) (void)hipStreamDestroy(n->st);
     delete n;
 }
@@ -352,12 +401,23 @@ int backward(Mlp* n, int rows, float* loss_out) {
     MLP_HIP(hipGetLastError());
     int cur = 0;
     for (int l = n->L - 1; l >= 0; --l) {
+        // weight gradient: K = rows is long and M x N small, so K is split
+        // over blocks into partial products summed in a fixed order
         sp::mlp::GemmArgs w{};
         w.A = n->xh[l]; w.lda = n->K[l]; w.transA = 1;
         w.B = n->gh[cur]; w.ldb = n->N[l];
         w.M = n->K[l]; w.N = n->N[l]; w.K = rows;
-        w.outF = n->g + n->woff[l];
-        MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_PLAIN>(w, n->st));
+        const int tiles = ((w.M + 63) / 64) * ((w.N + 63) / 64);
+        int splits = std::max(1, std::min(kMaxSplits, 256 / std::max(1, tiles)));
+        splits = std::min(splits, (rows + 127) / 128);
+        w.kchunk = ((rows + splits - 1) / splits + 31) / 32 * 32;
+        splits = (rows + w.kchunk - 1) / w.kchunk;
+        const long long mn = (long long)w.M * w.N;
+        w.outF = n->wpart;
+        MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_PLAIN>(w, n->st, splits));
+        hipLaunchKernelGGL(sp::mlp::split_sum_kernel, dim3(blocks(mn)), dim3(256), 0, n->st,
+                           n->wpart, splits, mn, n->g + n->woff[l]);
+        MLP_HIP(hipGetLastError());
         hipLaunchKernelGGL(sp::mlp::colsum_kernel, dim3((unsigned)n->N[l]), dim3(256), 0, n->st,
                            n->gh[cur], rows, n->N[l], n->g + n->boff[l]);
         MLP_HIP(hipGetLastError());
@@ -465,6 +525,9 @@ int sp_mlp_create(const sp_mlp_config* cfg, void** out) {
     if (e == hipSuccess) e = hipMalloc(&n->part, 4 * (size_t)kPartBlocks);
     if (e == hipSuccess) e = hipMalloc(&n->xin, 4 * B * (size_t)cfg->nin);
     if (e == hipSuccess) e = hipMalloc(&n->bidx, 4 * B);
+    long long wmaxmn = 0;
+    for (int l = 0; l < n->L; ++l) wmaxmn = std::max(wmaxmn, (long long)n->K[l] * n->N[l]);
+    if (e == hipSuccess) e = hipMalloc(&n->wpart, 4 * (size_t)kMaxSplits * (size_t)wmaxmn);
     if (e == hipSuccess) e = hipMalloc(&n->loss_hist, 4 * 16);
     if (e != hipSuccess) return fail_all(e, "device allocation for the network");
     n->loss_cap = 16;
@@ -535,6 +598,51 @@ int sp_mlp_forward(void* q, const float* x, int64_t rows, float* y) {
                                hipMemcpyDeviceToHost, n->st));
     }
     MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, int64_t rows,
+                            double floor, float* spec, float* spread) {
+    if (!nets || members < 1 || members > 8) return sp::set_error(SP_ERR_ARG, "members must be in [1, 8]");
+    if (rows < 0 || (rows > 0 && (!x || !spec || !spread))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
+    std::vector<Mlp*> ms;
+    int rc;
+    for (int i = 0; i < members; ++i) {
+        Mlp* n = live(nets[i], rc);
+        if (!n) return rc;
+        if (n->cfg.nin != static_cast<Mlp*>(nets[0])->cfg.nin || n->cfg.nout != static_cast<Mlp*>(nets[0])->cfg.nout)
+            return sp::set_error(SP_ERR_ARG, "ensemble members differ in nin or nout");
+        ms.push_back(n);
+    }
+    Mlp* h = ms[0];
+    int chunk = h->bmax;
+    for (auto* n : ms) chunk = std::min(chunk, n->bmax);
+    const int nout = h->cfg.nout;
+    if (!h->ens_spec) {
+        MLP_HIP(hipMalloc(&h->ens_spec, 4 * (size_t)h->bmax * nout));
+        MLP_HIP(hipMalloc(&h->ens_spread, 4 * (size_t)h->bmax));
+    }
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const int r = (int)std::min<int64_t>(chunk, rows - r0);
+        for (auto* n : ms) {
+            rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
+            if (rc == SP_OK) rc = forward(n, r);
+            if (rc != SP_OK) return rc;
+            MLP_HIP(hipStreamSynchronize(n->st));
+        }
+        sp::mlp::EnsembleArgs e{};
+        for (int i = 0; i < members; ++i) e.y[i] = ms[(size_t)i]->y;
+        e.members = members;
+        e.nout = nout;
+        e.floor = (float)floor;
+        e.spec = h->ens_spec;
+        e.spread = h->ens_spread;
+        hipLaunchKernelGGL(sp::mlp::ensemble_kernel, dim3(blocks(r)), dim3(256), 0, h->st, e, r);
+        MLP_HIP(hipGetLastError());
+        MLP_HIP(hipMemcpyAsync(spec + r0 * nout, h->ens_spec, 4 * (size_t)r * nout, hipMemcpyDeviceToHost, h->st));
+        MLP_HIP(hipMemcpyAsync(spread + r0, h->ens_spread, 4 * (size_t)r, hipMemcpyDeviceToHost, h->st));
+        MLP_HIP(hipStreamSynchronize(h->st));
+    }
     return SP_OK;
 }
 

@@ -102,3 +102,191 @@ class Mlp:
         self._call("sp_mlp_train", int(nsteps), int(batch), float(lr0), float(lr_min), int(seed),
                    self._f(losses))
         return losses[:nsteps]
+
+
+# ---------------------------------------------------------------------------
+# Query features
+LOG_KPARA0 = (-4.0, -1.0)
+LOG_KRET = (-3.0, 0.0)
+PINDEX = (1.0, 2.0)
+
+
+def _scale(v, lo, hi):
+    return 2.0 * (np.asarray(v, dtype=np.float64) - lo) / (hi - lo) - 1.0
+
+
+def features(kpara0, kret, pindex, region, t, tmax):
+    """(n, 6) network inputs; region is 0..3 (two bits), t in (0, tmax]."""
+    region = np.asarray(region, dtype=np.int64)
+    cols = [_scale(np.log10(kpara0), *LOG_KPARA0), _scale(np.log10(kret), *LOG_KRET),
+            _scale(pindex, *PINDEX), 2.0 * (region & 1) - 1.0, 2.0 * ((region >> 1) & 1) - 1.0,
+            _scale(t, 0.0, tmax)]
+    return np.stack(np.broadcast_arrays(*cols), axis=1).astype(np.float32)
+
+
+def in_domain(kpara0, kret, pindex, region):
+    lk, lr = np.log10(kpara0), np.log10(kret)
+    region = np.asarray(region)
+    return ((lk >= LOG_KPARA0[0]) & (lk <= LOG_KPARA0[1]) & (lr >= LOG_KRET[0]) & (lr <= LOG_KRET[1]) &
+            (np.asarray(pindex) >= PINDEX[0]) & (np.asarray(pindex) <= PINDEX[1]) &
+            (region >= 0) & (region <= 3))
+
+
+def spectra_from_log(logf):
+    """Spectra from network outputs: 10^y - FLOOR, clipped at 0."""
+    return np.clip(10.0 ** np.asarray(logf, dtype=np.float64) - FLOOR, 0.0, None)
+
+
+def l1(a, b):
+    """Per-row L1 distance of spectra (fractions of the injected particles)."""
+    return np.abs(np.asarray(a) - np.asarray(b)).sum(axis=-1)
+
+
+class SweepSDE:
+    """The plain method: spectra of parameter sets from the batched pusher.
+
+    Each query (kpara0, kret, pindex, region) injects `nptl` particles at p0 in
+    its region of the 2D snapshot and records f(p) after every interval of
+    `dt` for `ntimes` intervals: the fraction of the injected particles in each
+    of `nbins` log bins over prange."""
+
+    def __init__(self, grid=512, nptl=2048, dt=0.05, ntimes=8, nbins=64, prange=(0.5, 2.0),
+                 precision="f32", max_particles=4e7, nodes=None):
+        from . import workloads
+        from .pusher import ParkerPusher
+        self._pusher = ParkerPusher
+        self.grid, self.nptl, self.dt, self.ntimes, self.nbins = grid, nptl, dt, ntimes, nbins
+        self.cfg = workloads.config_2d(grid, pmin=prange[0], pmax=prange[1], dt_max_rel=1e-2,
+                                       precision=precision)
+        self.nodes = nodes if nodes is not None else workloads.fields_2d(grid, grid)
+        self.boxes = np.array(workloads.SWEEP_REGIONS, dtype=np.float64)
+        self.max_particles = int(max_particles)
+        self.tmax = dt * ntimes
+        self.times = dt * np.arange(1, ntimes + 1)
+        self.steps = 0
+
+    def run(self, kpara0, kret, pindex, region, nptl=None):
+        """(nq, ntimes, nbins) spectra; chunks of queries share one launch."""
+        nptl = int(nptl or self.nptl)
+        kpara0, kret, pindex, region = (np.atleast_1d(np.asarray(a)) for a in (kpara0, kret, pindex, region))
+        nq = len(kpara0)
+        out = np.zeros((nq, self.ntimes, self.nbins))
+        per = max(1, self.max_particles // nptl)
+        for q0 in range(0, nq, per):
+            q = slice(q0, min(nq, q0 + per))
+            m = q.stop - q.start
+            with self._pusher(**dict(self.cfg, nptl_max=m * nptl)) as pz:
+                pz.upload_fields(self.nodes)
+                pz.calc_fields_gradients()
+                pz.init_particle_distributions(npp_global=self.nbins, nmu_global=1)
+                pz.set_variants(list(zip(kpara0[q], kret[q], 3.0 - pindex[q])))
+                pz.inject_variants(np.full(m, nptl), 1e-7, 1, 5.0, 0.0, 0.0, self.boxes[region[q]])
+                for k in range(self.ntimes):
+                    st = pz.particle_mover(k * self.dt, self.dt, 1, 10)
+                    self.steps += st["steps"]
+                    f, _ = pz.variant_distributions(m)
+                    out[q, k] = f / nptl
+        return out
+
+
+def sample_queries(n, seed):
+    """Parameter sets drawn like the batched sweep: log-uniform kpara0 and
+    kret, uniform momentum exponent, one of four regions."""
+    rng = np.random.default_rng(seed)
+    return dict(kpara0=10.0 ** rng.uniform(*LOG_KPARA0, n), kret=10.0 ** rng.uniform(*LOG_KRET, n),
+                pindex=rng.uniform(*PINDEX, n), region=rng.integers(0, 4, n))
+
+
+def training_set(sde: SweepSDE, runs: dict):
+    """(X, Y, F): features of every (run, time), targets log10(f + FLOOR), and
+    the spectra themselves."""
+    f = sde.run(runs["kpara0"], runs["kret"], runs["pindex"], runs["region"])
+    nq, nt, nb = f.shape
+    rep = lambda a: np.repeat(np.asarray(a), nt)
+    t = np.tile(sde.times, nq)
+    x = features(rep(runs["kpara0"]), rep(runs["kret"]), rep(runs["pindex"]), rep(runs["region"]),
+                 t, sde.tmax)
+    y = np.log10(f.reshape(nq * nt, nb) + FLOOR).astype(np.float32)
+    return x, y, f
+
+
+class Surrogate:
+    """An ensemble of networks on log f(p); the spread between members is the
+    per-query error estimate that decides the fallback to the pusher."""
+
+    def __init__(self, members=2, nhidden=256, nlayers=4, nbins=64, seed=1, batch_max=65536):
+        self.nets = [Mlp(6, nhidden, nlayers, nbins, batch_max=batch_max, seed=seed + 101 * i)
+                     for i in range(members)]
+        self.scale = 1.0
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def fit(self, x, y, nsteps=3000, batch=4096, lr0=2e-3, lr_min=2e-5, seed=7):
+        losses = []
+        for i, n in enumerate(self.nets):
+            n.set_data(x, y)
+            losses.append(n.train(nsteps, batch, lr0, lr_min, seed + 17 * i))
+        return losses
+
+    def predict(self, x):
+        """(spectra, error estimate) per row: the members' mean spectrum and
+        the calibrated L1 spread between members (computed on the GPU)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = len(x)
+        spec = np.zeros((n, self.nets[0].shape[3]), dtype=np.float32)
+        spread = np.zeros(n, dtype=np.float32)
+        hs = (C.c_void_p * len(self.nets))(*[m.h for m in self.nets])
+        lib = self.nets[0].lib
+        f = Mlp._f
+        check(lib, "sp_mlp_ensemble_spectra",
+              lib.sp_mlp_ensemble_spectra(hs, len(self.nets), f(x), n, FLOOR, f(spec), f(spread)))
+        return spec, self.scale * spread
+
+    def calibrate(self, x, f_true, quantile=0.9):
+        """Scales the spread so that it covers the true L1 error on held-out
+        rows in `quantile` of the cases."""
+        f, spread = self.predict(x)
+        err = l1(f, f_true)
+        self.scale = 1.0
+        ratio = err / np.maximum(spread, 1e-12)
+        self.scale = float(np.quantile(ratio, quantile))
+        return self.scale
+
+
+class SpectrumService:
+    """Answers spectrum queries with the surrogate; a query outside the
+    trained domain or whose error estimate exceeds `tol` is answered by the
+    pusher (the same SweepSDE that made the training data)."""
+
+    def __init__(self, surrogate: Surrogate, sde: SweepSDE, tol=0.05):
+        self.s, self.sde, self.tol = surrogate, sde, tol
+
+    def answer(self, kpara0, kret, pindex, region):
+        """(spectra (nq, ntimes, nbins), used_sde (nq,) bool)."""
+        kpara0, kret, pindex, region = (np.atleast_1d(np.asarray(a)) for a in (kpara0, kret, pindex, region))
+        nq, nt = len(kpara0), self.sde.ntimes
+        out = np.zeros((nq, nt, self.sde.nbins))
+        ok = in_domain(kpara0, kret, pindex, region)
+        use_sde = ~ok
+        idx = np.nonzero(ok)[0]
+        if len(idx):
+            rep = lambda a: np.repeat(a[idx], nt)
+            x = features(rep(kpara0), rep(kret), rep(pindex), rep(region), np.tile(self.sde.times, len(idx)),
+                         self.sde.tmax)
+            f, est = self.s.predict(x)
+            f = f.reshape(len(idx), nt, -1)
+            est = est.reshape(len(idx), nt).max(axis=1)
+            out[idx] = f
+            use_sde[idx[est > self.tol]] = True
+        j = np.nonzero(use_sde)[0]
+        if len(j):
+            out[j] = self.sde.run(kpara0[j], kret[j], pindex[j], region[j])
+        return out, use_sde

@@ -77,3 +77,15 @@ def test_training_fits_and_is_deterministic():
             finals.append(net.get_params())
     assert losses[-20:].mean() < 0.1 * losses[:5].mean()
     np.testing.assert_array_equal(finals[0], finals[1])
+
+
+def test_ensemble_spectra_match_members():
+    from sp_hip.surrogate import FLOOR, Surrogate, spectra_from_log
+    x, _ = _data(3000, seed=5)
+    with Surrogate(members=2, seed=3, batch_max=1024) as s:
+        spec, spread = s.predict(x)
+        ys = [spectra_from_log(n.forward(x)) for n in s.nets]
+    mean = (ys[0] + ys[1]) / 2
+    np.testing.assert_allclose(spec, mean, rtol=1e-5, atol=1e-7 + 1e-5 * FLOOR)
+    ref = 0.5 * (np.abs(ys[0] - mean).sum(1) + np.abs(ys[1] - mean).sum(1))
+    np.testing.assert_allclose(spread, ref, rtol=1e-4, atol=1e-6)
