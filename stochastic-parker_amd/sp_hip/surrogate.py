@@ -217,7 +217,7 @@ class Surrogate:
     def __init__(self, members=2, nhidden=256, nlayers=4, nbins=64, seed=1, batch_max=65536):
         self.nets = [Mlp(6, nhidden, nlayers, nbins, batch_max=batch_max, seed=seed + 101 * i)
                      for i in range(members)]
-        self.scale = 1.0
+        self.c0, self.c1 = 0.0, 1.0
 
     def close(self):
         for n in self.nets:
@@ -238,7 +238,8 @@ class Surrogate:
 
     def predict(self, x):
         """(spectra, error estimate) per row: the members' mean spectrum and
-        the calibrated L1 spread between members (computed on the GPU)."""
+        the calibrated error estimate c0 + c1 * (L1 spread between members),
+        both computed on the GPU."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         n = len(x)
         spec = np.zeros((n, self.nets[0].shape[3]), dtype=np.float32)
@@ -248,17 +249,23 @@ class Surrogate:
         f = Mlp._f
         check(lib, "sp_mlp_ensemble_spectra",
               lib.sp_mlp_ensemble_spectra(hs, len(self.nets), f(x), n, FLOOR, f(spec), f(spread)))
-        return spec, self.scale * spread
+        return spec, self.c0 + self.c1 * spread
 
     def calibrate(self, x, f_true, quantile=0.9):
-        """Scales the spread so that it covers the true L1 error on held-out
-        rows in `quantile` of the cases."""
+        """Fits the error estimate on held-out rows: err ~ c0 + c1 spread by
+        least squares (c1 >= 0), then raises c0 until the estimate covers the
+        true L1 error in `quantile` of the rows.  Members share part of their
+        error, which the spread alone cannot see; c0 carries it."""
+        self.c0, self.c1 = 0.0, 1.0
         f, spread = self.predict(x)
         err = l1(f, f_true)
-        self.scale = 1.0
-        ratio = err / np.maximum(spread, 1e-12)
-        self.scale = float(np.quantile(ratio, quantile))
-        return self.scale
+        a = np.stack([np.ones_like(spread), spread], axis=1).astype(np.float64)
+        c0, c1 = np.linalg.lstsq(a, err, rcond=None)[0]
+        if c1 < 0:
+            c0, c1 = float(np.mean(err)), 0.0
+        resid = err - (c0 + c1 * spread)
+        self.c0, self.c1 = float(c0 + np.quantile(resid, quantile)), float(c1)
+        return self.c0, self.c1
 
 
 class SpectrumService:

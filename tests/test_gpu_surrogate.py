@@ -1,0 +1,91 @@
+"""The f(p) surrogate end to end on a small sweep: spectra from the batched
+pusher train the network on the GPU; on held-out parameter sets it is about as
+accurate as a pusher run with the training particle count, answers a sweep
+at least 100x faster than the pusher, and hands queries it cannot answer
+(outside the trained domain, or with a large error estimate) to the pusher."""
+import time
+
+import numpy as np
+import pytest
+
+from sp_hip import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+GRID, NPTL, NREF, NTIMES, DT = 128, 4096, 65536, 4, 0.05
+
+
+@pytest.fixture(scope="module")
+def trained():
+    sde = S.SweepSDE(grid=GRID, nptl=NPTL, dt=DT, ntimes=NTIMES)
+    runs = S.sample_queries(4000, 1)
+    t0 = time.perf_counter()
+    x, y, _ = S.training_set(sde, runs)
+    t_sde = (time.perf_counter() - t0) / 4000
+    sur = S.Surrogate(members=2, seed=5)
+    losses = sur.fit(x, y, nsteps=4000, batch=4096, lr0=2e-3, lr_min=2e-5)
+    ref = S.SweepSDE(grid=GRID, nptl=NREF, dt=DT, ntimes=NTIMES, nodes=sde.nodes)
+    yield sde, sur, ref, t_sde, losses
+    sur.close()
+
+
+def _rows(sde, q):
+    rep = lambda a: np.repeat(np.asarray(a), sde.ntimes)
+    return S.features(rep(q["kpara0"]), rep(q["kret"]), rep(q["pindex"]), rep(q["region"]),
+                      np.tile(sde.times, len(q["kpara0"])), sde.tmax)
+
+
+def test_surrogate_matches_held_out_pusher_runs(trained):
+    sde, sur, ref, _, losses = trained
+    for l in losses:
+        assert l[-50:].mean() < 0.01 * l[:5].mean()
+    q = S.sample_queries(48, 2)
+    f_ref = ref.run(q["kpara0"], q["kret"], q["pindex"], q["region"]).reshape(-1, sde.nbins)
+    f_sde = sde.run(q["kpara0"], q["kret"], q["pindex"], q["region"]).reshape(-1, sde.nbins)
+    f_sur, _ = sur.predict(_rows(sde, q))
+    err_sur = S.l1(f_sur, f_ref)
+    err_sde = S.l1(f_sde, f_ref)
+    print(f"L1 vs {NREF}-particle runs: surrogate {err_sur.mean():.4f} (p90 "
+          f"{np.quantile(err_sur, 0.9):.4f}), {NPTL}-particle pusher {err_sde.mean():.4f}")
+    # the network regresses the noise of its training runs away: it is at
+    # least as accurate as one pusher run with the training particle count
+    assert err_sur.mean() < 1.2 * err_sde.mean()
+    assert err_sur.mean() < 0.05
+
+
+def test_surrogate_sweep_is_100x_faster(trained):
+    sde, sur, _, t_sde, _ = trained
+    q = S.sample_queries(20000, 3)
+    x = _rows(sde, q)
+    sur.predict(x[:1000])
+    t0 = time.perf_counter()
+    sur.predict(x)
+    t_sur = (time.perf_counter() - t0) / 20000
+    assert t_sde / t_sur > 100.0, (t_sde, t_sur)
+
+
+def test_fallback_runs_the_pusher(trained):
+    sde, sur, ref, _, _ = trained
+    va = S.sample_queries(48, 4)
+    f_va = ref.run(va["kpara0"], va["kret"], va["pindex"], va["region"])
+    sur.calibrate(_rows(sde, va), f_va.reshape(-1, sde.nbins))
+    # outside the trained domain: kpara0 = 0.5 > 0.1; momentum exponent 2.5 > 2
+    q = dict(kpara0=np.array([0.5, 1e-3]), kret=np.array([0.1, 0.5]), pindex=np.array([1.5, 2.5]),
+             region=np.array([0, 1]))
+    f, used = S.SpectrumService(sur, sde, tol=1.0).answer(**q)
+    assert used.all()
+    np.testing.assert_array_equal(f, sde.run(**q))
+    # in the domain the calibrated estimate decides: a tolerance below every
+    # estimate sends all queries to the pusher, a looser one keeps most on the
+    # network, and the accepted answers meet the tolerance
+    q2 = S.sample_queries(64, 5)
+    _, used0 = S.SpectrumService(sur, sde, tol=0.0).answer(**q2)
+    assert used0.all()
+    tol = 0.1
+    f2, used2 = S.SpectrumService(sur, sde, tol=tol).answer(**q2)
+    f2_ref = ref.run(q2["kpara0"], q2["kret"], q2["pindex"], q2["region"])
+    err = S.l1(f2, f2_ref).max(axis=1)
+    print(f"error estimate c0 {sur.c0:.4f} c1 {sur.c1:.3f}; pusher fallback for {used2.mean():.2f}; "
+          f"accepted max-over-time L1: {np.round(np.sort(err[~used2])[-5:], 4)}")
+    assert used2.mean() < 0.5
+    assert np.mean(err[~used2] <= tol) >= 0.85
