@@ -10,6 +10,10 @@ Workloads:
   batch         64 diffusion-parameter variants over the 512^2 2D snapshot in
                 one mover launch: kpara0, kret, momentum exponent and injection
                 region per variant, 1e5 .. 1e7 particles each (1.39e8 total).
+  surrogate     the f(p) surrogate: 2e4 sweep runs of 8192 particles (eight
+                spectra each) train a 6-256x4-64 SiLU network ensemble on the
+                GPU; accuracy against held-out 131072-particle runs and the
+                time of a 1e5-query sweep against the pusher's.
 One bench "step" is one MHD interval of the hot path: the mover over every
 particle plus the f(p) and spatial-map histograms.  Inputs are resident in
 HBM before timing starts.
@@ -210,12 +214,85 @@ def measured_traffic(wl, n, precision):
     return t.get("traffic_bytes_per_launch")
 
 
+def surrogate_bench(a):
+    """Trains the surrogate on pusher spectra and measures a 1e5-query sweep."""
+    from sp_hip import surrogate as S
+
+    t0 = time.perf_counter()
+    sde = S.SweepSDE(grid=512, nptl=8192)
+    log(f"sweep snapshot 512^2 in {time.perf_counter() - t0:.1f} s")
+    runs = S.sample_queries(a.sur_runs, 1)
+    t0 = time.perf_counter()
+    x, y, _ = S.training_set(sde, runs)
+    t_data = time.perf_counter() - t0
+    steps_data = sde.steps
+    log(f"training data: {a.sur_runs} runs, {steps_data:.3e} particle-steps, {t_data:.1f} s")
+    ref = S.SweepSDE(grid=512, nptl=131072, nodes=sde.nodes)
+    val, test = S.sample_queries(128, 2), S.sample_queries(128, 3)
+    f_val = ref.run(val["kpara0"], val["kret"], val["pindex"], val["region"])
+    f_test = ref.run(test["kpara0"], test["kret"], test["pindex"], test["region"])
+    f_sde = sde.run(test["kpara0"], test["kret"], test["pindex"], test["region"])
+
+    def rows(q):
+        rep = lambda v: np.repeat(np.asarray(v), sde.ntimes)
+        return S.features(rep(q["kpara0"]), rep(q["kret"]), rep(q["pindex"]), rep(q["region"]),
+                          np.tile(sde.times, len(q["kpara0"])), sde.tmax)
+
+    with S.Surrogate(members=2) as sur:
+        t0 = time.perf_counter()
+        losses = sur.fit(x, y, nsteps=a.sur_steps)
+        t_train = time.perf_counter() - t0
+        log(f"training: {a.sur_steps} steps x 2 members in {t_train:.1f} s, final loss "
+            f"{float(losses[0][-20:].mean()):.4f}")
+        sur.calibrate(rows(val), f_val.reshape(-1, sde.nbins))
+        f_sur, est = sur.predict(rows(test))
+        err = S.l1(f_sur, f_test.reshape(-1, sde.nbins))
+        err_sde = S.l1(f_sde, f_test).reshape(-1)
+        q_err = err.reshape(128, sde.ntimes).max(axis=1)
+        q_ok = est.reshape(128, sde.ntimes).max(axis=1) <= 0.05
+        sweep = rows(S.sample_queries(100000, 9))
+        sur.predict(sweep[:4096])
+        t0 = time.perf_counter()
+        sur.predict(sweep)
+        t_sweep = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sur.nets[0].forward(sweep)
+        t_fwd = time.perf_counter() - t0
+    ms_sde = 1e3 * t_data / a.sur_runs
+    flops = 2.0 * sum(k * n for k, n in zip([6, 256, 256, 256, 256], [256, 256, 256, 256, 64]))
+    tflops = flops * len(sweep) / t_fwd / 1e12
+    result = {
+        "metric": "spectrum queries/s (f(p) surrogate, 1e5-query sweep) at f(p) L1 <=5% "
+                  "vs 131072-particle pusher runs",
+        "value": 1e5 / t_sweep, "unit": "queries/s", "n_gpus": 1, "steps": a.sur_steps,
+        "warmup": 0, "ms_per_step": 1e3 * t_train / (2 * a.sur_steps), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "6-256x4-64 SiLU ensemble of 2 on spectra of the 512^2 2D snapshot: "
+                               f"{a.sur_runs} runs x 8192 particles x 8 times",
+                   "train_rows": int(len(x)), "train_data_seconds": t_data,
+                   "train_data_particle_steps": int(steps_data),
+                   "train_seconds": t_train, "batch": 4096},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "traffic": None,
+                     "kernel": "gemm_bf16_kernel (forward of one member over the sweep)"},
+        "accuracy": {"l1_mean": float(err.mean()), "l1_p90": float(np.quantile(err, 0.9)),
+                     "l1_max": float(err.max()), "pusher_8192_l1_mean": float(err_sde.mean()),
+                     "accepted_queries": float(q_ok.mean()),
+                     "accepted_query_l1_max_mean": float(q_err[q_ok].mean()) if q_ok.any() else None},
+        "speed": {"pusher_ms_per_query": ms_sde, "surrogate_us_per_query": 1e6 * t_sweep / 1e5,
+                  "speedup": ms_sde * 1e-3 * 1e5 / t_sweep},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed MHD intervals")
     ap.add_argument("--warmup", type=int, default=1, help="untimed MHD intervals")
-    ap.add_argument("--workload", choices=sorted(DEFAULTS), default="2d")
+    ap.add_argument("--workload", choices=sorted(DEFAULTS) + ["surrogate"], default="2d")
+    ap.add_argument("--sur-runs", type=int, default=20000, help="surrogate training runs")
+    ap.add_argument("--sur-steps", type=int, default=8000, help="surrogate AdamW steps")
     ap.add_argument("--grid", type=int, default=0, help="cells per axis (default per workload)")
     ap.add_argument("--nptl", type=float, default=-1, help="particles (default per workload)")
     ap.add_argument("--nsteps", type=int, default=0,
@@ -231,6 +308,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the pusher shards by "
               "particles and this build measures one GPU's share", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "surrogate":
+        surrogate_bench(a)
+        return
 
     d = DEFAULTS[a.workload]
     wl = Workload(a.workload, a.grid or d["grid"], a.nsteps or d["nsteps"],
