@@ -126,6 +126,48 @@ def test_kappa_divergence_matches_finite_differences(ndim, mag):
             assert abs(getattr(k0, nm) - fd) < 1e-6 * scale + 1e-12, (nm, getattr(k0, nm), fd)
 
 
+def test_kappa_divergence_3d_magnetic_dependence_quirk():
+    """3D with kappa ~ B^(gamma-2): the reference takes d(ln kappa)/dx_j as
+    (gamma - 2) dB/dx_j, without the 1/B (particle_module.f90:2405-2409).  Its
+    divergence terms are then the exact divergence of a tensor whose magnitude
+    varies as exp((gamma - 2)(B - B0)) around the point instead of (B/B0)^(gamma-2):
+    finite differences of that field, built from the oracle's tensors at
+    neighbouring points, pin the formula."""
+    c = cfg3d(mag_dependency=1)
+    prm = oracle.params(c)
+    g2 = c["gamma_turb"] - 2.0
+    rng = np.random.default_rng(19)
+    h = 1e-5
+    comp = {"dkxx_dx": ("kxx", 0), "dkyy_dy": ("kyy", 1), "dkzz_dz": ("kzz", 2),
+            "dkxy_dx": ("kxy", 0), "dkxy_dy": ("kxy", 1), "dkxz_dx": ("kxz", 0),
+            "dkxz_dz": ("kxz", 2), "dkyz_dy": ("kyz", 1), "dkyz_dz": ("kyz", 2)}
+    worst_exact = 0.0
+    for _ in range(10):
+        x, y, z = rng.uniform(0, 1, 3)
+        p = np.exp(rng.uniform(-1, 2))
+        f0 = fields_at(x, y, z, 3)
+        b0 = f0[7]
+        k0 = kappa(prm, p, f0)
+        for nm, (attr, ax) in comp.items():
+            e = np.zeros(3)
+            e[ax] = h
+            vals, plain = [], []
+            for sgn in (1, -1):
+                f = fields_at(x + sgn * e[0], y + sgn * e[1], z + sgn * e[2], 3)
+                b = f[7]
+                ratio = np.exp(g2 * (b - b0)) * (b0 / b) ** g2
+                kv = getattr(kappa(prm, p, f), attr)
+                vals.append(kv * ratio)
+                plain.append(kv)
+            fd = (vals[0] - vals[1]) / (2 * h)
+            scale = max(abs(fd), 1e-3 * abs(k0.kpara))
+            assert abs(getattr(k0, nm) - fd) < 1e-6 * scale + 1e-12, (nm, getattr(k0, nm), fd)
+            exact = (plain[0] - plain[1]) / (2 * h)
+            worst_exact = max(worst_exact, abs(getattr(k0, nm) - exact) / scale)
+    # the exact divergence of B^(gamma-2) scaling is a different formula
+    assert worst_exact > 1e-3
+
+
 def curl_B_over_B2(x, y, z, ndim, h=1e-6):
     def g(px, py, pz):
         B, _ = analytic_B(px, py, pz, ndim)
