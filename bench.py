@@ -255,12 +255,14 @@ def surrogate_bench(a):
         t0 = time.perf_counter()
         sur.predict(sweep)
         t_sweep = time.perf_counter() - t0
-        t0 = time.perf_counter()
         sur.nets[0].forward(sweep)
-        t_fwd = time.perf_counter() - t0
+        t_fwd = 1e-3 * sur.nets[0].last_forward_ms()
     ms_sde = 1e3 * t_data / a.sur_runs
     flops = 2.0 * sum(k * n for k, n in zip([6, 256, 256, 256, 256], [256, 256, 256, 256, 64]))
     tflops = flops * len(sweep) / t_fwd / 1e12
+    # bytes per row through the five layers: fp32 input converted to bf16, bf16
+    # activations written and read back, fp32 output
+    row_bytes = 6 * 4 + 6 * 2 + 256 * 2 + 3 * (256 * 2 + 256 * 2) + 256 * 2 + 64 * 4
     result = {
         "metric": "spectrum queries/s (f(p) surrogate, 1e5-query sweep) at f(p) L1 <=5% "
                   "vs 131072-particle pusher runs",
@@ -274,7 +276,10 @@ def surrogate_bench(a):
                    "train_seconds": t_train, "batch": 4096},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": tflops / 2500.0, "traffic": None,
-                     "kernel": "gemm_bf16_kernel (forward of one member over the sweep)"},
+                     "kernel": "gemm_bf16_kernel: the five layers of one member over the "
+                               "8e5-row sweep, HIP-event time",
+                     "flop_per_byte": flops / row_bytes,
+                     "hbm_ceiling_tflops": flops / row_bytes * 8.0},
         "accuracy": {"l1_mean": float(err.mean()), "l1_p90": float(np.quantile(err, 0.9)),
                      "l1_max": float(err.max()), "pusher_8192_l1_mean": float(err_sde.mean()),
                      "accepted_queries": float(q_ok.mean()),

@@ -252,6 +252,8 @@ int sp_mlp_get_params(void* mlp, float* params);
 int sp_mlp_set_params(void* mlp, const float* params);
 /* y (rows x nout) = network(x (rows x nin)), row-major host arrays. */
 int sp_mlp_forward(void* mlp, const float* x, int64_t rows, float* y);
+/* Device time (HIP events) of the layer kernels of the last sp_mlp_forward. */
+int sp_mlp_last_forward_ms(void* mlp, double* ms);
 /* Mean squared error of network(x) against t and its gradient with respect
  * to every parameter (rows <= batch_max); no update.  grads may be NULL. */
 int sp_mlp_gradients(void* mlp, const float* x, const float* t, int64_t rows, float* grads,

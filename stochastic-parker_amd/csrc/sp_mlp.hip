@@ -10,9 +10,10 @@
 // and every back-propagated pre-activation gradient; all products accumulate
 // in FP32.
 //
-// GEMM: 64 x 64 output tiles per 256-lane block, four waves of 32 x 32, K in
-// steps of 32 staged through LDS as k-contiguous BF16 rows, two
-// v_mfma_f32_32x32x16_bf16 per wave per step.  Operand lane layout (checked
+// GEMM: 128 x 128 (or 128 x 64) output tiles per 256-lane block, four waves of
+// 64 x 64 (64 x 32), K in steps of 32 staged through LDS as k-contiguous BF16
+// rows with 16-byte loads, v_mfma_f32_32x32x16_bf16 on 2 x 2 (2 x 1) tiles per
+// wave.  Operand lane layout (checked
 // on gfx950): lane l holds row/column l % 32, k = 8 (l / 32) .. + 7; result
 // register j of lane l is row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32.
 #include <hip/hip_runtime.h>
@@ -32,7 +33,6 @@ namespace mlp {
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int kTile = 64;
 constexpr int kKT = 32;
 constexpr int kLdsPad = 8;   // row stride 40 halves = 80 B
 
@@ -47,6 +47,7 @@ __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-
 
 enum Epilogue : int {
     EPI_HIDDEN = 0,   // z = acc + b: store z (FP32) and bf16(silu(z))
+    EPI_INFER = 4,    // as EPI_HIDDEN without z (no backward pass follows)
     EPI_LINEAR = 1,   // y = acc + b: store y (FP32)
     EPI_BACK = 2,     // g = acc * silu'(zprev): store bf16(g)
     EPI_PLAIN = 3,    // store acc (FP32)
@@ -65,82 +66,124 @@ struct GemmArgs {
     uint16_t* outH;
 };
 
-template <int EPI>
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-    __shared__ uint16_t As[kTile][kKT + kLdsPad];
-    __shared__ uint16_t Bs[kTile][kKT + kLdsPad];
+// Stages one operand tile (R rows or columns x kKT k) into LDS as k-contiguous
+// rows.  X(r, k) = kmajor ? X[r ld + k] : X[k ld + r].  Chunks of 8 elements
+// along the contiguous axis move as one 16-byte load when they are in range
+// and aligned; the rest element by element (zero outside [0, nr) x [kb, ke)).
+template <int R>
+__device__ __forceinline__ void stage(uint16_t (*S)[kKT + kLdsPad], const uint16_t* X, int ld,
+                                      bool kmajor, int r0, int nr, int k0, int ke) {
     const int t = threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
+    constexpr int kChunks = R * kKT / 8;
+    const bool vec_ok = (ld % 8) == 0 && (reinterpret_cast<uintptr_t>(X) % 16) == 0;
+#pragma unroll
+    for (int c = t; c < kChunks; c += 256) {
+        if (kmajor) {
+            const int r = c / (kKT / 8), kc = (c % (kKT / 8)) * 8;
+            const int gr = r0 + r, gk = k0 + kc;
+            if (vec_ok && gr < nr && gk + 8 <= ke) {
+                *reinterpret_cast<uint4*>(&S[r][kc]) =
+                    *reinterpret_cast<const uint4*>(X + (long long)gr * ld + gk);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    S[r][kc + i] = (gr < nr && gk + i < ke) ? X[(long long)gr * ld + gk + i] : (uint16_t)0;
+            }
+        } else {
+            const int k = c / (R / 8), rc = (c % (R / 8)) * 8;
+            const int gk = k0 + k, gr = r0 + rc;
+            if (vec_ok && gk < ke && gr + 8 <= nr) {
+                const uint4 v = *reinterpret_cast<const uint4*>(X + (long long)gk * ld + gr);
+                const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) S[rc + i][k] = h[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    S[rc + i][k] = (gk < ke && gr + i < nr) ? X[(long long)gk * ld + gr + i] : (uint16_t)0;
+            }
+        }
+    }
+}
+
+// 128 x BN output tile per 256-lane block; wave (wm, wn) owns 64 x BN/2 as
+// 2 x BN/64 MFMA tiles of 32 x 32.
+template <int EPI, int BN>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+    constexpr int BM = 128;
+    constexpr int TN = BN / 64;   // 32-column MFMA tiles per wave
+    __shared__ __align__(16) uint16_t As[BM][kKT + kLdsPad];
+    __shared__ __align__(16) uint16_t Bs[BN][kKT + kLdsPad];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wm = w >> 1, wn = w & 1;
-    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     // split K: block z covers [kb, ke) and writes its own partial (EPI_PLAIN)
     const int kb = g.kchunk ? (int)blockIdx.z * g.kchunk : 0;
     const int ke = g.kchunk ? min(g.K, kb + g.kchunk) : g.K;
-    v16f acc = {};
+    v16f acc[2][TN];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = v16f{};
     for (int k0 = kb; k0 < ke; k0 += kKT) {
-        // stage A (64 x 32) and B (32 x 64, stored n-major) in LDS; the index
-        // mapping follows the contiguous axis of each operand in memory
-        for (int e = t; e < kTile * kKT; e += 256) {
-            int r, kk;
-            if (g.transA) { r = e % kTile; kk = e / kTile; } else { r = e / kKT; kk = e % kKT; }
-            const int gm = m0 + r, gk = k0 + kk;
-            uint16_t v = 0;
-            if (gm < g.M && gk < ke)
-                v = g.transA ? g.A[(long long)gk * g.lda + gm] : g.A[(long long)gm * g.lda + gk];
-            As[r][kk] = v;
-        }
-        for (int e = t; e < kTile * kKT; e += 256) {
-            int c, kk;
-            if (g.transB) { c = e / kKT; kk = e % kKT; } else { c = e % kTile; kk = e / kTile; }
-            const int gn = n0 + c, gk = k0 + kk;
-            uint16_t v = 0;
-            if (gn < g.N && gk < ke)
-                v = g.transB ? g.B[(long long)gn * g.ldb + gk] : g.B[(long long)gk * g.ldb + gn];
-            Bs[c][kk] = v;
-        }
+        stage<BM>(As, g.A, g.lda, !g.transA, m0, g.M, k0, ke);
+        stage<BN>(Bs, g.B, g.ldb, g.transB != 0, n0, g.N, k0, ke);
         __syncthreads();
 #pragma unroll
         for (int kc = 0; kc < kKT; kc += 16) {
-            v8bf a, b;
-            const uint16_t* ap = &As[wm * 32 + (lane & 31)][kc + 8 * (lane >> 5)];
-            const uint16_t* bp = &Bs[wn * 32 + (lane & 31)][kc + 8 * (lane >> 5)];
+            v8bf a[2], b[TN];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                a[i] = __builtin_bit_cast(__bf16, ap[i]);
-                b[i] = __builtin_bit_cast(__bf16, bp[i]);
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const v8bf*>(&As[wm * 64 + i * 32 + (lane & 31)][kc + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                b[j] = *reinterpret_cast<const v8bf*>(&Bs[wn * (BN / 2) + j * 32 + (lane & 31)][kc + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
-    const int col = n0 + wn * 32 + (lane & 31);
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int row = m0 + wm * 32 + (j >> 2) * 8 + (lane >> 5) * 4 + (j & 3);
-        if (row >= g.M || col >= g.N) continue;
-        const long long o = (long long)row * g.N + col;
-        const float v = acc[j];
-        if (EPI == EPI_HIDDEN) {
-            const float z = v + g.bias[col];
-            g.outF[o] = z;
-            g.outH[o] = bf16_rne(z * sigmoid(z));
-        } else if (EPI == EPI_LINEAR) {
-            g.outF[o] = v + g.bias[col];
-        } else if (EPI == EPI_BACK) {
-            const float z = g.zprev[o];
-            const float s = sigmoid(z);
-            g.outH[o] = bf16_rne(v * (s * (1.0f + z * (1.0f - s))));
-        } else {
-            g.outF[(long long)blockIdx.z * g.M * g.N + o] = v;
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int col = n0 + wn * (BN / 2) + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                if (row >= g.M || col >= g.N) continue;
+                const long long o = (long long)row * g.N + col;
+                const float v = acc[i][j][r];
+                if (EPI == EPI_HIDDEN || EPI == EPI_INFER) {
+                    const float z = v + g.bias[col];
+                    if (EPI == EPI_HIDDEN) g.outF[o] = z;
+                    g.outH[o] = bf16_rne(z * sigmoid(z));
+                } else if (EPI == EPI_LINEAR) {
+                    g.outF[o] = v + g.bias[col];
+                } else if (EPI == EPI_BACK) {
+                    const float z = g.zprev[o];
+                    const float s = sigmoid(z);
+                    g.outH[o] = bf16_rne(v * (s * (1.0f + z * (1.0f - s))));
+                } else {
+                    g.outF[(long long)blockIdx.z * g.M * g.N + o] = v;
+                }
+            }
         }
     }
 }
 
 template <int EPI>
 hipError_t gemm(const GemmArgs& g, hipStream_t s, int splits = 1) {
-    const dim3 grid((unsigned)((g.N + kTile - 1) / kTile), (unsigned)((g.M + kTile - 1) / kTile),
-                    (unsigned)splits);
-    hipLaunchKernelGGL(gemm_bf16_kernel<EPI>, grid, dim3(256), 0, s, g);
+    if (g.N > 64) {
+        const dim3 grid((unsigned)((g.N + 127) / 128), (unsigned)((g.M + 127) / 128), (unsigned)splits);
+        hipLaunchKernelGGL((gemm_bf16_kernel<EPI, 128>), grid, dim3(256), 0, s, g);
+    } else {
+        const dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 127) / 128), (unsigned)splits);
+        hipLaunchKernelGGL((gemm_bf16_kernel<EPI, 64>), grid, dim3(256), 0, s, g);
+    }
     return hipGetLastError();
 }
 
@@ -311,6 +354,8 @@ struct Mlp {
     long long nd = 0;
     uint32_t* bidx = nullptr;
     float* wpart = nullptr;             // split-K partial weight gradients
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double fwd_ms = 0.0;                // device time of the last sp_mlp_forward's layers
     float* ens_spec = nullptr;          // ensemble spectra and spreads (batch_max rows)
     float* ens_spread = nullptr;
     long long step = 0;                 // AdamW step count
@@ -332,6 +377,8 @@ void destroy(Mlp* n) {
     dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
     dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
     dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread);
+    for (auto& e : n->ev)
+        if (e) (void)hipEventDestroy(e);
     if (n->st) (void)hipStreamDestroy(n->st);
     delete n;
 }
@@ -367,7 +414,7 @@ void init_params(const Mlp* n, std::vector<float>& h) {
 unsigned blocks(long long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 // Forward pass of rows [0, rows) of X_0 (already in xh[0]).
-int forward(Mlp* n, int rows) {
+int forward(Mlp* n, int rows, bool train = true) {
     for (int l = 0; l < n->L; ++l) {
         sp::mlp::GemmArgs g{};
         g.A = n->xh[l];
@@ -379,7 +426,8 @@ int forward(Mlp* n, int rows) {
         if (l + 1 < n->L) {
             g.outF = n->z[l];
             g.outH = n->xh[l + 1];
-            MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_HIDDEN>(g, n->st));
+            if (train) MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_HIDDEN>(g, n->st));
+            else MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_INFER>(g, n->st));
         } else {
             g.outF = n->y;
             MLP_HIP(sp::mlp::gemm<sp::mlp::EPI_LINEAR>(g, n->st));
@@ -407,7 +455,8 @@ int backward(Mlp* n, int rows, float* loss_out) {
         w.A = n->xh[l]; w.lda = n->K[l]; w.transA = 1;
         w.B = n->gh[cur]; w.ldb = n->N[l];
         w.M = n->K[l]; w.N = n->N[l]; w.K = rows;
-        const int tiles = ((w.M + 63) / 64) * ((w.N + 63) / 64);
+        const int bn = w.N > 64 ? 128 : 64;
+        const int tiles = ((w.M + 127) / 128) * ((w.N + bn - 1) / bn);
         int splits = std::max(1, std::min(kMaxSplits, 256 / std::max(1, tiles)));
         splits = std::min(splits, (rows + 127) / 128);
         w.kchunk = ((rows + splits - 1) / splits + 31) / 32 * 32;
@@ -508,6 +557,9 @@ int sp_mlp_create(const sp_mlp_config* cfg, void** out) {
     };
     hipError_t e = hipStreamCreate(&n->st);
     if (e != hipSuccess) return fail_all(e, "hipStreamCreate");
+    for (auto& ev : n->ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    if (e != hipSuccess) return fail_all(e, "hipEventCreate");
     const size_t P = (size_t)n->P, B = (size_t)n->bmax;
     e = hipMalloc(&n->p, 4 * P);
     if (e == hipSuccess) e = hipMalloc(&n->g, 4 * P);
@@ -588,16 +640,32 @@ int sp_mlp_forward(void* q, const float* x, int64_t rows, float* y) {
     Mlp* n = live(q, rc);
     if (!n) return rc;
     if (rows < 0 || (rows > 0 && (!x || !y))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
+    n->fwd_ms = 0.0;
     for (int64_t r0 = 0; r0 < rows; r0 += n->bmax) {
         const int r = (int)std::min<int64_t>(n->bmax, rows - r0);
         rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
         if (rc != SP_OK) return rc;
-        rc = forward(n, r);
+        MLP_HIP(hipEventRecord(n->ev[0], n->st));
+        rc = forward(n, r, false);
         if (rc != SP_OK) return rc;
+        MLP_HIP(hipEventRecord(n->ev[1], n->st));
         MLP_HIP(hipMemcpyAsync(y + r0 * n->cfg.nout, n->y, 4 * (size_t)r * n->cfg.nout,
                                hipMemcpyDeviceToHost, n->st));
+        MLP_HIP(hipStreamSynchronize(n->st));
+        float ms = 0.f;
+        MLP_HIP(hipEventElapsedTime(&ms, n->ev[0], n->ev[1]));
+        n->fwd_ms += ms;
     }
     MLP_HIP(hipStreamSynchronize(n->st));
+    return SP_OK;
+}
+
+int sp_mlp_last_forward_ms(void* q, double* ms) {
+    int rc;
+    Mlp* n = live(q, rc);
+    if (!n) return rc;
+    if (!ms) return sp::set_error(SP_ERR_ARG, "ms is NULL");
+    *ms = n->fwd_ms;
     return SP_OK;
 }
 
@@ -626,7 +694,7 @@ int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, 
         const int r = (int)std::min<int64_t>(chunk, rows - r0);
         for (auto* n : ms) {
             rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
-            if (rc == SP_OK) rc = forward(n, r);
+            if (rc == SP_OK) rc = forward(n, r, false);
             if (rc != SP_OK) return rc;
             MLP_HIP(hipStreamSynchronize(n->st));
         }

@@ -149,6 +149,7 @@ EXPORTS = {
     "sp_mlp_get_params": [_P, _F],
     "sp_mlp_set_params": [_P, _F],
     "sp_mlp_forward": [_P, _F, C.c_int64, _F],
+    "sp_mlp_last_forward_ms": [_P, _D],
     "sp_mlp_gradients": [_P, _F, _F, C.c_int64, _F, _D],
     "sp_mlp_train_step": [_P, _F, _F, C.c_int64, C.c_double, _D],
     "sp_mlp_ensemble_spectra": [C.POINTER(C.c_void_p), C.c_int32, _F, C.c_int64, C.c_double, _F,

@@ -77,6 +77,12 @@ class Mlp:
         self._call("sp_mlp_forward", self._f(x), len(x), self._f(y))
         return y
 
+    def last_forward_ms(self) -> float:
+        """Device time of the layer kernels of the last forward call."""
+        ms = C.c_double()
+        self._call("sp_mlp_last_forward_ms", C.byref(ms))
+        return ms.value
+
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.shape[0])
         t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, self.shape[3])
