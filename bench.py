@@ -6,7 +6,9 @@ Workloads:
                 sized for 2000 diffusion-limited steps per particle.
   3d            3D turbulence snapshot on a 512^3 grid with a compression
                 slab, 1e8 particles, 1000 diffusion-limited steps per interval,
-                particles re-sorted every 200 steps.
+                particles re-sorted every 200 steps; FP32 arithmetic on node
+                records with FP16-stored gradients (--precision f32 for FP32
+                records).
   batch         64 diffusion-parameter variants over the 512^2 2D snapshot in
                 one mover launch: kpara0, kret, momentum exponent and injection
                 region per variant, 1e5 .. 1e7 particles each (1.39e8 total).
@@ -48,9 +50,9 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_STEP = {2: 4 * 15 * 4, 3: 8 * 21 * 4}
 COPY_BYTES = 1 << 30
 DEFAULTS = {
-    "2d": dict(grid=2048, nsteps=2000, nptl=1e8, cpu_nptl=1e6),
-    "3d": dict(grid=512, nsteps=1000, nptl=1e8, cpu_nptl=2e5),
-    "batch": dict(grid=512, nsteps=1000, nptl=0, cpu_nptl=1e6),
+    "2d": dict(grid=2048, nsteps=2000, nptl=1e8, cpu_nptl=1e6, precision="f32"),
+    "3d": dict(grid=512, nsteps=1000, nptl=1e8, cpu_nptl=2e5, precision="f32h"),
+    "batch": dict(grid=512, nsteps=1000, nptl=0, cpu_nptl=1e6, precision="f32"),
 }
 BOX = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
 
@@ -144,17 +146,17 @@ def l1(a, b):
     return float(np.abs(a / a.sum() - b / b.sum()).sum())
 
 
-def accuracy(wl, n, intervals):
-    """f(p) and map L1 of FP32 vs FP64 on the same particles and increments."""
+def accuracy(wl, n, intervals, fast="f32"):
+    """f(p) and map L1 of the fast path vs FP64 on the same particles and increments."""
     out = {}
     res = {}
-    for prec in ("f64", "f32"):
+    for prec in ("f64", fast):
         with ParkerPusher(**dict(wl.cfg, precision=prec, nptl_max=n)) as pz:
             wl.setup(pz, n)
             r = run_intervals(wl, pz, intervals - 1, 1)
             res[prec] = r
             out[f"{prec}_steps_per_s"] = r["steps"] / r["seconds"]
-    f32, f64 = res["f32"]["fp"], res["f64"]["fp"]
+    f32, f64 = res[fast]["fp"], res["f64"]["fp"]
     if wl.variants is not None:
         per = [l1(a, b) for a, b in zip(f32, f64) if b.sum() > 0]
         out["fp_l1"] = max(per)
@@ -162,7 +164,7 @@ def accuracy(wl, n, intervals):
         out["map_l1"] = None
     else:
         out["fp_l1"] = l1(f32, f64)
-        out["map_l1"] = l1(res["f32"]["fmap"], res["f64"]["fmap"])
+        out["map_l1"] = l1(res[fast]["fmap"], res["f64"]["fmap"])
     return out
 
 
@@ -302,7 +304,8 @@ def main():
     ap.add_argument("--nptl", type=float, default=-1, help="particles (default per workload)")
     ap.add_argument("--nsteps", type=int, default=0,
                     help="diffusion-limited steps per interval (default per workload)")
-    ap.add_argument("--precision", choices=("f32", "f64"), default="f32")
+    ap.add_argument("--precision", choices=("f32", "f32h", "f64"), default=None,
+                    help="mover arithmetic (default per workload: f32; f32h for 3d)")
     ap.add_argument("--accuracy-nptl", type=float, default=1e7)
     ap.add_argument("--cpu-nptl", type=float, default=0, help="CPU sample (default per workload)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -318,6 +321,7 @@ def main():
         return
 
     d = DEFAULTS[a.workload]
+    a.precision = a.precision or d["precision"]
     wl = Workload(a.workload, a.grid or d["grid"], a.nsteps or d["nsteps"],
                   d["nptl"] if a.nptl < 0 else a.nptl)
     n = wl.nptl
@@ -361,12 +365,13 @@ def main():
     }
     if not a.no_accuracy:
         na = int(a.accuracy_nptl)
-        log(f"accuracy sample: {na} particles, FP32 vs FP64")
-        acc = accuracy(wl, na, a.warmup + 1)
-        result["accuracy"] = dict(sample_nptl=na, fp_l1_vs_f64=acc["fp_l1"],
+        fast = a.precision if a.precision != "f64" else "f32"
+        log(f"accuracy sample: {na} particles, {fast} vs f64")
+        acc = accuracy(wl, na, a.warmup + 1, fast)
+        result["accuracy"] = dict(sample_nptl=na, path=fast, fp_l1_vs_f64=acc["fp_l1"],
                                   map_l1_vs_f64=acc["map_l1"],
                                   f64_steps_per_s=acc["f64_steps_per_s"],
-                                  f32_steps_per_s=acc["f32_steps_per_s"])
+                                  fast_steps_per_s=acc[f"{fast}_steps_per_s"])
         if "fp_l1_mean" in acc:
             result["accuracy"]["fp_l1_mean_over_variants"] = acc["fp_l1_mean"]
     if not a.no_cpu:

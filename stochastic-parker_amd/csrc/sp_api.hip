@@ -84,7 +84,9 @@ struct Ctx {
     int nvar = 0, nvar_cap = 0;
     unsigned long long* vhist = nullptr;   // per-variant f(p) bins, then per-variant counts
     size_t vhist_len = 0;
-    int mover_blocks_per_cu[2] = {0, 0};
+    int mover_blocks_per_cu[3] = {0, 0, 0};   // per precision (F64, F32, F32H)
+    float gscale[2] = {1.0f, 1.0f};           // FP16 records: gradient scale per frame
+    unsigned* gmax = nullptr;
     // distributions
     bool dist_ready = false;
     sp::DistArgs da{};
@@ -141,7 +143,10 @@ int validate(const sp_config* c) {
     if (c->charge == 0) return fail(SP_ERR_ARG, "charge must be non-zero (the drift prefactor divides by it)");
     if (c->drift1 == 0 && c->drift2 == 0) return fail(SP_ERR_ARG, "drift1 and drift2 are both zero (the drift speed would be infinite)");
     if (c->pbcx < 0 || c->pbcx > 1 || c->pbcy < 0 || c->pbcy > 1 || c->pbcz < 0 || c->pbcz > 1) return fail(SP_ERR_ARG, "pbc must be 0 or 1");
-    if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32) return fail(SP_ERR_ARG, "precision must be SP_PREC_F64 or SP_PREC_F32");
+    if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32 && c->precision != SP_PREC_F32H)
+        return fail(SP_ERR_ARG, "precision must be SP_PREC_F64, SP_PREC_F32 or SP_PREC_F32H");
+    if (c->precision == SP_PREC_F32H && c->ndim != 3)
+        return fail(SP_ERR_UNSUPPORTED, "SP_PREC_F32H (FP16 gradient records) is for 3D grids");
     if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
     if (c->sort_steps < 0 || c->reserved0 != 0) return fail(SP_ERR_ARG, "need sort_steps >= 0 and reserved0 == 0");
     if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
@@ -191,7 +196,7 @@ Ctx* live_ctx(void* p, int& rc) {
 // Occupancy-sized persistent grid for the mover (bpc > 0: that many blocks
 // per CU, within the occupancy limit).
 int mover_grid(Ctx* c, long long n, int bpc = 0) {
-    const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
+    const int prec = c->cfg.precision;
     // Resident 256-lane blocks per CU.  Particles in flight are, on each XCD,
     // a window of consecutive (spatially sorted) indices; a smaller window
     // keeps its field records in that XCD's L2.  Measured on MI355X: two
@@ -314,6 +319,8 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.half_dx = 0.5 * f.dx; m.half_dy = 0.5 * f.dy; m.half_dz = 0.5 * f.dz;
     m.dt_min = f.dt_min_rel * dtf;  // set_dt_min_max (:5519-5524)
     m.dt_max = f.dt_max_rel * dtf;
+    m.f_gs1 = 1.0f / c->gscale[0];
+    m.f_gs2 = 1.0f / (f.time_interp ? c->gscale[1] : c->gscale[0]);
     m.f_kpara0 = kv.f_kpara0;
     m.f_kret = kv.f_kret;
     m.f_pindex = kv.f_pindex;
@@ -354,6 +361,7 @@ void destroy(Ctx* c) {
     for (auto* q : c->dloc) free_dev(q);
     free_dev(c->dvar);
     free_dev(c->vhist);
+    free_dev(c->gmax);
     free_dev(c->pbuf_save);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -382,7 +390,7 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     c->dp = make_params(*cfg);
     c->nmax = cfg->nptl_max;
     c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
-    c->recw = cfg->ndim == 2 ? sp::kRec2D : sp::kRec3D;
+    c->recw = cfg->ndim == 2 ? sp::kRec2D : (cfg->precision == SP_PREC_F32H ? sp::kRec3DH : sp::kRec3D);
     auto bail = [&](int code) {
         destroy(c);
         return code;
@@ -416,6 +424,8 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
         const bool ti = cfg->time_interp != 0, acc = cfg->acc_region_flag == 1;
         c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, acc));
         c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, acc));
+        c->mover_blocks_per_cu[2] =
+            cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, acc)) : 1;
     }
     SP_HIP_C(hipStreamSynchronize(c->st));
 #undef SP_HIP_C
@@ -454,7 +464,29 @@ int sp_fields_gradients(void* ctx, int32_t slot, float* farray32) {
     if (!c->rec[slot]) SP_HIP(hipMalloc(&c->rec[slot], sizeof(float) * c->recw * c->nnodes));
     float* fa = nullptr;
     if (farray32) SP_HIP(hipMalloc(&fa, sizeof(float) * 32 * c->nnodes));
-    hipError_t e = sp::launch_gradients(c->dp, c->n8[slot], c->rec[slot], fa, c->st);
+    hipError_t e = hipSuccess;
+    if (c->cfg.precision == SP_PREC_F32H) {
+        // FP16 gradients: the largest |gradient| of the frame sets a power-of-two
+        // scale that maps it to 2^14 (exact to undo; no FP16 overflow)
+        if (!c->gmax) e = hipMalloc(&c->gmax, sizeof(unsigned));
+        unsigned bits = 0;
+        if (e == hipSuccess) e = hipMemsetAsync(c->gmax, 0, sizeof(unsigned), c->st);
+        if (e == hipSuccess) e = sp::launch_gradient_max(c->dp, c->n8[slot], c->gmax, c->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&bits, c->gmax, sizeof(unsigned), hipMemcpyDeviceToHost, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        float gmax_v = 0.0f;
+        std::memcpy(&gmax_v, &bits, sizeof(float));
+        float scale = 1.0f;
+        if (gmax_v > 0.0f && std::isfinite(gmax_v)) {
+            int ex = 0;
+            std::frexp(16384.0f / gmax_v, &ex);   // 16384 / gmax in [2^(ex-1), 2^ex)
+            scale = std::ldexp(1.0f, ex - 1);
+        }
+        c->gscale[slot] = scale;
+        if (e == hipSuccess) e = sp::launch_pack_half(c->dp, c->n8[slot], scale, c->rec[slot], fa, c->st);
+    } else {
+        e = sp::launch_gradients(c->dp, c->n8[slot], c->rec[slot], fa, c->st);
+    }
     if (e == hipSuccess && farray32)
         e = hipMemcpyAsync(farray32, fa, sizeof(float) * 32 * c->nnodes, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -475,6 +507,7 @@ int sp_fields_copy(void* ctx) {
     SP_HIP(hipMemcpyAsync(c->rec[0], c->rec[1], sizeof(float) * c->recw * c->nnodes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipMemcpyAsync(c->n8[0], c->n8[1], sizeof(float) * 8 * c->nnodes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
+    c->gscale[0] = c->gscale[1];
     c->have_fields[0] = true;
     return SP_OK;
 }
@@ -590,7 +623,8 @@ int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_inte
         SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
         SP_HIP(hipEventRecord(c->ev[1], c->st));
         const int grid = mover_grid(c, nrun, bpc);
-        if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
+        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, acc, grid, c->st));
+        else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
         else SP_HIP(sp::launch_mover_f64(m, ti, acc, c->cfg.ndim, grid, c->st));
         SP_HIP(hipEventRecord(c->ev[3], c->st));
         unsigned long long sp_[2] = {0, 0};
@@ -721,7 +755,9 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     m.n = n;
     const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
     if (e == hipSuccess) {
-        if (c->cfg.precision == SP_PREC_F32)
+        if (c->cfg.precision == SP_PREC_F32H)
+            e = sp::launch_step_once_f32h(m, ti, acc, ds, dr, df, dd, c->st);
+        else if (c->cfg.precision == SP_PREC_F32)
             e = sp::launch_step_once_f32(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
         else
             e = sp::launch_step_once_f64(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
@@ -859,7 +895,7 @@ int sp_autotune_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     SP_HIP(hipMemcpyAsync(&saved, c->ctr, sizeof(saved), hipMemcpyDeviceToHost, c->st));
     SP_HIP(hipMemcpyAsync(c->pbuf_save, c->pbuf[cur0], bytes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
-    const int prec = c->cfg.precision == SP_PREC_F32 ? 1 : 0;
+    const int prec = c->cfg.precision;
     const int max_bpc = std::max(1, std::min(4, c->mover_blocks_per_cu[prec]));
     std::vector<int> budgets = {0};
     if (c->cfg.sort_every > 0) budgets = {0, 100, 200, 400};

@@ -16,6 +16,10 @@ namespace sp {
 // holds 21 components plus three pads (96 B = six dwordx4 loads).
 constexpr int kRec2D = 16;
 constexpr int kRec3D = 24;
+// 3D record with FP16 gradients (SP_PREC_F32H), in 4-byte words: V and B as
+// six floats, then the 15 gradient slots of Rec3 (R3_DVXDX ..) as halves, two
+// per word, then padding (64 B = four dwordx4 loads).
+constexpr int kRec3DH = 16;
 
 // 2D record slots
 enum Rec2 { R2_VX, R2_VY, R2_BX, R2_BY, R2_BZ, R2_DVXDX, R2_DVYDY, R2_DBXDX, R2_DBXDY,
@@ -148,6 +152,7 @@ struct MoverParams {
     // FP32 physics
     float f_kpara0, f_kret, f_pindex, f_gm2, f_vdp_pre, f_d1p0, f_d2p02;
     float f_half_dx, f_half_dy, f_half_dz, f_dt_min, f_dt_max;
+    float f_gs1, f_gs2;           // FP16 records: 1 / gradient scale of frames 1 and 2
     double inv_p0;
     // acceleration region (template-enabled)
     double acc_region[6];
@@ -160,6 +165,17 @@ int set_error(int code, const std::string& msg);
 // Launchers (defined in the .hip translation units)
 hipError_t launch_gradients(const DevParams& prm, const float* nodes8, float* rec, float* farray32,
                             hipStream_t s);
+// FP16-gradient 3D records: max |gradient| over the 15 stored slots (as float
+// bits, into *maxbits), then the packing with the power-of-two scale.
+hipError_t launch_gradient_max(const DevParams& prm, const float* nodes8, unsigned* maxbits,
+                               hipStream_t s);
+hipError_t launch_pack_half(const DevParams& prm, const float* nodes8, float scale, float* rec,
+                            float* farray32, hipStream_t s);
+int mover_blocks_per_cu_f32h(bool ti, bool acc);
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, bool acc, int grid, hipStream_t s);
+hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, bool acc, double* state,
+                                 const double* rands, const int8_t* fixed, double* deltas,
+                                 hipStream_t s);
 int mover_blocks_per_cu_f64(int ndim, bool ti, bool acc);
 int mover_blocks_per_cu_f32(int ndim, bool ti, bool acc);
 hipError_t launch_mover_f64(const MoverParams& m, bool ti, bool acc, int ndim, int grid,

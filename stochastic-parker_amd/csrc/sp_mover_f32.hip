@@ -6,6 +6,7 @@
 namespace sp {
 
 struct PolicyF32 {
+    static constexpr bool kOnly3D = false;
     template <int NDIM, bool TI, bool ACC>
     __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
                                                 uint32_t tagi, uint32_t tags, uint32_t rstep,
@@ -21,6 +22,39 @@ struct PolicyF32 {
         f32::gather_kappa_push<NDIM, TI, ACC>(m, s, fixed, rf, d);
     }
 };
+
+// FP32 arithmetic on 3D records with FP16 gradients.
+struct PolicyF32H {
+    static constexpr bool kOnly3D = true;
+    template <int NDIM, bool TI, bool ACC>
+    __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
+                                                uint32_t tagi, uint32_t tags, uint32_t rstep,
+                                                double d[4]) {
+        float r[4];
+        f32::step_rands(m, tagi, tags, rstep, r);
+        f32::gather_kappa_push<NDIM, TI, ACC, true>(m, s, fixed, r, d);
+    }
+    template <int NDIM, bool TI, bool ACC>
+    __device__ __forceinline__ static void step_given(const MoverParams& m, State& s, bool fixed,
+                                                      const double* r, double d[4]) {
+        const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
+        f32::gather_kappa_push<NDIM, TI, ACC, true>(m, s, fixed, rf, d);
+    }
+};
+
+int mover_blocks_per_cu_f32h(bool ti, bool acc) {
+    return mover_occupancy<PolicyF32H>(3, ti, acc);
+}
+
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, bool acc, int grid, hipStream_t s) {
+    return launch_mover<PolicyF32H>(m, ti, acc, 3, grid, s);
+}
+
+hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, bool acc, double* state,
+                                 const double* rands, const int8_t* fixed, double* deltas,
+                                 hipStream_t s) {
+    return launch_step_once<PolicyF32H>(m, ti, acc, 3, state, rands, fixed, deltas, s);
+}
 
 int mover_blocks_per_cu_f32(int ndim, bool ti, bool acc) {
     return mover_occupancy<PolicyF32>(ndim, ti, acc);

@@ -6,6 +6,7 @@
 namespace sp {
 
 struct PolicyF64 {
+    static constexpr bool kOnly3D = false;
     template <int NDIM, bool TI, bool ACC>
     __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
                                                 uint32_t tagi, uint32_t tags, uint32_t rstep,

@@ -307,12 +307,15 @@ struct Variant {
     static constexpr bool acc = A;
 };
 
-// Calls f(Variant<...>{}) for the runtime (ndim, time_interp, acc_region) combination.
-template <class F>
+// Calls f(Variant<...>{}) for the runtime (ndim, time_interp, acc_region)
+// combination; a policy with kOnly3D builds (and serves) 3D variants only.
+template <bool ONLY3D = false, class F>
 auto dispatch_variant(int ndim, bool ti, bool acc, F&& f) {
-    if (ndim == 2) {
-        if (ti) return acc ? f(Variant<2, true, true>{}) : f(Variant<2, true, false>{});
-        return acc ? f(Variant<2, false, true>{}) : f(Variant<2, false, false>{});
+    if constexpr (!ONLY3D) {
+        if (ndim == 2) {
+            if (ti) return acc ? f(Variant<2, true, true>{}) : f(Variant<2, true, false>{});
+            return acc ? f(Variant<2, false, true>{}) : f(Variant<2, false, false>{});
+        }
     }
     if (ti) return acc ? f(Variant<3, true, true>{}) : f(Variant<3, true, false>{});
     return acc ? f(Variant<3, false, true>{}) : f(Variant<3, false, false>{});
@@ -320,7 +323,7 @@ auto dispatch_variant(int ndim, bool ti, bool acc, F&& f) {
 
 template <class Pol>
 int mover_occupancy(int ndim, bool ti, bool acc) {
-    return dispatch_variant(ndim, ti, acc, [](auto v) {
+    return dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [](auto v) {
         using V = decltype(v);
         int nb = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -331,7 +334,7 @@ int mover_occupancy(int ndim, bool ti, bool acc) {
 
 template <class Pol>
 hipError_t launch_mover(const MoverParams& m, bool ti, bool acc, int ndim, int grid, hipStream_t st) {
-    dispatch_variant(ndim, ti, acc, [&](auto v) {
+    dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [&](auto v) {
         using V = decltype(v);
         hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::acc>), dim3(grid), dim3(256), 0,
                            st, m);
@@ -346,7 +349,7 @@ hipError_t launch_step_once(const MoverParams& m, bool ti, bool acc, int ndim, d
                             hipStream_t st) {
     if (m.n <= 0) return hipSuccess;
     const int grid = (int)((m.n + 255) / 256);
-    dispatch_variant(ndim, ti, acc, [&](auto v) {
+    dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [&](auto v) {
         using V = decltype(v);
         hipLaunchKernelGGL((step_once_kernel<Pol, V::ndim, V::ti, V::acc>), dim3(grid), dim3(256),
                            0, st, m, state, rands, fixed, deltas);

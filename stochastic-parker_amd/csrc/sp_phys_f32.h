@@ -97,6 +97,60 @@ __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, fl
 }
 
 
+// Gather from 3D records with FP16 gradients (kRec3DH): V, B as floats, the 15
+// gradient slots as halves scaled by the frame's power of two; the scale is
+// undone after the sum over corners (scaling by a power of two is exact).
+template <bool TINTERP>
+__device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& I, float rt,
+                                            float* __restrict__ f) {
+    constexpr int NC = 21;
+    const long long sx = 1, sy = m.nxg, sz = (long long)m.nxg * m.nyg;
+    const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
+    float g[TINTERP ? NC : 1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        f[c] = 0.0f;
+        if (TINTERP) g[c] = 0.0f;
+    }
+#pragma unroll
+    for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
+        const float* rec = frame == 0 ? m.rec1 : m.rec2;
+        float* acc = frame == 0 ? f : g;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
+            const long long node = base + i * sx + j * sy + k * sz;
+            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * kRec3DH);
+            uint32_t wd[14];
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const uint4 t = r4[u];
+                wd[4 * u] = t.x; wd[4 * u + 1] = t.y; wd[4 * u + 2] = t.z; wd[4 * u + 3] = t.w;
+            }
+            {
+                const uint4 t = r4[3];
+                wd[12] = t.x; wd[13] = t.y;
+            }
+            const float wq = I.w[q];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] += __uint_as_float(wd[c]) * wq;
+#pragma unroll
+            for (int c = 6; c < NC; ++c) {
+                const uint32_t word = wd[6 + (c - 6) / 2];
+                const uint16_t bits = (uint16_t)(((c - 6) & 1) ? (word >> 16) : (word & 0xFFFFu));
+                acc[c] += (float)__builtin_bit_cast(_Float16, bits) * wq;
+            }
+        }
+        const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
+#pragma unroll
+        for (int c = 6; c < NC; ++c) acc[c] *= gs;
+    }
+    if (TINTERP) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c] = f[c] * (1.0f - rt) + g[c] * rt;
+    }
+}
+
 // One Parker step in FP32 coefficients.  Semantics match f64::push: same dt
 // rule (including the dt_min fallback when a rate is exactly zero), same
 // increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.
@@ -269,14 +323,15 @@ __device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i,
     r[3] = (float)u.d * sc + of;
 }
 
-template <int NDIM, bool TINTERP, bool ACC>
+template <int NDIM, bool TINTERP, bool ACC, bool HALF = false>
 __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
                                                   const float r[4], double d[4]) {
     constexpr int NC = NDIM == 2 ? 15 : 21;
     float f[NC];
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
-    gather<NDIM, TINTERP>(m, I, rt, f);
+    if constexpr (HALF && NDIM == 3) gather_half<TINTERP>(m, I, rt, f);
+    else gather<NDIM, TINTERP>(m, I, rt, f);
     kappa_push<NDIM, ACC>(m, s, f, fixed_dt, r, d);
 }
 

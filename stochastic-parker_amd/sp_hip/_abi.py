@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsp_hip.so")
 SP_OK = 0
 SP_PREC_F64 = 0
 SP_PREC_F32 = 1
+SP_PREC_F32H = 2
 COUNT_FLAG_INBOX = 1
 COUNT_FLAG_OTHERS = 0
 

@@ -48,7 +48,8 @@ def make_config(**kw) -> SpConfig:
         raise KeyError(f"unknown config keys: {sorted(unknown)}")
     d.update(kw)
     if isinstance(d["precision"], str):
-        d["precision"] = {"f64": _abi.SP_PREC_F64, "f32": _abi.SP_PREC_F32}[d["precision"]]
+        d["precision"] = {"f64": _abi.SP_PREC_F64, "f32": _abi.SP_PREC_F32,
+                          "f32h": _abi.SP_PREC_F32H}[d["precision"]]
     c = SpConfig()
     for k, v in d.items():
         if k == "acc_region":

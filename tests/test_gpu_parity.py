@@ -316,3 +316,58 @@ def test_spatial_sort_is_a_stable_permutation():
     same = np.diff(k.astype(np.int64)) == 0
     assert np.all(np.diff(g["tag_injected"])[same] > 0)
     pz.close()
+
+
+def test_f32h_is_3d_only():
+    with pytest.raises(Exception) as ei:
+        ParkerPusher(**dict(cfg2d(16), precision="f32h"))
+    assert "3D" in str(ei.value)
+
+
+def test_f32h_steps_close_to_f64():
+    """FP16-stored gradients change only the drift and kappa-divergence terms,
+    by about the FP16 rounding (2^-11 relative) of the gradients."""
+    c = cfg3d(12, mag_dependency=0)
+    nodes = nodes_for(c)
+    n = 3000
+    a = random_states(c, n, seed=21)
+    rng = np.random.default_rng(22)
+    r = rng.uniform(-SQ3, SQ3, size=(n, 4))
+    fixed = np.zeros(n, dtype=np.int8)
+    state = np.stack([a[k] for k in STATE_COLS], 1)
+    out = {}
+    for prec in ("f64", "f32", "f32h"):
+        pz, _ = make(c, nodes, precision=prec)
+        out[prec] = pz.step_once(state, r, fixed, 0.0, 0.05)
+        pz.close()
+    d64, d32, d16 = out["f64"][1], out["f32"][1], out["f32h"][1]
+    for d in (d32, d16):
+        scale = np.abs(d64[:, :3]).max(axis=1, keepdims=True)
+        rel = np.abs(d[:, :3] - d64[:, :3]) / scale
+        assert np.median(rel) < 2e-4 and rel.max() < 2e-2
+    # the FP16 records move the result by more than FP32 rounding, but not much
+    e32 = np.abs(d32[:, :3] - d64[:, :3]).mean()
+    e16 = np.abs(d16[:, :3] - d64[:, :3]).mean()
+    assert e16 > e32 and e16 < 1e3 * e32 + 1e-12
+
+
+def test_f32h_mover_distribution_close_to_f64():
+    c = cfg3d(12, pbcx=1, dt_max_rel=1e-2)
+    nodes = nodes_for(c)
+    res = {}
+    for prec in ("f64", "f32h"):
+        pz, _ = make(c, nodes, precision=prec, nptl_max=8000)
+        pz.inject_particles_spatial_uniform(6000, 1e-7, 1, 5.0, 0.0, 0.2, (0, 0, 0, 1, 1, 1))
+        pz.particle_mover(0.0, 0.2, 1, 10)
+        res[prec] = by_id(pz.get_particles())
+        pz.close()
+    a, b = res["f64"], res["f32h"]
+    common = np.intersect1d(a["tag_injected"], b["tag_injected"])
+    assert len(common) > 0.98 * len(a["p"])
+    ia = np.searchsorted(a["tag_injected"], common)
+    ib = np.searchsorted(b["tag_injected"], common)
+    dp = np.abs(a["p"][ia] - b["p"][ib]) / a["p"][ia]
+    assert np.median(dp) < 1e-4
+    h64, _ = np.histogram(np.log(a["p"]), bins=40, range=(-1, 1))
+    h16, _ = np.histogram(np.log(b["p"]), bins=40, range=(-1, 1))
+    assert np.abs(h64 / h64.sum() - h16 / h16.sum()).sum() < 0.01
