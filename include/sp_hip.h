@@ -274,7 +274,9 @@ int sp_mlp_set_data(void* mlp, const float* x, const float* t, int64_t rows);
 /* nsteps AdamW steps on device-drawn mini-batches of `batch` rows (with
  * replacement, Philox keyed on seed and step), learning rate decaying from
  * lr0 to lr_min on a cosine; losses[s] receives each step's batch loss (NULL
- * skips).  The loop runs without host synchronisation. */
+ * skips).  Every kernel of a step reads the step number from device memory,
+ * so one step is captured as a HIP graph and replayed nsteps times, without
+ * host synchronisation (SP_MLP_EAGER=1 launches the same kernels one by one). */
 int sp_mlp_train(void* mlp, int64_t nsteps, int32_t batch, double lr0, double lr_min,
                  uint64_t seed, float* losses);
 

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -306,6 +307,61 @@ __global__ void __launch_bounds__(256) colsum_kernel(const uint16_t* gmat, int M
     if (threadIdx.x == 0) out[c] = red[0];
 }
 
+// State of a device-side training loop: t counts the AdamW steps taken; the
+// loop's kernels read it, so one captured step replays unchanged.
+struct TrainState {
+    uint32_t t, t0, nsteps, pad;
+    float lr0, lr_min, b1, b2;
+};
+
+__global__ void batch_index_dev_kernel(uint32_t* idx, int nrow, uint32_t ndata,
+                                       const TrainState* ts, uint32_t k0, uint32_t k1) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const U4 u = philox4x32_10((uint32_t)r, ts->t - ts->t0, 0x6d6c70u, 0u, k0, k1);
+    idx[r] = (uint32_t)(((unsigned long long)u.a * ndata) >> 32);
+}
+
+// loss_hist[t - t0] = scale * sum(part[0 .. np)), one block.
+__global__ void __launch_bounds__(256) loss_dev_kernel(const float* part, int np, float scale,
+                                                       float* loss_hist, const TrainState* ts) {
+    __shared__ float red[256];
+    float acc = 0.0f;
+    for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_hist[ts->t - ts->t0] = red[0] * scale;
+}
+
+// AdamW step t + 1 with the learning rate of the loop's cosine schedule.
+__global__ void adamw_dev_kernel(float* p, const float* gr, float* m, float* v, uint16_t* pb,
+                                 long long n, const TrainState* ts, float eps, float wd) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = ts->t - ts->t0;
+    const float f = ts->nsteps > 1 ? (float)s / (float)(ts->nsteps - 1) : 0.0f;
+    const float lr = ts->lr_min + 0.5f * (ts->lr0 - ts->lr_min) * (1.0f + cosf(3.14159265358979f * f));
+    const float tt = (float)(ts->t + 1);
+    const float c1 = 1.0f - powf(ts->b1, tt), c2 = 1.0f - powf(ts->b2, tt);
+    const float g = gr[i];
+    const float mi = ts->b1 * m[i] + (1.0f - ts->b1) * g;
+    const float vi = ts->b2 * v[i] + (1.0f - ts->b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float upd = (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float pi = p[i] - lr * (upd + wd * p[i]);
+    p[i] = pi;
+    pb[i] = bf16_rne(pi);
+}
+
+__global__ void train_tick_kernel(TrainState* ts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ts->t += 1u;
+}
+
 // AdamW (decoupled weight decay) with bias correction; refreshes the BF16 copy.
 __global__ void adamw_kernel(float* p, const float* gr, float* m, float* v, uint16_t* pb,
                              long long n, float lr, float b1, float b2, float eps, float wd,
@@ -356,6 +412,7 @@ struct Mlp {
     float* wpart = nullptr;             // split-K partial weight gradients
     hipEvent_t ev[2] = {nullptr, nullptr};
     double fwd_ms = 0.0;                // device time of the last sp_mlp_forward's layers
+    sp::mlp::TrainState* ts = nullptr;  // device-side training loop state
     float* ens_spec = nullptr;          // ensemble spectra and spreads (batch_max rows)
     float* ens_spread = nullptr;
     long long step = 0;                 // AdamW step count
@@ -376,7 +433,7 @@ void destroy(Mlp* n) {
     for (auto* q : n->z) dfree(q);
     dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
     dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
-    dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread);
+    dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread); dfree(n->ts);
     for (auto& e : n->ev)
         if (e) (void)hipEventDestroy(e);
     if (n->st) (void)hipStreamDestroy(n->st);
@@ -436,16 +493,20 @@ int forward(Mlp* n, int rows, bool train = true) {
     return SP_OK;
 }
 
-// Loss, gradients of all parameters into n->g; loss_out (device) receives the
-// mean squared error.
-int backward(Mlp* n, int rows, float* loss_out) {
+// Loss, gradients of all parameters into n->g; the mean squared error goes to
+// loss_out (device), or with a training state to loss_out[t - t0].
+int backward(Mlp* n, int rows, float* loss_out, const sp::mlp::TrainState* ts = nullptr) {
     const long long cnt = (long long)rows * n->cfg.nout;
     const float scale = 1.0f / (float)cnt;
     hipLaunchKernelGGL(sp::mlp::mse_kernel, dim3(kPartBlocks), dim3(256), 0, n->st, n->y, n->t, cnt,
                        scale, n->gh[0], n->part);
     MLP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sp::mlp::sum_scale_kernel, dim3(1), dim3(256), 0, n->st, n->part,
-                       kPartBlocks, scale, loss_out);
+    if (ts)
+        hipLaunchKernelGGL(sp::mlp::loss_dev_kernel, dim3(1), dim3(256), 0, n->st, n->part,
+                           kPartBlocks, scale, loss_out, ts);
+    else
+        hipLaunchKernelGGL(sp::mlp::sum_scale_kernel, dim3(1), dim3(256), 0, n->st, n->part,
+                           kPartBlocks, scale, loss_out);
     MLP_HIP(hipGetLastError());
     int cur = 0;
     for (int l = n->L - 1; l >= 0; --l) {
@@ -775,21 +836,62 @@ int sp_mlp_train(void* q, int64_t nsteps, int32_t batch, double lr0, double lr_m
         MLP_HIP(hipMalloc(&n->loss_hist, 4 * (size_t)nsteps));
         n->loss_cap = nsteps;
     }
+    if (!n->ts) MLP_HIP(hipMalloc(&n->ts, sizeof(sp::mlp::TrainState)));
+    if (nsteps > 0xFFFFFFFFll - n->step) return sp::set_error(SP_ERR_ARG, "too many steps");
+    sp::mlp::TrainState hs{};
+    hs.t = hs.t0 = (uint32_t)n->step;
+    hs.nsteps = (uint32_t)nsteps;
+    hs.lr0 = (float)lr0;
+    hs.lr_min = (float)lr_min;
+    hs.b1 = (float)n->cfg.beta1;
+    hs.b2 = (float)n->cfg.beta2;
+    MLP_HIP(hipMemcpyAsync(n->ts, &hs, sizeof(hs), hipMemcpyHostToDevice, n->st));
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    for (int64_t s = 0; s < nsteps; ++s) {
-        hipLaunchKernelGGL(sp::mlp::batch_index_kernel, dim3(blocks(batch)), dim3(256), 0, n->st,
-                           n->bidx, batch, (uint32_t)n->nd, (uint32_t)s, k0, k1);
+    // One training step: every kernel reads the step from n->ts, so the same
+    // launch sequence serves every step (and is captured once as a graph).
+    auto one_step = [&]() -> int {
+        hipLaunchKernelGGL(sp::mlp::batch_index_dev_kernel, dim3(blocks(batch)), dim3(256), 0, n->st,
+                           n->bidx, batch, (uint32_t)n->nd, n->ts, k0, k1);
         hipLaunchKernelGGL(sp::mlp::gather_kernel, dim3((unsigned)batch), dim3(64), 0, n->st, n->xd,
                            n->td, n->bidx, batch, n->cfg.nin, n->cfg.nout, n->xh[0], n->t);
         MLP_HIP(hipGetLastError());
-        rc = forward(n, batch);
-        if (rc == SP_OK) rc = backward(n, batch, n->loss_hist + s);
-        // cosine decay from lr0 to lr_min
-        const double f = nsteps > 1 ? (double)s / (double)(nsteps - 1) : 0.0;
-        const double lr = lr_min + 0.5 * (lr0 - lr_min) * (1.0 + std::cos(3.141592653589793 * f));
-        if (rc == SP_OK) rc = adamw(n, lr);
-        if (rc != SP_OK) return rc;
+        int r = forward(n, batch);
+        if (r == SP_OK) r = backward(n, batch, n->loss_hist, n->ts);
+        if (r != SP_OK) return r;
+        hipLaunchKernelGGL(sp::mlp::adamw_dev_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->p,
+                           n->g, n->m, n->v, n->pb, n->P, n->ts, (float)n->cfg.eps,
+                           (float)n->cfg.weight_decay);
+        hipLaunchKernelGGL(sp::mlp::train_tick_kernel, dim3(1), dim3(64), 0, n->st, n->ts);
+        MLP_HIP(hipGetLastError());
+        return SP_OK;
+    };
+    const char* eager_env = std::getenv("SP_MLP_EAGER");
+    const bool eager = eager_env && std::atoi(eager_env) != 0;
+    if (eager || nsteps < 2) {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            rc = one_step();
+            if (rc != SP_OK) return rc;
+        }
+    } else {
+        // the step's ~30 launches are recorded once and replayed nsteps times
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        MLP_HIP(hipStreamBeginCapture(n->st, hipStreamCaptureModeThreadLocal));
+        rc = one_step();
+        hipError_t e = hipStreamEndCapture(n->st, &graph);
+        if (rc != SP_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (int64_t s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(exec, n->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(n->st);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+            return sp::set_error(SP_ERR_HIP, std::string("training graph failed: ") + hipGetErrorString(e));
     }
+    n->step += nsteps;
     if (losses && nsteps > 0)
         MLP_HIP(hipMemcpyAsync(losses, n->loss_hist, 4 * (size_t)nsteps, hipMemcpyDeviceToHost, n->st));
     MLP_HIP(hipStreamSynchronize(n->st));

@@ -89,3 +89,23 @@ def test_ensemble_spectra_match_members():
     np.testing.assert_allclose(spec, mean, rtol=1e-5, atol=1e-7 + 1e-5 * FLOOR)
     ref = 0.5 * (np.abs(ys[0] - mean).sum(1) + np.abs(ys[1] - mean).sum(1))
     np.testing.assert_allclose(spread, ref, rtol=1e-4, atol=1e-6)
+
+
+def test_training_graph_replays_match_eager(monkeypatch):
+    """sp_mlp_train records one step as a HIP graph and replays it; every
+    chunk of replays, with other work on the stream between chunks, equals
+    the same steps launched one by one."""
+    x, t = _data(8000, seed=6)
+    with _net(seed=9, batch_max=1024) as g, _net(seed=9, batch_max=1024) as e:
+        g.set_data(x, t)
+        e.set_data(x, t)
+        for _ in range(3):
+            lg = g.train(40, batch=1024, lr0=1e-3, lr_min=1e-4, seed=3)
+            monkeypatch.setenv("SP_MLP_EAGER", "1")
+            le = e.train(40, batch=1024, lr0=1e-3, lr_min=1e-4, seed=3)
+            monkeypatch.delenv("SP_MLP_EAGER")
+            g.forward(x[:300])
+            e.forward(x[:300])
+            np.testing.assert_array_equal(lg, le)
+            np.testing.assert_array_equal(g.get_params(), e.get_params())
+        assert lg[-1] < lg[0]
