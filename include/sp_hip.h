@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SP_HIP_ABI_VERSION 3
+#define SP_HIP_ABI_VERSION 4
 
 typedef enum sp_status {
     SP_OK = 0,
@@ -124,7 +124,7 @@ typedef struct sp_local_dist {
 typedef struct sp_dist_config {
     int32_t npp_global, nmu_global;
     int32_t local_dist;           /* -ld */
-    int32_t reserved0;
+    int32_t escaped_dist;         /* -ded: accumulate distributions of escaped particles */
     sp_local_dist local[4];
 } sp_dist_config;
 
@@ -290,6 +290,19 @@ int sp_distributions_init(void* ctx, const sp_dist_config* dc);
 int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal2,
                      double* flocal3, double* flocal4);
 int sp_local_dist_dims(void* ctx, int32_t set, int64_t dims[5]);
+/* remove_particles' copy of escaped particles (particle_module.f90:5382-5390),
+ * calc_escaped_distributions (diagnostics.f90:913-1231),
+ * clean_escaped_distributions (:492-584) and reset_escaped_particles
+ * (:2227-2246) in one: every particle that sp_particle_mover removed with an
+ * escape flag since the previous call is binned by the mover's removal pass;
+ * this returns the bins and clears them.  Needs escaped_dist in
+ * sp_distributions_init.  fescaped is (nmu_global, npp_global, 2 ndim), the
+ * last index the face |count_flag|.  fescK is set K's face arrays back to back:
+ * x faces (nmu, npbins, nry, nrz, 2), y faces (nmu, npbins, nrx, nrz, 2) and in
+ * 3D z faces (nmu, npbins, nrx, nry, 2), dimensions as sp_local_dist_dims;
+ * NULL skips an array. */
+int sp_escaped_distributions(void* ctx, double* fescaped, double* fesc1, double* fesc2,
+                             double* fesc3, double* fesc4);
 /* No reference counterpart: a plain device-to-device streaming copy of
  * `bytes` (multiple of 16) on the context stream, timed with HIP events.  It
  * calibrates hardware memory counters against a kernel of known traffic. */

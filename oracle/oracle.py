@@ -114,6 +114,9 @@ def load():
         "orc_local_dist_size": (C.c_int64, [P, C.POINTER(OrcDistConfig), C.c_int]),
         "orc_calc_particle_distributions": (None, [P, C.POINTER(OrcDistConfig), _PP, C.c_int64,
                                                    _D, C.POINTER(_D)]),
+        "orc_escaped_dist_size": (C.c_int64, [P, C.POINTER(OrcDistConfig), C.c_int]),
+        "orc_calc_escaped_distributions": (None, [P, C.POINTER(OrcDistConfig), _PP, C.c_int64,
+                                                  _D, C.POINTER(_D)]),
         "orc_quick_check": (None, [_PP, C.c_int64, C.c_int64, C.c_double, C.c_double, _D]),
         "orc_sizeof_particle": (C.c_int, []),
     }
@@ -188,8 +191,10 @@ def soa_from_particles(a: np.ndarray) -> dict:
 
 
 def mover(prm, fa1, ptls, t0, dtf, num_fine_steps=1, nsteps_interval=10, rng_mode=0,
-          nthreads=1, fa2=None):
-    """Runs orc_particle_mover in place on a copy; returns (particles, acc)."""
+          nthreads=1, fa2=None, return_removed=False):
+    """Runs orc_particle_mover in place on a copy; returns (particles, acc), and
+    with return_removed also the removed particles (remove_particles swaps
+    them to the tail of the array, where they are read back)."""
     lib = load()
     buf = np.ascontiguousarray(ptls.copy())
     n = C.c_int64(len(buf))
@@ -197,6 +202,8 @@ def mover(prm, fa1, ptls, t0, dtf, num_fine_steps=1, nsteps_interval=10, rng_mod
     lib.orc_particle_mover(C.byref(prm), fptr(fa1), fptr(fa2 if fa2 is not None else fa1),
                            buf.ctypes.data_as(_PP), C.byref(n), t0, dtf, num_fine_steps,
                            nsteps_interval, rng_mode, nthreads, C.byref(acc))
+    if return_removed:
+        return buf[: n.value].copy(), acc, buf[n.value:].copy()
     return buf[: n.value].copy(), acc
 
 
@@ -276,6 +283,25 @@ def distributions(prm, dcfg: dict, ptls):
     lib.orc_calc_particle_distributions(C.byref(prm), C.byref(dc), buf.ctypes.data_as(_PP),
                                         len(buf), dptr(fg), arr)
     return fg, locs
+
+
+def escaped_distributions(prm, dcfg: dict, escaped):
+    """calc_escaped_distributions over the given removed particles (those with
+    count_flag >= 0 are skipped).  Returns (fescaped, [set arrays or None])."""
+    lib = load()
+    dc = dist_config(dcfg)
+    fe = np.zeros(lib.orc_escaped_dist_size(C.byref(prm), C.byref(dc), -1))
+    locs = []
+    arr = (_D * 4)()
+    for k in range(4):
+        size = lib.orc_escaped_dist_size(C.byref(prm), C.byref(dc), k)
+        a = np.zeros(max(size, 1))
+        locs.append(a if size else None)
+        arr[k] = dptr(a)
+    buf = np.ascontiguousarray(escaped)
+    lib.orc_calc_escaped_distributions(C.byref(prm), C.byref(dc), buf.ctypes.data_as(_PP),
+                                       len(buf), dptr(fe), arr)
+    return fe, locs
 
 
 def quick_check(ptls, nptl_split, leak, leak_negp):

@@ -991,6 +991,88 @@ void orc_calc_particle_distributions(const orc_params* prm, const orc_dist_confi
     }
 }
 
+/* Sizes of calc_escaped_distributions' arrays: k < 0 the global fescaped
+ * (nmu_global, npp_global, 2 ndim), k = 0..3 local set k+1's face arrays
+ * x (nmu, npbins, nry, nrz, 2), y (nmu, npbins, nrx, nrz, 2) and in 3D
+ * z (nmu, npbins, nrx, nry, 2) back to back (diagnostics.f90:188-191,359-395). */
+int64_t orc_escaped_dist_size(const orc_params* prm, const orc_dist_config* dc, int k)
+{
+    if (k < 0) return (int64_t)dc->nmu_global * dc->npp_global * 2 * prm->ndim;
+    if (!dc->local_dist || !dc->local[k][0]) return 0;
+    int64_t nr[3];
+    double dd[3];
+    local_geometry(prm, dc, k, nr, dd);
+    const int64_t per = (int64_t)dc->local[k][2] * dc->local[k][1] * 2;
+    return per * (nr[1] * nr[2] + nr[0] * nr[2] + (prm->ndim > 2 ? nr[0] * nr[1] : 0));
+}
+
+/* calc_escaped_distributions, diagnostics.f90:913-1173, over the particles
+ * remove_particles copied out (:5382-5390), Cartesian grid.  The global bins
+ * take p in (pmin, pmax] and mu in [-1, 1] with face index |count_flag|;
+ * local set k bins the two coordinates along the escape face and drops the
+ * top momentum bin (condp = ip > 0 and ip < npbins).  Accumulates into the
+ * caller's zeroed arrays (sizes from orc_escaped_dist_size). */
+void orc_calc_escaped_distributions(const orc_params* prm, const orc_dist_config* dc,
+                                    const orc_particle* escaped, int64_t nesc,
+                                    double* fescaped, double* fesc[4])
+{
+    const int npp = dc->npp_global, nmu = dc->nmu_global;
+    const double pmin_log = log10(prm->pmin);
+    const double dp_log = (log10(prm->pmax) - pmin_log) / npp;
+    const double dmu = (double)(2.0f / (float)nmu);
+    int64_t nr[4][3];
+    double dd[4][3], lp0[4], ldp[4], ldmu[4];
+    for (int k = 0; k < 4; ++k) {
+        if (!dc->local_dist || !dc->local[k][0]) continue;
+        local_geometry(prm, dc, k, nr[k], dd[k]);
+        lp0[k] = log10(dc->local_p[k][0]);
+        ldp[k] = (log10(dc->local_p[k][1]) - lp0[k]) / dc->local[k][1];
+        ldmu[k] = (double)(2.0f / (float)dc->local[k][2]);
+    }
+    for (int64_t i = 0; i < nesc; ++i) {
+        const orc_particle* q = &escaped[i];
+        const double x = q->x, y = q->y, z = q->z, p = q->p, mu = q->mu, weight = q->weight;
+        if (q->count_flag >= 0) continue;
+        if (p > prm->pmin && p <= prm->pmax && mu >= -1.0 && mu <= 1.0) {
+            const int64_t ip = floor_i64((log10(p) - pmin_log) / dp_log) + 1;
+            const int64_t imu = floor_i64((mu + 1.0) / dmu) + 1;
+            const int ibc = -q->count_flag;
+            if (ip >= 1 && ip <= npp && imu >= 1 && imu <= nmu && ibc <= 2 * prm->ndim)
+                fescaped[(imu - 1) + (int64_t)nmu * ((ip - 1) + (int64_t)npp * (ibc - 1))] += q->weight;
+        }
+        if (!dc->local_dist) continue;
+        for (int k = 0; k < 4; ++k) {
+            if (!dc->local[k][0]) continue;
+            const int npb = dc->local[k][1], nmk = dc->local[k][2];
+            const int64_t ix = floor_i64((x - prm->xmin) / dd[k][0]) + 1;
+            const int64_t iy = floor_i64((y - prm->ymin) / dd[k][1]) + 1;
+            const int64_t iz = floor_i64((z - prm->zmin) / dd[k][2]) + 1;
+            const int64_t ip = floor_i64((log10(p) - lp0[k]) / ldp[k]) + 1;
+            const int64_t imu = floor_i64((mu + 1.0) / ldmu[k]) + 1;
+            const int condx = ix >= 1 && ix <= nr[k][0];
+            const int condy = iy >= 1 && iy <= nr[k][1];
+            const int condz = iz >= 1 && iz <= nr[k][2];
+            const int condp = ip > 0 && ip < npb;
+            const int condmu = imu >= 1 && imu <= nmk;
+            if (!(condp && condmu)) continue;
+            const int64_t per = (int64_t)nmk * npb;   /* one (mu, p) plane */
+            const int64_t nx = nr[k][0], ny = nr[k][1], nz = nr[k][2];
+            const int64_t off_y = per * ny * nz * 2, off_z = off_y + per * nx * nz * 2;
+            const int64_t ipm = (imu - 1) + nmk * (ip - 1);
+            double* f = fesc[k];
+            switch (q->count_flag) {
+            case ORC_ESC_LX: if (condy && condz) f[ipm + per * ((iy - 1) + ny * (iz - 1))] += weight; break;
+            case ORC_ESC_HX: if (condy && condz) f[ipm + per * ((iy - 1) + ny * ((iz - 1) + nz))] += weight; break;
+            case ORC_ESC_LY: if (condx && condz) f[off_y + ipm + per * ((ix - 1) + nx * (iz - 1))] += weight; break;
+            case ORC_ESC_HY: if (condx && condz) f[off_y + ipm + per * ((ix - 1) + nx * ((iz - 1) + nz))] += weight; break;
+            case ORC_ESC_LZ: if (condx && condy) f[off_z + ipm + per * ((ix - 1) + nx * (iy - 1))] += weight; break;
+            case ORC_ESC_HZ: if (condx && condy) f[off_z + ipm + per * ((ix - 1) + nx * ((iy - 1) + ny))] += weight; break;
+            default: break;
+            }
+        }
+    }
+}
+
 void orc_quick_check(const orc_particle* ptls, int64_t nptl, int64_t nptl_split,
                      double leak, double leak_negp, double out[9])
 {

@@ -161,6 +161,10 @@ typedef struct {
     double local_p[4][2];
 } orc_dist_config;
 int64_t orc_local_dist_size(const orc_params* prm, const orc_dist_config* dc, int k);
+int64_t orc_escaped_dist_size(const orc_params* prm, const orc_dist_config* dc, int k);
+void orc_calc_escaped_distributions(const orc_params* prm, const orc_dist_config* dc,
+                                    const orc_particle* escaped, int64_t nesc,
+                                    double* fescaped, double* fesc[4]);
 void orc_calc_particle_distributions(const orc_params* prm, const orc_dist_config* dc,
                                      const orc_particle* ptls, int64_t nptl,
                                      double* fglobal, double* flocal[4]);

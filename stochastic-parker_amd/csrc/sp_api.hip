@@ -95,6 +95,11 @@ struct Ctx {
     long long loc_dims[4][5] = {};
     unsigned long long* dglob = nullptr;
     unsigned long long* dloc[4] = {nullptr, nullptr, nullptr, nullptr};
+    // distributions of escaped particles: global, then per set the x, y, z
+    // face arrays back to back
+    sp::EscArgs ea{};
+    long long esc_size[5] = {0, 0, 0, 0, 0};
+    unsigned long long* desc[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
@@ -359,6 +364,7 @@ void destroy(Ctx* c) {
     free_dev(c->soff);
     free_dev(c->dglob);
     for (auto* q : c->dloc) free_dev(q);
+    for (auto* q : c->desc) free_dev(q);
     free_dev(c->dvar);
     free_dev(c->vhist);
     free_dev(c->gmax);
@@ -669,7 +675,8 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     if (c->nptl > 0) {
         const long long nt = sp::num_tiles(c->nptl);
         const sp::SplitArgs none{};
-        SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 0, none, c->tile_cnt, c->tile_esc, c->st));
+        SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 0, none, c->tile_cnt, c->tile_esc,
+                                     c->ea.on ? &c->da : nullptr, &c->ea, c->st));
         SP_HIP(sp::launch_tile_scan(c->tile_cnt, c->tile_esc, nt, c->tile_off, c->totals, c->st));
         SP_HIP(sp::launch_compact_scatter(c->P[c->cur], c->P[1 - c->cur], c->nptl, c->tile_off, c->st));
     }
@@ -707,7 +714,8 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split) {
     if (c->nptl == 0) return SP_OK;
     sp::SplitArgs sa{split_ratio, pmin_split, c->cfg.p0, c->cfg.pmax};
     const long long nt = sp::num_tiles(c->nptl);
-    SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 1, sa, c->tile_cnt, nullptr, c->st));
+    SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 1, sa, c->tile_cnt, nullptr, nullptr,
+                                 nullptr, c->st));
     SP_HIP(sp::launch_tile_scan(c->tile_cnt, nullptr, nt, c->tile_off, c->totals, c->st));
     SP_HIP(sp::launch_split_apply(c->P[c->cur], c->nptl, c->nmax, sa, c->tile_off, c->st));
     unsigned long long tot[2] = {0, 0};
@@ -795,6 +803,10 @@ int sp_distributions_init(void* ctx, const sp_dist_config* dc) {
     d.dmu = (double)(2.0f / (float)dc->nmu_global);
     d.xmin = cf.xmin; d.ymin = cf.ymin; d.zmin = cf.zmin;
     for (auto*& q : c->dloc) { free_dev(q); q = nullptr; }
+    for (auto*& q : c->desc) { free_dev(q); q = nullptr; }
+    c->ea = sp::EscArgs{};
+    std::memset(c->esc_size, 0, sizeof(c->esc_size));
+    c->dist_ready = false;
     free_dev(c->dglob);
     c->dglob = nullptr;
     c->nglob = (long long)d.npp * d.nmu;
@@ -829,7 +841,61 @@ int sp_distributions_init(void* ctx, const sp_dist_config* dc) {
         c->loc_size[k] = (long long)L.nmu * L.npbins * A.nr[0] * A.nr[1] * A.nr[2];
         SP_HIP(hipMalloc(&c->dloc[k], sizeof(unsigned long long) * c->loc_size[k]));
     }
+    if (dc->escaped_dist) {
+        // init_particle_distributions (diagnostics.f90:188-191) and
+        // init_local_escaped_distributions (:359-395)
+        const int nd = cf.ndim;
+        c->esc_size[0] = c->nglob * 2 * nd;
+        for (int k = 0; k < 4; ++k) {
+            const sp::LocalArgs& A = d.loc[k];
+            if (!A.enabled) continue;
+            const long long per = (long long)A.nmu * A.npbins * 2;
+            c->esc_size[k + 1] = per * (A.nr[1] * A.nr[2] + A.nr[0] * A.nr[2] + (nd > 2 ? A.nr[0] * A.nr[1] : 0));
+        }
+        for (int j = 0; j < 5; ++j) {
+            if (c->esc_size[j] == 0) continue;
+            SP_HIP(hipMalloc(&c->desc[j], sizeof(unsigned long long) * c->esc_size[j]));
+            SP_HIP(hipMemsetAsync(c->desc[j], 0, sizeof(unsigned long long) * c->esc_size[j], c->st));
+        }
+        SP_HIP(hipStreamSynchronize(c->st));
+        sp::EscArgs& e = c->ea;
+        e.on = 1;
+        e.ndim = nd;
+        e.glob = c->desc[0];
+        for (int k = 0; k < 4; ++k) {
+            const sp::LocalArgs& A = d.loc[k];
+            unsigned long long* f = c->desc[k + 1];
+            if (!f) continue;
+            const long long per = (long long)A.nmu * A.npbins * 2;
+            e.face[k][0] = f;
+            e.face[k][1] = f + per * A.nr[1] * A.nr[2];
+            if (nd > 2) e.face[k][2] = e.face[k][1] + per * A.nr[0] * A.nr[2];
+        }
+    }
     c->dist_ready = true;
+    return SP_OK;
+}
+
+int sp_escaped_distributions(void* ctx, double* fescaped, double* fesc1, double* fesc2,
+                             double* fesc3, double* fesc4) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (!c->dist_ready || !c->ea.on)
+        return fail(SP_ERR_STATE, "escaped distributions are not initialised (escaped_dist = 0)");
+    double* outs[5] = {fescaped, fesc1, fesc2, fesc3, fesc4};
+    for (int j = 0; j < 5; ++j) {
+        if (!c->desc[j]) continue;
+        const size_t bytes = sizeof(unsigned long long) * (size_t)c->esc_size[j];
+        if (outs[j]) {
+            std::vector<unsigned long long> h((size_t)c->esc_size[j]);
+            SP_HIP(hipMemcpyAsync(h.data(), c->desc[j], bytes, hipMemcpyDeviceToHost, c->st));
+            SP_HIP(hipStreamSynchronize(c->st));
+            for (long long i = 0; i < c->esc_size[j]; ++i) outs[j][i] = (double)h[(size_t)i] * kInvFx;
+        }
+        SP_HIP(hipMemsetAsync(c->desc[j], 0, bytes, c->st));
+    }
+    SP_HIP(hipStreamSynchronize(c->st));
     return SP_OK;
 }
 

@@ -33,6 +33,16 @@ struct DistArgs {
     LocalArgs loc[4];
 };
 
+// Distributions of escaped particles (diagnostics.f90:913-1231), filled while
+// the removal pass counts them.  glob is (nmu, npp, 2 ndim); face[k][a] is
+// set k's array for the faces normal to axis a: (nmu, npbins, n_b, n_c, 2)
+// with (b, c) the other two axes in x, y, z order; nullptr where absent.
+struct EscArgs {
+    int on, ndim;
+    unsigned long long* glob;
+    unsigned long long* face[4][3];
+};
+
 struct SortArgs {
     int ndim, nx, ny, nz;       // interior cells per axis
     int shift;                  // cells per sort block = 2^shift along each axis
@@ -51,7 +61,8 @@ hipError_t launch_spatial_sort(const Ptl& S, const Ptl& D, long long n, const So
 hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, long long nptl,
                          long long nmax, int tag0, const InjectArgs& ia, hipStream_t s);
 hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,
-                             unsigned* tile_cnt, unsigned* tile_esc, hipStream_t s);
+                             unsigned* tile_cnt, unsigned* tile_esc, const DistArgs* da,
+                             const EscArgs* ea, hipStream_t s);
 hipError_t launch_tile_scan(const unsigned* tile_cnt, const unsigned* tile_esc, long long ntiles,
                             unsigned long long* tile_off, unsigned long long* totals, hipStream_t s);
 hipError_t launch_compact_scatter(const Ptl& S, const Ptl& D, long long n,

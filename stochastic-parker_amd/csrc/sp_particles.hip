@@ -144,9 +144,62 @@ __device__ __forceinline__ unsigned block_exclusive(unsigned v, unsigned* lds_wa
     return wbase + x - v;
 }
 
+__device__ __forceinline__ long long floor_ll(double v) {
+    const double f = floor(v);
+    if (!(f > -9.0e18 && f < 9.0e18)) return LLONG_MIN;
+    return (long long)f;
+}
+
+// calc_escaped_distributions (diagnostics.f90:913-1173) for one escaped
+// particle, face ibc = |count_flag| (1..6 = low x, high x, low y, ...).  The
+// global bins take p in (pmin, pmax] and mu in [-1, 1]; the local face maps
+// drop the top momentum bin and bin the two coordinates along the face.
+__device__ void escaped_hist(const Ptl& P, long long i, int ibc, const DistArgs& da,
+                             const EscArgs& ea) {
+    const double p = P.p[i], mu = P.mu[i];
+    const unsigned long long wfx = weight_fx(P.w[i]);
+    const double lp = log10(p);
+    if (p > da.pmin && p <= da.pmax && mu >= -1.0 && mu <= 1.0) {
+        const long long ip = floor_ll((lp - da.pmin_log) / da.dp_log) + 1;
+        const long long imu = floor_ll((mu + 1.0) / da.dmu) + 1;
+        if (ip >= 1 && ip <= da.npp && imu >= 1 && imu <= da.nmu && ibc <= 2 * ea.ndim)
+            atomicAdd(&ea.glob[(imu - 1) + (long long)da.nmu * ((ip - 1) + (long long)da.npp * (ibc - 1))],
+                      wfx);
+    }
+    if (!da.local_dist) return;
+    const int axis = (ibc - 1) >> 1, side = (ibc - 1) & 1;
+    if (axis >= ea.ndim) return;
+    const double x = P.x[i], y = P.y[i], z = P.z[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const LocalArgs& L = da.loc[k];
+        unsigned long long* f = ea.face[k][axis];
+        if (!L.enabled || !f) continue;
+        const long long ix = floor_ll((x - da.xmin) / L.dd[0]) + 1;
+        const long long iy = floor_ll((y - da.ymin) / L.dd[1]) + 1;
+        const long long iz = floor_ll((z - da.zmin) / L.dd[2]) + 1;
+        const long long ip = floor_ll((lp - L.pmin_log) / L.dp_log) + 1;
+        const long long imu = floor_ll((mu + 1.0) / L.dmu) + 1;
+        if (!(ip > 0 && ip < L.npbins && imu >= 1 && imu <= L.nmu)) continue;
+        const bool cx = ix >= 1 && ix <= L.nr[0];
+        const bool cy = iy >= 1 && iy <= L.nr[1];
+        const bool cz = iz >= 1 && iz <= L.nr[2];
+        long long ib, ic, nb, nc;
+        bool ok;
+        if (axis == 0) { ok = cy && cz; ib = iy; ic = iz; nb = L.nr[1]; nc = L.nr[2]; }
+        else if (axis == 1) { ok = cx && cz; ib = ix; ic = iz; nb = L.nr[0]; nc = L.nr[2]; }
+        else { ok = cx && cy; ib = ix; ic = iy; nb = L.nr[0]; nc = L.nr[1]; }
+        if (!ok) continue;
+        const long long idx = (imu - 1) + (long long)L.nmu * ((ip - 1) + (long long)L.npbins *
+                              ((ib - 1) + nb * ((ic - 1) + nc * side)));
+        atomicAdd(&f[idx], wfx);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) tile_count_kernel(Ptl P, long long n, int mode,
                                                             SplitArgs sa, unsigned* tile_cnt,
-                                                            unsigned* tile_esc) {
+                                                            unsigned* tile_esc, DistArgs da,
+                                                            EscArgs ea) {
     __shared__ unsigned lds[kBlock / kWave];
     const long long t0 = (long long)blockIdx.x * kTile;
     unsigned c = 0, e = 0;
@@ -155,7 +208,11 @@ __global__ void __launch_bounds__(kBlock) tile_count_kernel(Ptl P, long long n, 
         const long long i = t0 + (long long)it * kBlock + threadIdx.x;
         if (i < n) {
             c += predicate(P, i, mode, sa) ? 1u : 0u;
-            if (mode == 0) e += P.count_flag[i] < 0 ? 1u : 0u;
+            if (mode == 0) {
+                const int cf = P.count_flag[i];
+                e += cf < 0 ? 1u : 0u;
+                if (cf < 0 && ea.on) escaped_hist(P, i, -cf, da, ea);
+            }
         }
     }
     unsigned tc = 0, te = 0;
@@ -285,12 +342,6 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(Ptl P, long long n,
 // Distributions (diagnostics.f90:738-906).  Weights are accumulated as
 // unsigned 2^-32 fixed point, so every sum is exact and independent of the
 // order of the atomics.
-__device__ __forceinline__ long long floor_ll(double v) {
-    const double f = floor(v);
-    if (!(f > -9.0e18 && f < 9.0e18)) return LLONG_MIN;
-    return (long long)f;
-}
-
 __global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, DistArgs da,
                                                       unsigned long long* gglob,
                                                       unsigned long long* gloc0,
@@ -431,11 +482,15 @@ hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, lo
 long long num_tiles(long long n) { return (n + kTile - 1) / kTile; }
 
 hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,
-                             unsigned* tile_cnt, unsigned* tile_esc, hipStream_t s) {
+                             unsigned* tile_cnt, unsigned* tile_esc, const DistArgs* da,
+                             const EscArgs* ea, hipStream_t s) {
     const long long nt = num_tiles(n);
     if (nt == 0) return hipSuccess;
+    const DistArgs d = da ? *da : DistArgs{};
+    EscArgs e = ea ? *ea : EscArgs{};
+    if (!da || mode != 0) e.on = 0;
     hipLaunchKernelGGL(tile_count_kernel, dim3((unsigned)nt), dim3(kBlock), 0, s, P, n, mode, sa,
-                       tile_cnt, tile_esc);
+                       tile_cnt, tile_esc, d, e);
     return hipGetLastError();
 }
 

@@ -58,6 +58,8 @@ module gpat_hip
     real(dp), allocatable :: fglobal(:), pbins_edges_global(:)
     real(dp), allocatable, target :: flocal(:, :)
     integer(c_int64_t) :: loc_size(4) = 0
+    real(dp), allocatable, target :: fescaped(:), fescaped_local(:, :)
+    integer(c_int64_t) :: esc_size(4) = 0
 
 contains
 
@@ -364,7 +366,6 @@ contains
         real(dp) :: t0, dtf
         if (focused_transport) call unsupported("focused transport (-ft .true.)")
         if (nlgc) call unsupported("NLGC perpendicular diffusion (-nl .true.)")
-        if (dump_escaped_dist) call unsupported("escaped-particle distributions (-ded .true.)")
         t0 = tstamps_mhd(mhd_tframe)
         dtf = tstamps_mhd(mhd_tframe + 1) - t0
         call check_status(sp_particle_mover(ctx, t0, dtf, int(num_fine_steps, c_int32_t), &
@@ -393,7 +394,7 @@ contains
         integer :: i, k
         integer(c_int64_t) :: dims(5)
         real(dp) :: pmin_log, dp_log
-        if (dump_escaped_dist) call unsupported("escaped-particle distributions")
+        dcfg%escaped_dist = merge(1, 0, dump_escaped_dist)
         call check_status(sp_distributions_init(ctx, dcfg), "sp_distributions_init")
         allocate(fglobal(dcfg%npp_global), pbins_edges_global(dcfg%npp_global + 1))
         pmin_log = log10(pmin_)
@@ -406,8 +407,19 @@ contains
                 "sp_local_dist_dims")
             loc_size(k) = product(dims)
             if (dims(1) == 0) loc_size(k) = 0
+            ! init_local_escaped_distributions (diagnostics.f90:359-395): the
+            ! x, y[, z] face arrays of set k, (nmu, npbins, n_b, n_c, 2) each
+            esc_size(k) = 0
+            if (dump_escaped_dist .and. loc_size(k) > 0) then
+                esc_size(k) = dims(1) * dims(2) * 2 * (dims(4) * dims(5) + dims(3) * dims(5))
+                if (ndim_field > 2) esc_size(k) = esc_size(k) + dims(1) * dims(2) * 2 * dims(3) * dims(4)
+            endif
         enddo
         allocate(flocal(max(1_c_int64_t, maxval(loc_size)), 4))
+        if (dump_escaped_dist) then
+            allocate(fescaped(dcfg%nmu_global * dcfg%npp_global * 2 * ndim_field))
+            allocate(fescaped_local(max(1_c_int64_t, maxval(esc_size)), 4))
+        endif
     end subroutine init_particle_distributions
 
     !< distributions_diagnostics (diagnostics.f90:1652-1682): f(p) and the local
@@ -433,6 +445,28 @@ contains
         do k = 1, 4
             if (local_dist .and. loc_size(k) > 0) then
                 write(fh) int(k, c_int32_t), loc_size(k), flocal(1:loc_size(k), k)
+            endif
+        enddo
+        close(fh)
+        if (.not. dump_escaped_dist) return
+        ! calc_escaped_distributions + save_escaped_distributions
+        ! (diagnostics.f90:913-1231,1331-1372) and the local face arrays, written
+        ! as escaped_dists_NNNN.dat (stream: npp, pbins edges, number of faces,
+        ! fescaped, then for each enabled local set its size and values)
+        do k = 1, 4
+            lp(k) = c_null_ptr
+            if (esc_size(k) > 0) lp(k) = c_loc(fescaped_local(1, k))
+        enddo
+        call check_status(sp_escaped_distributions(ctx, c_loc_dp(fescaped), lp(1), lp(2), &
+            lp(3), lp(4)), "sp_escaped_distributions")
+        write(fname, "(A,A,I4.4,A)") trim(file_path), "escaped_dists_", iframe, ".dat"
+        open(newunit=fh, file=fname, access='stream', status='replace', form='unformatted', &
+             action='write')
+        write(fh) int(dcfg%npp_global, c_int32_t), pbins_edges_global, &
+            int(2 * ndim_field, c_int32_t), fescaped
+        do k = 1, 4
+            if (esc_size(k) > 0) then
+                write(fh) int(k, c_int32_t), esc_size(k), fescaped_local(1:esc_size(k), k)
             endif
         enddo
         close(fh)

@@ -15,7 +15,8 @@ module sp_hip_binding
         sp_particles_upload, sp_particles_download, sp_particles_counts, &
         sp_inject_particles_spatial_uniform, sp_particle_mover, sp_sort_particles, &
         sp_split_particle, sp_step_once, sp_distributions_init, sp_distributions, &
-        sp_local_dist_dims, sp_quick_check, sp_calibration_copy, sp_error_message, &
+        sp_local_dist_dims, sp_escaped_distributions, sp_quick_check, sp_calibration_copy, &
+        sp_error_message, &
         sp_set_variants, sp_inject_variants, sp_variant_distributions, sp_autotune_mover
     public SP_OK, SP_PREC_F64, SP_PREC_F32
 
@@ -71,7 +72,7 @@ module sp_hip_binding
     end type sp_tune_result
 
     type, bind(C) :: sp_dist_config
-        integer(c_int32_t) :: npp_global, nmu_global, local_dist, reserved0
+        integer(c_int32_t) :: npp_global, nmu_global, local_dist, escaped_dist
         type(sp_local_dist) :: local(4)
     end type sp_dist_config
 
@@ -208,6 +209,12 @@ module sp_hip_binding
             integer(c_int32_t), value :: set
             integer(c_int64_t), intent(out) :: dims(5)
         end function sp_local_dist_dims
+
+        integer(c_int) function sp_escaped_distributions(ctx, fescaped, fesc1, fesc2, fesc3, &
+                fesc4) bind(C, name="sp_escaped_distributions")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, fescaped, fesc1, fesc2, fesc3, fesc4
+        end function sp_escaped_distributions
 
         integer(c_int) function sp_quick_check(ctx, out) bind(C, name="sp_quick_check")
             import :: c_int, c_double, c_ptr

@@ -78,7 +78,7 @@ class SpLocalDist(C.Structure):
 
 class SpDistConfig(C.Structure):
     _fields_ = [("npp_global", C.c_int32), ("nmu_global", C.c_int32),
-                ("local_dist", C.c_int32), ("reserved0", C.c_int32),
+                ("local_dist", C.c_int32), ("escaped_dist", C.c_int32),
                 ("local", SpLocalDist * 4)]
 
 
@@ -136,6 +136,7 @@ EXPORTS = {
     "sp_distributions_init": [_P, C.POINTER(SpDistConfig)],
     "sp_distributions": [_P, _D, _D, _D, _D, _D],
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],
+    "sp_escaped_distributions": [_P, _D, _D, _D, _D, _D],
     "sp_quick_check": [_P, _D],
     "sp_calibration_copy": [_P, C.c_int64, _D],
     "sp_autotune_mover": [_P, C.c_double, C.c_double, C.c_int32, C.c_int32,

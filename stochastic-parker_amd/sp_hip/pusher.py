@@ -189,11 +189,12 @@ class ParkerPusher:
 
     # -- diagnostics ------------------------------------------------------
     def init_particle_distributions(self, npp_global=256, nmu_global=1, local_dist=0,
-                                    local=()):
+                                    local=(), escaped_dist=0):
         dc = SpDistConfig()
         dc.npp_global = npp_global
         dc.nmu_global = nmu_global
         dc.local_dist = int(local_dist)
+        dc.escaped_dist = int(escaped_dist)
         for k, L in enumerate(local):
             if L is None:
                 continue
@@ -229,6 +230,45 @@ class ParkerPusher:
         for a, d in zip(outs, self._dist_dims):
             shaped.append(None if a is None else a.reshape(d[4], d[3], d[2], d[1], d[0]))
         return fg, shaped
+
+    def calc_escaped_distributions(self):
+        """Distributions of the particles removed with an escape flag since the
+        previous call (the bins are cleared).  Returns (fescaped, sets):
+        fescaped has C shape (2 ndim, npp_global, nmu_global), faces in the
+        order low x, high x, low y, high y[, low z, high z]; sets[k] is None or
+        a dict of face arrays 'x', 'y'[, 'z'], each of C shape
+        (2, n_c, n_b, npbins, nmu) with (b, c) the other two axes."""
+        nd = self.cfg.ndim
+        fe = np.zeros(2 * nd * self._npp, dtype=np.float64)
+        outs, ptrs = [], []
+        for d in self._dist_dims:
+            if d[0]:
+                nmu, npb, nrx, nry, nrz = d
+                per = nmu * npb * 2
+                size = per * (nry * nrz + nrx * nrz + (nrx * nry if nd > 2 else 0))
+                a = np.zeros(size, dtype=np.float64)
+                outs.append(a)
+                ptrs.append(dptr(a))
+            else:
+                outs.append(None)
+                ptrs.append(None)
+        self._call("sp_escaped_distributions", dptr(fe), *ptrs)
+        sets = []
+        for a, d in zip(outs, self._dist_dims):
+            if a is None:
+                sets.append(None)
+                continue
+            nmu, npb, nrx, nry, nrz = d
+            shapes = {"x": (2, nrz, nry, npb, nmu), "y": (2, nrz, nrx, npb, nmu)}
+            if nd > 2:
+                shapes["z"] = (2, nry, nrx, npb, nmu)
+            faces, off = {}, 0
+            for axis, shp in shapes.items():
+                m = int(np.prod(shp))
+                faces[axis] = a[off:off + m].reshape(shp)
+                off += m
+            sets.append(faces)
+        return fe.reshape(2 * nd, self._npp_p, -1), sets
 
     # -- batched parameter variants (no reference counterpart) -----------
     def set_variants(self, variants):

@@ -108,3 +108,46 @@ def random_states(c, n, seed=3, slack=True):
     a["tag_injected"] = np.arange(n)
     a["tag_splitted"] = 1
     return a
+
+
+def escaped_particles(c, n, seed):
+    """Particles just outside a random face each (count_flag = -face), their
+    other coordinates spread 5 % beyond the domain so that the local face maps'
+    range checks matter; p and mu partly outside the binned ranges."""
+    rng = np.random.default_rng(seed)
+    nd = c["ndim"]
+    a = np.zeros(n, dtype=oracle.PARTICLE_DTYPE)
+    faces = rng.integers(1, 2 * nd + 1, n)
+    lo = np.array([c["xmin"], c["ymin"], c["zmin"]])
+    hi = np.array([c["xmax"], c["ymax"], c["zmax"]])
+    pos = rng.uniform(lo - 0.05, hi + 0.05, size=(n, 3))
+    if nd == 2:
+        pos[:, 2] = rng.uniform(c["zmin"], c["zmax"], n)
+    axis, side = (faces - 1) // 2, (faces - 1) % 2
+    depth = rng.uniform(1e-6, 0.01, n)
+    pos[np.arange(n), axis] = np.where(side == 0, lo[axis] - depth, hi[axis] + depth)
+    a["x"], a["y"], a["z"] = pos.T
+    a["p"] = np.exp(rng.uniform(np.log(5e-3), np.log(2e4), n))
+    a["mu"] = rng.uniform(-1.05, 1.05, n)
+    a["weight"] = 0.5 ** rng.integers(0, 4, n)
+    a["count_flag"] = -faces
+    a["tag_injected"] = np.arange(n)
+    a["tag_splitted"] = 1
+    return a
+
+
+ESC_DIST = dict(npp_global=64, nmu_global=2, local_dist=1,
+                local=[dict(enabled=1, npbins=16, nmu=1, rx=4, ry=4, rz=4, pmin=1e-2, pmax=1e4),
+                       None, None,
+                       dict(enabled=1, npbins=4, nmu=2, rx=8, ry=2, rz=3, pmin=0.1, pmax=100.0)])
+
+
+def face_layout(c, dcfg, k):
+    """(axis name, C shape) of local set k's face arrays, back to back."""
+    L = dcfg["local"][k]
+    nr = [-(-c["nx"] // L["rx"]), -(-c["ny"] // L["ry"]), -(-c["nz"] // L["rz"])]
+    tail = (L["npbins"], L["nmu"])
+    out = [("x", (2, nr[2], nr[1]) + tail), ("y", (2, nr[2], nr[0]) + tail)]
+    if c["ndim"] == 3:
+        out.append(("z", (2, nr[1], nr[0]) + tail))
+    return out, nr

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _abi.load().sp_abi_version() == 3
+    assert _abi.load().sp_abi_version() == 4
 
 
 def test_struct_sizes_match_header_layout():

@@ -102,7 +102,7 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
     args = [DRIVER, "-dm", "mhd/", "-np", str(npart), "-nm", "20000", "-ts", "0", "-te", "2",
             "-st", "1", "-df", "1", "-sf", "1", "-sr", "1.3", "-ps", "1.05", "-nl", ".false.",
             "-ug", "1", "-nd", "2", "-ld", ".true.", "-dd", "data/", "-dt", "1e-7",
-            "-gp", precision]
+            "-gp", precision, "-ded", ".true."]
     print("driver:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -119,7 +119,7 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
     with ParkerPusher(**cfg) as pz:
         pz.upload_fields(nodes)
         pz.calc_fields_gradients()
-        pz.init_particle_distributions(**dist)
+        pz.init_particle_distributions(**dist, escaped_dist=1)
         pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, 0.0, dtf, box)
         fg0, loc0 = pz.calc_particle_distributions()
         rows.append((0, pz.quick_check()))
@@ -128,6 +128,7 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
             pz.split_particle(1.3, 1.05)
             rows.append((tf, pz.quick_check()))
             fg, loc = pz.calc_particle_distributions()
+            fe, esc = pz.calc_escaped_distributions()
     keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
             "pdt_avg")
     for line, (tf, q) in zip(quick[1:], rows):
@@ -149,3 +150,17 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
     assert kset == 1 and size == loc[0].size
     floc = np.frombuffer(raw, dtype="<f8", count=size, offset=off + 12)
     np.testing.assert_array_equal(floc, loc[0].ravel())
+    # escaped_dists_0002.dat: npp, edges, number of faces, fescaped, then
+    # (set, size, values) of set 1's face arrays
+    raw = (tmp_path / "data" / "escaped_dists_0002.dat").read_bytes()
+    assert struct.unpack_from("<i", raw, 0)[0] == 64
+    off = 4 + 8 * 65
+    assert struct.unpack_from("<i", raw, off)[0] == 4
+    fesc = np.frombuffer(raw, dtype="<f8", count=fe.size, offset=off + 4)
+    np.testing.assert_array_equal(fesc, fe.ravel())
+    assert fe.sum() > 0
+    off += 4 + 8 * fe.size
+    kset, size = struct.unpack_from("<iq", raw, off)
+    flat = np.concatenate([esc[0]["x"].ravel(), esc[0]["y"].ravel()])
+    assert kset == 1 and size == flat.size
+    np.testing.assert_array_equal(np.frombuffer(raw, dtype="<f8", count=size, offset=off + 12), flat)

@@ -6,7 +6,7 @@ import pytest
 from scipy import stats
 
 import oracle
-from helpers import cfg2d, random_states
+from helpers import ESC_DIST, cfg2d, escaped_particles, face_layout, random_states
 
 BOX = (0.1, 0.2, 0.0, 0.6, 0.9, 1.0)
 
@@ -162,3 +162,73 @@ def test_quick_check_matches_numpy():
     assert q["pdt_min"] == min(1.0, a["dt"].min()) and q["pdt_max"] == a["dt"].max()
     assert q["pdt_avg"] == pytest.approx(a["dt"].mean(), rel=1e-13)
     assert q["pmax"] == a["p"].max()
+
+
+def split_faces(c, dcfg, k, flat):
+    layout, _ = face_layout(c, dcfg, k)
+    faces, off = {}, 0
+    for name, shp in layout:
+        m = int(np.prod(shp))
+        faces[name] = flat[off:off + m].reshape(shp)
+        off += m
+    assert off == len(flat)
+    return faces
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_escaped_distributions_match_numpy_histograms(ndim):
+    from helpers import cfg3d
+    c = cfg2d(32) if ndim == 2 else cfg3d(12)
+    prm = oracle.params(c)
+    a = escaped_particles(c, 30000, seed=20 + ndim)
+    others = random_states(c, 500, seed=5)
+    others["count_flag"] = 0    # removed but not escaped: never binned
+    fe, locs = oracle.escaped_distributions(prm, ESC_DIST, np.concatenate([a, others]))
+    fe = fe.reshape(2 * ndim, 64, 2)
+    pe = 10.0 ** np.linspace(-2, 4, 65)
+    for f in range(1, 2 * ndim + 1):
+        s = a[(a["count_flag"] == -f) & (a["p"] > 1e-2) & (a["p"] <= 1e4)]
+        H, _, _ = np.histogram2d(s["p"], s["mu"], bins=[pe, [-1.0, 0.0, 1.0]], weights=s["weight"])
+        np.testing.assert_allclose(fe[f - 1], H, rtol=0, atol=1e-12, err_msg=f"face {f}")
+    lo = np.array([c["xmin"], c["ymin"], c["zmin"]])
+    ln = np.array([c["lx"], c["ly"], c["lz"]])
+    names = "xyz"
+    for k in (0, 3):
+        L = ESC_DIST["local"][k]
+        faces = split_faces(c, ESC_DIST, k, locs[k])
+        _, nr = face_layout(c, ESC_DIST, k)
+        pe_k = 10.0 ** np.linspace(np.log10(L["pmin"]), np.log10(L["pmax"]), L["npbins"] + 1)
+        mue = np.linspace(-1.0, 1.0, L["nmu"] + 1)
+        for axis in range(ndim):
+            b_ax, c_ax = [q for q in range(3) if q != axis]
+            for side in (0, 1):
+                s = a[a["count_flag"] == -(2 * axis + side + 1)]
+                coords = np.stack([s["xyz"[c_ax]], s["xyz"[b_ax]], s["p"], s["mu"]], 1)
+                H, _ = np.histogramdd(coords, bins=[
+                    np.linspace(lo[c_ax], lo[c_ax] + ln[c_ax], nr[c_ax] + 1),
+                    np.linspace(lo[b_ax], lo[b_ax] + ln[b_ax], nr[b_ax] + 1), pe_k, mue],
+                    weights=s["weight"])
+                H[:, :, -1, :] = 0.0     # the top momentum bin is dropped
+                np.testing.assert_allclose(faces[names[axis]][side], H, rtol=0, atol=1e-12,
+                                           err_msg=f"set {k + 1} axis {axis} side {side}")
+
+
+def test_escaped_weight_equals_leak_through_open_faces():
+    """Every particle the mover flags as escaped lands in the escaped
+    distributions, and their weight is the leak the boundary pass counted."""
+    from helpers import nodes_for
+    c = cfg2d(32, pbcx=1, pbcy=1, dt_max_rel=1e-2)
+    prm = oracle.params(c)
+    fa = oracle.gradients(prm, nodes_for(c))
+    box = (0.0, 0.0, 0.0, 1.0, 1.0, 0.0)
+    a, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), 3000, 3000, 1e-7, 1, 5.0, 0.0,
+                         0.2, box, 7.0)
+    kept, acc, removed = oracle.mover(prm, fa, a, 0.0, 0.2, nthreads=4, return_removed=True)
+    esc = removed[removed["count_flag"] < 0]
+    assert len(esc) == acc.nptl_escaped > 20
+    assert len(kept) + len(removed) == len(a)
+    fe, _ = oracle.escaped_distributions(prm, dict(npp_global=256, nmu_global=1), removed)
+    assert fe.sum() == acc.leak
+    per_face = fe.reshape(4, -1).sum(1)
+    for f in range(1, 5):
+        assert per_face[f - 1] == esc["weight"][esc["count_flag"] == -f].sum()
