@@ -91,6 +91,13 @@ typedef struct sp_config {
                                    do not depend on it */
     int32_t reserved0;          /* must be 0 */
     int64_t nptl_max;           /* -nm */
+    /* momentum diffusion in the push (set_dpp_params, particle_module.f90:260-272;
+     * stochastic-mhd.f90 flags -dw, -ds, -ws, -t0) */
+    int32_t dpp_wave;           /* 1: wave scattering (calc_dpp_wave_scattering, :2918-2933) */
+    int32_t dpp_shear;          /* 1: flow shear (calc_dpp_flow_shear, :2947-2979) */
+    int32_t weak_scattering;    /* 1: the weak-scattering form of the flow-shear rate */
+    int32_t reserved1;          /* must be 0 */
+    double tau0;                /* scattering time of particles with p0 (flow shear) */
 } sp_config;
 
 /* Structure-of-arrays view of particle_type (particle_module.f90:38-50) plus

@@ -43,6 +43,8 @@ class OrcParams(C.Structure):
         ("acc_region_flag", C.c_int32), ("acc_region", C.c_double * 6),
         ("time_interp", C.c_int32),
         ("seed", C.c_uint64), ("stream", C.c_uint32),
+        ("dpp_wave", C.c_int32), ("dpp_shear", C.c_int32), ("weak_scattering", C.c_int32),
+        ("tau0", C.c_double),
     ]
 
 
@@ -133,7 +135,8 @@ def params(cfg: dict, seed: int = 20261202, stream: int = 0) -> OrcParams:
     """OrcParams from an sp_config-style dict (same key names)."""
     p = OrcParams()
     for name, _ in OrcParams._fields_:
-        if name in ("pindex", "pcharge", "acc_region", "seed", "stream"):
+        if name in ("pindex", "pcharge", "acc_region", "seed", "stream", "dpp_wave", "dpp_shear",
+                    "weak_scattering", "tau0"):
             continue
         setattr(p, name, cfg[name])
     p.pindex = 3.0 - cfg["gamma_turb"]
@@ -141,6 +144,10 @@ def params(cfg: dict, seed: int = 20261202, stream: int = 0) -> OrcParams:
     ar = cfg.get("acc_region", (0.0, 1.0, 0.0, 1.0, 0.0, 1.0))
     for i in range(6):
         p.acc_region[i] = ar[i]
+    p.dpp_wave = cfg.get("dpp_wave", 0)
+    p.dpp_shear = cfg.get("dpp_shear", 0)
+    p.weak_scattering = cfg.get("weak_scattering", 0)
+    p.tau0 = cfg.get("tau0", 0.0)
     p.seed = cfg.get("seed", seed)
     p.stream = cfg.get("stream", stream)
     return p

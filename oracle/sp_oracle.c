@@ -398,14 +398,50 @@ static double drift_vdp(const orc_params* prm, double p)
 
 static double fmin_seq(double a, double b) { return b < a ? b : a; }
 
+/* calc_dpp_wave_scattering, particle_module.f90:2918-2933: va = b/sqrt(rho) */
+static void dpp_wave_scattering(const orc_params* prm, double rho, double b, double kpara,
+                                double p, double* dp_dt, double* dpp)
+{
+    const double va = b / sqrt(rho);
+    if (prm->momentum_dependency == 1)
+        *dp_dt = *dp_dt + (8 * p / (27 * kpara)) * (va * va);
+    else
+        *dp_dt = *dp_dt + (4 * p / (9 * kpara)) * (va * va);
+    *dpp = *dpp + ((p * va) * (p * va)) / (9 * kpara);
+}
+
+/* calc_dpp_flow_shear, particle_module.f90:2947-2979 */
+static void dpp_flow_shear(const orc_params* prm, double b, double bx, double by, double bz,
+                           double knorm_para, double sxx, double syy, double szz, double sxy,
+                           double sxz, double syz, double p, double* dp_dt, double* dpp)
+{
+    double gshear;
+    if (prm->weak_scattering) {
+        double ib;
+        if (b < DBL_EPSILON) ib = 0.0; else ib = 1.0 / b;
+        double bbsigma = sxx * (bx * bx) + syy * (by * by) + szz * (bz * bz) +
+                         2.0 * (sxy * bx * by + sxz * bx * bz + syz * by * bz);
+        bbsigma = bbsigma * ib * ib;
+        gshear = (bbsigma * bbsigma) / 5;
+    } else {
+        gshear = 2 * (sxx * sxx + syy * syy + szz * szz +
+                      2 * (sxy * sxy + sxz * sxz + syz * syz)) / 15;
+    }
+    if (gshear > 0.0) {
+        const double pindex = prm->pindex;
+        *dp_dt = *dp_dt + (2 + pindex) * gshear * prm->tau0 * knorm_para *
+                 pow(p, pindex - 1) * pow(prm->p0, 2.0 - pindex);
+        *dpp = *dpp + gshear * prm->tau0 * knorm_para * pow(p, pindex) * pow(prm->p0, 2.0 - pindex);
+    }
+}
+
 /* momentum update shared by both pushers (particle_module.f90:3589-3605,
- * :4885-4906): acceleration-region gate and the 0.25*p0 floor.  The 2*dpp
- * momentum-diffusion term is zero in this path (dpp flags off). */
-static void update_momentum(const orc_params* prm, orc_particle* ptl, double dp_dt,
+ * :4885-4906): the increment with the momentum-diffusion term, the
+ * acceleration-region gate and the 0.25*p0 floor. */
+static void update_momentum(const orc_params* prm, orc_particle* ptl, double dp_dt, double dpp,
                             double sdt, double ran_p, double* deltap)
 {
-    const double dpp = 0.0;
-    double dp = dp_dt * ptl->dt + ran_p * sqrt(2.0 * dpp) * sdt;
+    double dp = dp_dt * ptl->dt + ran_p * sqrt(2 * dpp) * sdt;
     if (prm->acc_region_flag == 1) {
         if (in_acc_region(prm, ptl)) ptl->p = ptl->p + dp;
         else dp = 0.0;
@@ -450,7 +486,19 @@ void orc_push_particle_2d(const orc_params* prm, const double dtmm[2], orc_parti
     const double dy_dt = vy + vdy + k->dkxy_dx + k->dkyy_dy;
     const double dz_dt = vdz;
     const double divv = dvx_dx + dvy_dy;
-    const double dp_dt = -(ptl->p * divv / 3.0);
+    double dp_dt = -(ptl->p * divv / 3.0);
+    double dpp = 0.0;
+    /* momentum diffusion, :3479-3500 */
+    if (prm->dpp_wave) dpp_wave_scattering(prm, f[3], b, k->kpara, ptl->p, &dp_dt, &dpp);
+    if (prm->dpp_shear) {
+        const double dvx_dy = f[9], dvy_dx = f[11];
+        const double sxx = dvx_dx - divv / 3;
+        const double syy = dvy_dy - divv / 3;
+        const double szz = -(divv / 3);
+        const double sxy = (dvx_dy + dvy_dx) / 2;
+        dpp_flow_shear(prm, b, bx, by, bz, k->knorm_para, sxx, syy, szz, sxy, 0.0, 0.0, ptl->p,
+                       &dp_dt, &dpp);
+    }
 
     if (!fixed_dt) {
         if (dx_dt != 0.0 && dy_dt != 0.0 && dp_dt != 0.0) {
@@ -482,7 +530,7 @@ void orc_push_particle_2d(const orc_params* prm, const double dtmm[2], orc_parti
     ptl->t = ptl->t + ptl->dt;
     delta[0] = deltax;
     delta[1] = deltay;
-    update_momentum(prm, ptl, dp_dt, sdt, rands[3], &delta[3]);
+    update_momentum(prm, ptl, dp_dt, dpp, sdt, rands[3], &delta[3]);
 }
 
 /* push_particle_3d, particle_module.f90:4625-4907 (Cartesian, uniform grid). */
@@ -518,7 +566,22 @@ void orc_push_particle_3d(const orc_params* prm, const double dtmm[2], orc_parti
     const double dy_dt = vy + vdy + k->dkxy_dx + k->dkyy_dy + k->dkyz_dz;
     const double dz_dt = vz + vdz + k->dkxz_dx + k->dkyz_dy + k->dkzz_dz;
     const double divv = dvx_dx + dvy_dy + dvz_dz;
-    const double dp_dt = -(ptl->p * divv / 3.0);
+    double dp_dt = -(ptl->p * divv / 3.0);
+    double dpp = 0.0;
+    /* momentum diffusion, :4764-4787 */
+    if (prm->dpp_wave) dpp_wave_scattering(prm, f[3], b, k->kpara, ptl->p, &dp_dt, &dpp);
+    if (prm->dpp_shear) {
+        const double dvx_dy = f[9], dvx_dz = f[10], dvy_dx = f[11];
+        const double dvy_dz = f[13], dvz_dx = f[14], dvz_dy = f[15];
+        const double sxx = dvx_dx - divv / 3;
+        const double syy = dvy_dy - divv / 3;
+        const double szz = dvz_dz - divv / 3;
+        const double sxy = (dvx_dy + dvy_dx) / 2;
+        const double sxz = (dvx_dz + dvz_dx) / 2;
+        const double syz = (dvy_dz + dvz_dy) / 2;
+        dpp_flow_shear(prm, b, bx, by, bz, k->knorm_para, sxx, syy, szz, sxy, sxz, syz, ptl->p,
+                       &dp_dt, &dpp);
+    }
 
     if (!fixed_dt) {
         /* the zero test omits dz_dt (particle_module.f90:4792-4794) */
@@ -555,7 +618,7 @@ void orc_push_particle_3d(const orc_params* prm, const double dtmm[2], orc_parti
     delta[0] = deltax;
     delta[1] = deltay;
     delta[2] = deltaz;
-    update_momentum(prm, ptl, dp_dt, sdt, rands[3], &delta[3]);
+    update_momentum(prm, ptl, dp_dt, dpp, sdt, rands[3], &delta[3]);
 }
 
 /* set_dt_min_max, particle_module.f90:5519-5524 */

@@ -71,6 +71,9 @@ typedef struct {
     int32_t time_interp;
     uint64_t seed;                /* Philox key */
     uint32_t stream;              /* Philox counter word 3 (variant id) */
+    /* momentum diffusion (set_dpp_params, particle_module.f90:260-272) */
+    int32_t dpp_wave, dpp_shear, weak_scattering;
+    double tau0;
 } orc_params;
 
 /* kappa_type, particle_module.f90:93-104 */

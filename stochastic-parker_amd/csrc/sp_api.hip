@@ -155,7 +155,18 @@ int validate(const sp_config* c) {
     if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
     if (c->sort_steps < 0 || c->reserved0 != 0) return fail(SP_ERR_ARG, "need sort_steps >= 0 and reserved0 == 0");
     if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
+    if (c->dpp_wave < 0 || c->dpp_wave > 1 || c->dpp_shear < 0 || c->dpp_shear > 1 ||
+        c->weak_scattering < 0 || c->weak_scattering > 1 || c->reserved1 != 0)
+        return fail(SP_ERR_ARG, "dpp_wave, dpp_shear, weak_scattering must be 0 or 1 and reserved1 0");
+    if (c->dpp_shear && !(c->tau0 >= 0)) return fail(SP_ERR_ARG, "tau0 must be >= 0");
+    if ((c->dpp_wave || c->dpp_shear) && c->precision == SP_PREC_F32H)
+        return fail(SP_ERR_UNSUPPORTED, "momentum diffusion needs SP_PREC_F64 or SP_PREC_F32 records");
     return SP_OK;
+}
+
+// Step features of a configuration (template argument of the mover).
+int step_features(const sp_config& c) {
+    return (c.acc_region_flag == 1 ? sp::FX_ACC : 0) | (c.dpp_wave || c.dpp_shear ? sp::FX_DPP : 0);
 }
 
 sp::DevParams make_params(const sp_config& c) {
@@ -182,6 +193,7 @@ sp::DevParams make_params(const sp_config& c) {
     d.acc_region_flag = c.acc_region_flag;
     d.time_interp = c.time_interp;
     d.precision = c.precision;
+    d.ext_rec = (c.dpp_wave || c.dpp_shear) ? 1 : 0;
     for (int i = 0; i < 6; ++i) d.acc_region[i] = c.acc_region[i];
     d.key0 = (uint32_t)c.seed;
     d.key1 = (uint32_t)(c.seed >> 32);
@@ -265,12 +277,13 @@ int spatial_sort(Ctx* c, bool paused_first = false, long long n = -1) {
 // Diffusion parameters of one variant in the forms both step paths use, by
 // the same expressions as the configuration's (pindex = 3 - gamma_turb,
 // particle_module.f90:2798).
-sp::KappaVar kappa_var(double kpara0, double kret, double gamma_turb) {
+sp::KappaVar kappa_var(double p0, double kpara0, double kret, double gamma_turb) {
     sp::KappaVar k{};
     k.kpara0 = kpara0;
     k.kret = kret;
     k.pindex = 3.0 - gamma_turb;
     k.gamma_m2 = gamma_turb - 2.0;
+    k.p0_2mpi = std::pow(p0, 2.0 - k.pindex);   // p0**(2.0-pindex) (:2975-2977)
     k.f_kpara0 = (float)kpara0;
     k.f_kret = (float)kret;
     k.f_pindex = (float)k.pindex;
@@ -295,7 +308,9 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
               (f.ndim == 3 && f.pbcz == 1 ? sp::MF_OPEN_Z : 0) |
               (f.momentum_dependency == 1 ? sp::MF_MOMENTUM_DEP : 0) |
               (f.mag_dependency == 1 ? sp::MF_MAG_DEP : 0) |
-              (f.ndim == 2 && f.check_drift_2d ? sp::MF_DRIFT_2D : 0);
+              (f.ndim == 2 && f.check_drift_2d ? sp::MF_DRIFT_2D : 0) |
+              (f.dpp_wave ? sp::MF_DPP_WAVE : 0) | (f.dpp_shear ? sp::MF_DPP_SHEAR : 0) |
+              (f.weak_scattering ? sp::MF_WEAK_SCATTERING : 0);
     m.key0 = d.key0; m.key1 = d.key1; m.stream = d.stream;
     m.t0 = t0;
     m.dtf = dtf;
@@ -311,7 +326,7 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.ehi[0] = f.xmax; m.ehi[1] = f.ymax; m.ehi[2] = f.zmax;
     m.p0 = f.p0;
     m.p_floor = 0.25 * f.p0;
-    const sp::KappaVar kv = kappa_var(f.kpara0, f.kret, f.gamma_turb);
+    const sp::KappaVar kv = kappa_var(f.p0, f.kpara0, f.kret, f.gamma_turb);
     m.kpara0 = kv.kpara0;
     m.kret = kv.kret;
     m.pindex = kv.pindex;
@@ -339,6 +354,10 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.inv_p0 = 1.0 / f.p0;
     for (int i = 0; i < 6; ++i) m.acc_region[i] = f.acc_region[i];
     m.lx = f.lx; m.ly = f.ly; m.lz = f.lz;
+    m.tau0 = f.tau0;
+    m.p0_2mpi = kv.p0_2mpi;
+    m.f_tau0 = (float)f.tau0;
+    m.f_p02 = (float)(f.p0 * f.p0);
     return m;
 }
 
@@ -396,7 +415,8 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     c->dp = make_params(*cfg);
     c->nmax = cfg->nptl_max;
     c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
-    c->recw = cfg->ndim == 2 ? sp::kRec2D : (cfg->precision == SP_PREC_F32H ? sp::kRec3DH : sp::kRec3D);
+    if (c->dp.ext_rec) c->recw = cfg->ndim == 2 ? sp::kRec2DX : sp::kRec3DX;
+    else c->recw = cfg->ndim == 2 ? sp::kRec2D : (cfg->precision == SP_PREC_F32H ? sp::kRec3DH : sp::kRec3D);
     auto bail = [&](int code) {
         destroy(c);
         return code;
@@ -427,11 +447,12 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     SP_HIP_C(hipMalloc(&c->stats_part, sizeof(double) * 5 * c->stats_grid));
     // occupancy of the persistent mover decides its grid
     {
-        const bool ti = cfg->time_interp != 0, acc = cfg->acc_region_flag == 1;
-        c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, acc));
-        c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, acc));
+        const bool ti = cfg->time_interp != 0;
+        const int fx = step_features(*cfg);
+        c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, fx));
+        c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, fx));
         c->mover_blocks_per_cu[2] =
-            cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, acc)) : 1;
+            cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, fx & sp::FX_ACC)) : 1;
     }
     SP_HIP_C(hipStreamSynchronize(c->st));
 #undef SP_HIP_C
@@ -602,7 +623,8 @@ int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int3
 int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_interval, int bpc,
               int sort_steps, unsigned long long& steps, float& ms_push) {
     int rc = SP_OK;
-    const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
+    const bool ti = c->cfg.time_interp != 0;
+    const int fx = step_features(c->cfg);
     // With sort_steps, the interval runs as launches of at most sort_steps
     // steps per particle; between launches the paused particles are sorted
     // to the front in spatial order and the next launch takes only them.
@@ -629,9 +651,9 @@ int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_inte
         SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
         SP_HIP(hipEventRecord(c->ev[1], c->st));
         const int grid = mover_grid(c, nrun, bpc);
-        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, acc, grid, c->st));
-        else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, acc, c->cfg.ndim, grid, c->st));
-        else SP_HIP(sp::launch_mover_f64(m, ti, acc, c->cfg.ndim, grid, c->st));
+        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, fx, grid, c->st));
+        else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, fx, c->cfg.ndim, grid, c->st));
+        else SP_HIP(sp::launch_mover_f64(m, ti, fx, c->cfg.ndim, grid, c->st));
         SP_HIP(hipEventRecord(c->ev[3], c->st));
         unsigned long long sp_[2] = {0, 0};
         SP_HIP(hipMemcpyAsync(sp_, &c->ctr->steps, sizeof(sp_), hipMemcpyDeviceToHost, c->st));
@@ -761,14 +783,15 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     if (e == hipSuccess) e = hipMemcpyAsync(df, fixed_dt, (size_t)n, hipMemcpyHostToDevice, c->st);
     sp::MoverParams m = mover_params(c, t0, dtf, 1, 1);
     m.n = n;
-    const bool ti = c->cfg.time_interp != 0, acc = c->cfg.acc_region_flag == 1;
+    const bool ti = c->cfg.time_interp != 0;
+    const int fx = step_features(c->cfg);
     if (e == hipSuccess) {
         if (c->cfg.precision == SP_PREC_F32H)
-            e = sp::launch_step_once_f32h(m, ti, acc, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f32h(m, ti, fx, ds, dr, df, dd, c->st);
         else if (c->cfg.precision == SP_PREC_F32)
-            e = sp::launch_step_once_f32(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f32(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st);
         else
-            e = sp::launch_step_once_f64(m, ti, acc, c->cfg.ndim, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f64(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(state, ds, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipMemcpyAsync(deltas, dd, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->st);
@@ -1030,7 +1053,7 @@ int sp_set_variants(void* ctx, int32_t nvar, const sp_variant* variants) {
         if (!(q.kret >= 0) || q.kret > 1) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": kret must be in [0, 1]");
         if (!std::isfinite(q.gamma_turb)) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": gamma_turb must be finite");
         if (q.reserved0 != 0) return fail(SP_ERR_ARG, "variant " + std::to_string(v) + ": reserved0 must be 0");
-        hv[(size_t)v] = kappa_var(q.kpara0, q.kret, q.gamma_turb);
+        hv[(size_t)v] = kappa_var(c->cfg.p0, q.kpara0, q.kret, q.gamma_turb);
     }
     if (nvar > c->nvar_cap) {
         free_dev(c->dvar);

@@ -49,6 +49,18 @@ __device__ __forceinline__ void axis_grad(const float* __restrict__ n8, long lon
     }
 }
 
+// One node record: the first NC slots of the record layout, zero padding
+// up to W floats, stored as W/4 dwordx4.
+template <int W, int NC, int NDIM>
+__device__ __forceinline__ void pack_record(const float* out, float* rec) {
+    float4* r4 = reinterpret_cast<float4*>(rec);
+    float v[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) v[s] = s < NC ? out[NDIM == 2 ? rec2_src(s) : rec3_src(s)] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < W / 4; ++u) r4[u] = make_float4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+}
+
 __global__ void __launch_bounds__(256) gradients_kernel(DevParams prm, double idxh, double idyh,
                                                         double idzh, const float* __restrict__ n8,
                                                         float* __restrict__ rec,
@@ -66,19 +78,11 @@ __global__ void __launch_bounds__(256) gradients_kernel(DevParams prm, double id
     axis_grad(n8, node, j, prm.nyg, prm.nxg, idyh, out, 9);
     axis_grad(n8, node, k, prm.nzg, (long long)prm.nxg * prm.nyg, idzh, out, 10);
     if (prm.ndim == 2) {
-        float4* r4 = reinterpret_cast<float4*>(rec + node * kRec2D);
-        float v[kRec2D];
-#pragma unroll
-        for (int s = 0; s < kRec2D; ++s) v[s] = s < 15 ? out[rec2_src(s)] : 0.0f;
-#pragma unroll
-        for (int u = 0; u < kRec2D / 4; ++u) r4[u] = make_float4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+        if (prm.ext_rec) pack_record<kRec2DX, 18, 2>(out, rec + node * kRec2DX);
+        else pack_record<kRec2D, 15, 2>(out, rec + node * kRec2D);
     } else {
-        float4* r4 = reinterpret_cast<float4*>(rec + node * kRec3D);
-        float v[kRec3D];
-#pragma unroll
-        for (int s = 0; s < kRec3D; ++s) v[s] = s < 21 ? out[rec3_src(s)] : 0.0f;
-#pragma unroll
-        for (int u = 0; u < kRec3D / 4; ++u) r4[u] = make_float4(v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]);
+        if (prm.ext_rec) pack_record<kRec3DX, 28, 3>(out, rec + node * kRec3DX);
+        else pack_record<kRec3D, 21, 3>(out, rec + node * kRec3D);
     }
     if (fa32) {
 #pragma unroll

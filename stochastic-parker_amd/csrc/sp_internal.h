@@ -20,24 +20,38 @@ constexpr int kRec3D = 24;
 // six floats, then the 15 gradient slots of Rec3 (R3_DVXDX ..) as halves, two
 // per word, then padding (64 B = four dwordx4 loads).
 constexpr int kRec3DH = 16;
+// Records for steps with momentum diffusion (particle_module.f90:2918-2979):
+// the 2D / 3D slots above, then rho and the off-diagonal velocity gradients
+// of the flow-shear tensor.  2D: 18 components + 2 pads (80 B); 3D: 28
+// components (112 B).
+constexpr int kRec2DX = 20;
+constexpr int kRec3DX = 28;
 
 // 2D record slots
 enum Rec2 { R2_VX, R2_VY, R2_BX, R2_BY, R2_BZ, R2_DVXDX, R2_DVYDY, R2_DBXDX, R2_DBXDY,
-            R2_DBYDX, R2_DBYDY, R2_DBZDX, R2_DBZDY, R2_DBDX, R2_DBDY, R2_PAD };
+            R2_DBYDX, R2_DBYDY, R2_DBZDX, R2_DBZDY, R2_DBDX, R2_DBDY, R2_PAD,
+            R2_RHO = R2_PAD, R2_DVXDY, R2_DVYDX };
 // 3D record slots
 enum Rec3 { R3_VX, R3_VY, R3_VZ, R3_BX, R3_BY, R3_BZ, R3_DVXDX, R3_DVYDY, R3_DVZDZ,
             R3_DBXDX, R3_DBXDY, R3_DBXDZ, R3_DBYDX, R3_DBYDY, R3_DBYDZ,
-            R3_DBZDX, R3_DBZDY, R3_DBZDZ, R3_DBDX, R3_DBDY, R3_DBDZ };
+            R3_DBZDX, R3_DBZDY, R3_DBZDZ, R3_DBDX, R3_DBDY, R3_DBDZ,
+            R3_RHO, R3_DVXDY, R3_DVXDZ, R3_DVYDX, R3_DVYDZ, R3_DVZDX, R3_DVZDY };
 
 // farray component (0-based) stored in each record slot
+// (slots past 14 / 20 belong to the momentum-diffusion records)
 __host__ __device__ inline int rec2_src(int s) {
-    const int m[15] = {0, 1, 4, 5, 6, 8, 12, 20, 21, 23, 24, 26, 27, 29, 30};
-    return s < 15 ? m[s] : -1;
+    const int m[18] = {0, 1, 4, 5, 6, 8, 12, 20, 21, 23, 24, 26, 27, 29, 30, 3, 9, 11};
+    return s < 18 ? m[s] : -1;
 }
 __host__ __device__ inline int rec3_src(int s) {
-    const int m[21] = {0, 1, 2, 4, 5, 6, 8, 12, 16, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31};
-    return s < 21 ? m[s] : -1;
+    const int m[28] = {0, 1, 2, 4, 5, 6, 8, 12, 16, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31,
+                       3, 9, 10, 11, 13, 14, 15};
+    return s < 28 ? m[s] : -1;
 }
+
+// Step features chosen at compile time (template argument of the mover):
+// the acceleration-region gate and the momentum-diffusion terms.
+enum StepFeatures { FX_ACC = 1, FX_DPP = 2 };
 
 // Kernel-side copy of the configuration (passed by value).
 struct DevParams {
@@ -53,6 +67,7 @@ struct DevParams {
     double gamma_turb, pindex, kpara0, kret;
     double drift1, drift2;
     int check_drift_2d, acc_region_flag, time_interp, precision;
+    int ext_rec;                   // records carry the momentum-diffusion slots
     double acc_region[6];
     uint32_t key0, key1, stream;
 };
@@ -111,13 +126,15 @@ struct PtlBase {
 // Diffusion parameters of one variant, in the forms the two step paths use.
 struct KappaVar {
     double kpara0, kret, pindex, gamma_m2;
+    double p0_2mpi;               // p0**(2 - pindex), flow-shear momentum diffusion
     float f_kpara0, f_kret, f_pindex, f_gm2;
 };
 
 // Bit flags of MoverParams::flags
 enum MoverFlags {
     MF_OPEN_X = 1, MF_OPEN_Y = 2, MF_OPEN_Z = 4,
-    MF_MOMENTUM_DEP = 8, MF_MAG_DEP = 16, MF_DRIFT_2D = 32
+    MF_MOMENTUM_DEP = 8, MF_MAG_DEP = 16, MF_DRIFT_2D = 32,
+    MF_DPP_WAVE = 64, MF_DPP_SHEAR = 128, MF_WEAK_SCATTERING = 256
 };
 
 // Kernel arguments of the mover and of the single-step kernel: only the
@@ -157,6 +174,9 @@ struct MoverParams {
     // acceleration region (template-enabled)
     double acc_region[6];
     double lx, ly, lz;
+    // momentum diffusion (template-enabled)
+    double tau0, p0_2mpi;         // p0_2mpi = p0**(2 - pindex)
+    float f_tau0, f_p02;          // FP32 path: tau0, p0 * p0
 };
 
 // Records the calling thread's error message (sp_last_error) and returns code.
@@ -171,21 +191,21 @@ hipError_t launch_gradient_max(const DevParams& prm, const float* nodes8, unsign
                                hipStream_t s);
 hipError_t launch_pack_half(const DevParams& prm, const float* nodes8, float scale, float* rec,
                             float* farray32, hipStream_t s);
-int mover_blocks_per_cu_f32h(bool ti, bool acc);
-hipError_t launch_mover_f32h(const MoverParams& m, bool ti, bool acc, int grid, hipStream_t s);
-hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, bool acc, double* state,
+int mover_blocks_per_cu_f32h(bool ti, int fx);
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int grid, hipStream_t s);
+hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* state,
                                  const double* rands, const int8_t* fixed, double* deltas,
                                  hipStream_t s);
-int mover_blocks_per_cu_f64(int ndim, bool ti, bool acc);
-int mover_blocks_per_cu_f32(int ndim, bool ti, bool acc);
-hipError_t launch_mover_f64(const MoverParams& m, bool ti, bool acc, int ndim, int grid,
+int mover_blocks_per_cu_f64(int ndim, bool ti, int fx);
+int mover_blocks_per_cu_f32(int ndim, bool ti, int fx);
+hipError_t launch_mover_f64(const MoverParams& m, bool ti, int fx, int ndim, int grid,
                             hipStream_t s);
-hipError_t launch_mover_f32(const MoverParams& m, bool ti, bool acc, int ndim, int grid,
+hipError_t launch_mover_f32(const MoverParams& m, bool ti, int fx, int ndim, int grid,
                             hipStream_t s);
-hipError_t launch_step_once_f64(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+hipError_t launch_step_once_f64(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                 const double* rands, const int8_t* fixed, double* deltas,
                                 hipStream_t s);
-hipError_t launch_step_once_f32(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+hipError_t launch_step_once_f32(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                 const double* rands, const int8_t* fixed, double* deltas,
                                 hipStream_t s);
 

@@ -7,34 +7,35 @@ namespace sp {
 
 struct PolicyF64 {
     static constexpr bool kOnly3D = false;
-    template <int NDIM, bool TI, bool ACC>
+    static constexpr bool kDpp = true;
+    template <int NDIM, bool TI, int FX>
     __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
                                                 uint32_t tagi, uint32_t tags, uint32_t rstep,
                                                 double d[4]) {
         double r[4];
         f64::step_rands(m, tagi, tags, rstep, r);
-        f64::gather_kappa_push<NDIM, TI, ACC>(m, s, fixed, r, d);
+        f64::gather_kappa_push<NDIM, TI, FX>(m, s, fixed, r, d);
     }
-    template <int NDIM, bool TI, bool ACC>
+    template <int NDIM, bool TI, int FX>
     __device__ __forceinline__ static void step_given(const MoverParams& m, State& s, bool fixed,
                                                       const double* r, double d[4]) {
-        f64::gather_kappa_push<NDIM, TI, ACC>(m, s, fixed, r, d);
+        f64::gather_kappa_push<NDIM, TI, FX>(m, s, fixed, r, d);
     }
 };
 
-int mover_blocks_per_cu_f64(int ndim, bool ti, bool acc) {
-    return mover_occupancy<PolicyF64>(ndim, ti, acc);
+int mover_blocks_per_cu_f64(int ndim, bool ti, int fx) {
+    return mover_occupancy<PolicyF64>(ndim, ti, fx);
 }
 
-hipError_t launch_mover_f64(const MoverParams& m, bool ti, bool acc, int ndim, int grid,
+hipError_t launch_mover_f64(const MoverParams& m, bool ti, int fx, int ndim, int grid,
                             hipStream_t s) {
-    return launch_mover<PolicyF64>(m, ti, acc, ndim, grid, s);
+    return launch_mover<PolicyF64>(m, ti, fx, ndim, grid, s);
 }
 
-hipError_t launch_step_once_f64(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+hipError_t launch_step_once_f64(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                 const double* rands, const int8_t* fixed, double* deltas,
                                 hipStream_t s) {
-    return launch_step_once<PolicyF64>(m, ti, acc, ndim, state, rands, fixed, deltas, s);
+    return launch_step_once<PolicyF64>(m, ti, fx, ndim, state, rands, fixed, deltas, s);
 }
 
 }  // namespace sp

@@ -68,7 +68,7 @@ __device__ __forceinline__ void pin_constants(MoverParams& m) {
     pin_vgpr(m.lx); pin_vgpr(m.ly); pin_vgpr(m.lz);
 }
 
-template <class Pol, int NDIM, bool TINTERP, bool ACC>
+template <class Pol, int NDIM, bool TINTERP, int FX>
 __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
     MoverParams m = m0;
     // 2D steps are instruction-bound and gain from this; 3D steps are
@@ -137,6 +137,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
                         const KappaVar kv = m0.var[v];
                         m.kpara0 = kv.kpara0; m.kret = kv.kret;
                         m.pindex = kv.pindex; m.gamma_m2 = kv.gamma_m2;
+                        m.p0_2mpi = kv.p0_2mpi;
                         m.f_kpara0 = kv.f_kpara0; m.f_kret = kv.f_kret;
                         m.f_pindex = kv.f_pindex; m.f_gm2 = kv.f_gm2;
                     }
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
         // ---- one particle-step for every lane that takes one ----
         if (do_push) {
             const bool fixed = act == ACT_PUSH_FIXED;
-            Pol::template step<NDIM, TINTERP, ACC>(m, s, fixed, tagi, tags, rstep, dl);
+            Pol::template step<NDIM, TINTERP, FX>(m, s, fixed, tagi, tags, rstep, dl);
             rstep += 1u;
             int v = nsp + 1;  // mod(nsteps_pushed + 1, nsteps_interval)
             if ((unsigned)v >= (unsigned)m.nsteps_interval) v %= m.nsteps_interval;
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
 }
 
 // Single-step kernel for parity checks: caller-supplied states and increments.
-template <class Pol, int NDIM, bool TINTERP, bool ACC>
+template <class Pol, int NDIM, bool TINTERP, int FX>
 __global__ void __launch_bounds__(256) step_once_kernel(MoverParams m, double* state,
                                                         const double* rands, const int8_t* fixed,
                                                         double* deltas) {
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(256) step_once_kernel(MoverParams m, double* s
     State s{state[6 * i + 0], state[6 * i + 1], state[6 * i + 2],
             state[6 * i + 3], state[6 * i + 4], state[6 * i + 5]};
     double d[4] = {0.0, 0.0, 0.0, 0.0};
-    Pol::template step_given<NDIM, TINTERP, ACC>(m, s, fixed[i] != 0, rands + 4 * i, d);
+    Pol::template step_given<NDIM, TINTERP, FX>(m, s, fixed[i] != 0, rands + 4 * i, d);
     state[6 * i + 0] = s.x;
     state[6 * i + 1] = s.y;
     state[6 * i + 2] = s.z;
@@ -300,43 +301,53 @@ __global__ void __launch_bounds__(256) step_once_kernel(MoverParams m, double* s
     for (int c = 0; c < 4; ++c) deltas[4 * i + c] = d[c];
 }
 
-template <int N, bool T, bool A>
+template <int N, bool T, int X>
 struct Variant {
     static constexpr int ndim = N;
     static constexpr bool ti = T;
-    static constexpr bool acc = A;
+    static constexpr int fx = X;
 };
 
-// Calls f(Variant<...>{}) for the runtime (ndim, time_interp, acc_region)
-// combination; a policy with kOnly3D builds (and serves) 3D variants only.
-template <bool ONLY3D = false, class F>
-auto dispatch_variant(int ndim, bool ti, bool acc, F&& f) {
-    if constexpr (!ONLY3D) {
-        if (ndim == 2) {
-            if (ti) return acc ? f(Variant<2, true, true>{}) : f(Variant<2, true, false>{});
-            return acc ? f(Variant<2, false, true>{}) : f(Variant<2, false, false>{});
+template <bool DPP, int N, class F>
+auto dispatch_fx(bool ti, int fx, F&& f) {
+    const bool with_acc = (fx & FX_ACC) != 0;
+    if constexpr (DPP) {
+        if (fx & FX_DPP) {
+            if (ti) return with_acc ? f(Variant<N, true, FX_ACC | FX_DPP>{}) : f(Variant<N, true, FX_DPP>{});
+            return with_acc ? f(Variant<N, false, FX_ACC | FX_DPP>{}) : f(Variant<N, false, FX_DPP>{});
         }
     }
-    if (ti) return acc ? f(Variant<3, true, true>{}) : f(Variant<3, true, false>{});
-    return acc ? f(Variant<3, false, true>{}) : f(Variant<3, false, false>{});
+    if (ti) return with_acc ? f(Variant<N, true, FX_ACC>{}) : f(Variant<N, true, 0>{});
+    return with_acc ? f(Variant<N, false, FX_ACC>{}) : f(Variant<N, false, 0>{});
+}
+
+// Calls f(Variant<...>{}) for the runtime (ndim, time_interp, step features)
+// combination; a policy with kOnly3D builds (and serves) 3D variants only, one
+// without kDpp no momentum-diffusion variants.
+template <bool ONLY3D, bool DPP, class F>
+auto dispatch_variant(int ndim, bool ti, int fx, F&& f) {
+    if constexpr (!ONLY3D) {
+        if (ndim == 2) return dispatch_fx<DPP, 2>(ti, fx, f);
+    }
+    return dispatch_fx<DPP, 3>(ti, fx, f);
 }
 
 template <class Pol>
-int mover_occupancy(int ndim, bool ti, bool acc) {
-    return dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [](auto v) {
+int mover_occupancy(int ndim, bool ti, int fx) {
+    return dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [](auto v) {
         using V = decltype(v);
         int nb = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, mover_kernel<Pol, V::ndim, V::ti, V::acc>, 256, 0);
+            &nb, mover_kernel<Pol, V::ndim, V::ti, V::fx>, 256, 0);
         return e == hipSuccess ? nb : 1;
     });
 }
 
 template <class Pol>
-hipError_t launch_mover(const MoverParams& m, bool ti, bool acc, int ndim, int grid, hipStream_t st) {
-    dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [&](auto v) {
+hipError_t launch_mover(const MoverParams& m, bool ti, int fx, int ndim, int grid, hipStream_t st) {
+    dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
         using V = decltype(v);
-        hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::acc>), dim3(grid), dim3(256), 0,
+        hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::fx>), dim3(grid), dim3(256), 0,
                            st, m);
         return 0;
     });
@@ -344,14 +355,14 @@ hipError_t launch_mover(const MoverParams& m, bool ti, bool acc, int ndim, int g
 }
 
 template <class Pol>
-hipError_t launch_step_once(const MoverParams& m, bool ti, bool acc, int ndim, double* state,
+hipError_t launch_step_once(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                             const double* rands, const int8_t* fixed, double* deltas,
                             hipStream_t st) {
     if (m.n <= 0) return hipSuccess;
     const int grid = (int)((m.n + 255) / 256);
-    dispatch_variant<Pol::kOnly3D>(ndim, ti, acc, [&](auto v) {
+    dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
         using V = decltype(v);
-        hipLaunchKernelGGL((step_once_kernel<Pol, V::ndim, V::ti, V::acc>), dim3(grid), dim3(256),
+        hipLaunchKernelGGL((step_once_kernel<Pol, V::ndim, V::ti, V::fx>), dim3(grid), dim3(256),
                            0, st, m, state, rands, fixed, deltas);
         return 0;
     });

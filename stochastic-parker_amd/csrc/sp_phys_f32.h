@@ -51,11 +51,17 @@ __device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, 
     return I;
 }
 
-template <int NDIM, bool TINTERP>
+template <int NDIM, bool DPP>
+struct RecShape {
+    static constexpr int W = NDIM == 2 ? (DPP ? kRec2DX : kRec2D) : (DPP ? kRec3DX : kRec3D);
+    static constexpr int NC = NDIM == 2 ? (DPP ? 18 : 15) : (DPP ? 28 : 21);
+};
+
+template <int NDIM, bool TINTERP, bool DPP = false>
 __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, float rt,
                                        float* __restrict__ f) {
-    constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
-    constexpr int NC = NDIM == 2 ? 15 : 21;
+    constexpr int W = RecShape<NDIM, DPP>::W;
+    constexpr int NC = RecShape<NDIM, DPP>::NC;
     constexpr int NCORN = NDIM == 2 ? 4 : 8;
     const long long sx = 1, sy = m.nxg, sz = (long long)m.nxg * m.nyg;
     const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
@@ -151,12 +157,63 @@ __device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& 
     }
 }
 
+// Momentum diffusion in FP32 (f64::dpp_terms, particle_module.f90:2918-2979):
+// va^2 = B^2 / rho; p^(pindex-1) p0^(2-pindex) = p0^2 (p/p0)^pindex / p.
+template <int NDIM>
+__device__ __forceinline__ void dpp_terms(const MoverParams& m, float pf, const float* f,
+                                          float kpara, float knorm_para, float b2, float bx,
+                                          float by, float bz, float ib, float divv, float& dp_dt,
+                                          float& dpp) {
+    if (m.flags & MF_DPP_WAVE) {
+        const float rho = NDIM == 2 ? f[R2_RHO] : f[R3_RHO];
+        const float va2 = b2 * frcp(rho);
+        const float c = (m.flags & MF_MOMENTUM_DEP) ? (8.0f / 27.0f) : (4.0f / 9.0f);
+        const float pk = pf * frcp(kpara) * va2;
+        dp_dt += c * pk;
+        dpp += (1.0f / 9.0f) * pf * pk;
+    }
+    if (m.flags & MF_DPP_SHEAR) {
+        const float d3 = divv * (1.0f / 3.0f);
+        float sxx, syy, szz, sxy, sxz = 0.f, syz = 0.f;
+        if (NDIM == 2) {
+            sxx = f[R2_DVXDX] - d3;
+            syy = f[R2_DVYDY] - d3;
+            szz = -d3;
+            sxy = 0.5f * (f[R2_DVXDY] + f[R2_DVYDX]);
+        } else {
+            sxx = f[R3_DVXDX] - d3;
+            syy = f[R3_DVYDY] - d3;
+            szz = f[R3_DVZDZ] - d3;
+            sxy = 0.5f * (f[R3_DVXDY] + f[R3_DVYDX]);
+            sxz = 0.5f * (f[R3_DVXDZ] + f[R3_DVZDX]);
+            syz = 0.5f * (f[R3_DVYDZ] + f[R3_DVZDY]);
+        }
+        float gshear;
+        if (m.flags & MF_WEAK_SCATTERING) {
+            const float bbs = (sxx * bx * bx + syy * by * by + szz * bz * bz +
+                               2.0f * (sxy * bx * by + sxz * bx * bz + syz * by * bz)) * ib * ib;
+            gshear = bbs * bbs * 0.2f;
+        } else {
+            gshear = (2.0f / 15.0f) * (sxx * sxx + syy * syy + szz * szz +
+                                       2.0f * (sxy * sxy + sxz * sxz + syz * syz));
+        }
+        if (gshear > 0.0f) {
+            const float q = __powf(pf * (float)m.inv_p0, m.f_pindex);
+            const float dd = gshear * m.f_tau0 * knorm_para * m.f_p02 * q;
+            dp_dt += (2.0f + m.f_pindex) * dd * frcp(pf);
+            dpp += dd;
+        }
+    }
+}
+
 // One Parker step in FP32 coefficients.  Semantics match f64::push: same dt
 // rule (including the dt_min fallback when a rate is exactly zero), same
 // increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.
-template <int NDIM, bool ACC>
+template <int NDIM, int FX>
 __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const float* f,
                                            bool fixed_dt, const float r[4], double d[4]) {
+    constexpr bool ACC = (FX & FX_ACC) != 0;
+    constexpr bool DPP = (FX & FX_DPP) != 0;
     const float bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
     const float by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
     const float bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
@@ -172,6 +229,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     // --- kappa ---------------------------------------------------------
     float knorm = 1.0f;
     if (mag) knorm = __powf(b, m.f_gm2);
+    const float knorm_para = knorm;
     if (m.flags & MF_MOMENTUM_DEP) knorm *= __powf((float)(s.p * m.inv_p0), m.f_pindex);
     const float kpara = m.f_kpara0 * knorm;
     const float kperp = kpara * m.f_kret;
@@ -244,7 +302,9 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         dz_dt = f[R3_VZ] + vdz + dkxz_dx + dkyz_dy + dkzz_dz;
         divv = f[R3_DVXDX] + f[R3_DVYDY] + f[R3_DVZDZ];
     }
-    const float dp_dt = -pf * divv * (1.0f / 3.0f);
+    float dp_dt = -pf * divv * (1.0f / 3.0f);
+    float dpp = 0.0f;
+    if constexpr (DPP) dpp_terms<NDIM>(m, pf, f, kpara, knorm_para, b2, bx, by, bz, ib, divv, dp_dt, dpp);
 
     float dtf32;
     if (!fixed_dt) {
@@ -286,7 +346,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     s.y += (double)deltay;
     s.z += (double)deltaz;
     s.t += s.dt;
-    double dp = (double)(dp_dt * dtf32);
+    double dp = DPP ? (double)(dp_dt * dtf32 + r[3] * fsqrt(2.0f * dpp) * sdt) : (double)(dp_dt * dtf32);
     if (ACC) {
         const double xn = (s.x - m.xmin) / m.lx;
         const double yn = (s.y - m.ymin) / m.ly;
@@ -323,16 +383,17 @@ __device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i,
     r[3] = (float)u.d * sc + of;
 }
 
-template <int NDIM, bool TINTERP, bool ACC, bool HALF = false>
+template <int NDIM, bool TINTERP, int FX, bool HALF = false>
 __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
                                                   const float r[4], double d[4]) {
-    constexpr int NC = NDIM == 2 ? 15 : 21;
-    float f[NC];
+    constexpr bool DPP = (FX & FX_DPP) != 0;
+    static_assert(!(HALF && DPP), "FP16-gradient records carry no momentum-diffusion slots");
+    float f[RecShape<NDIM, DPP>::NC];
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (HALF && NDIM == 3) gather_half<TINTERP>(m, I, rt, f);
-    else gather<NDIM, TINTERP>(m, I, rt, f);
-    kappa_push<NDIM, ACC>(m, s, f, fixed_dt, r, d);
+    else gather<NDIM, TINTERP, DPP>(m, I, rt, f);
+    kappa_push<NDIM, FX>(m, s, f, fixed_dt, r, d);
 }
 
 }  // namespace f32

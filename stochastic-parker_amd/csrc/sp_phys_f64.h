@@ -51,11 +51,17 @@ __device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, 
 // interp_fields (mhd_data_parallel.f90:1751-1793) over node records:
 // fields = 0; for corners (k, j, i; i fastest) fields += f_node * w; then the
 // optional (1-rt, rt) blend of the two frames.
-template <int NDIM, bool TINTERP>
+template <int NDIM, bool DPP>
+struct RecShape {
+    static constexpr int W = NDIM == 2 ? (DPP ? kRec2DX : kRec2D) : (DPP ? kRec3DX : kRec3D);
+    static constexpr int NC = NDIM == 2 ? (DPP ? 18 : 15) : (DPP ? 28 : 21);
+};
+
+template <int NDIM, bool TINTERP, bool DPP = false>
 __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, double rt,
                                        double* __restrict__ f) {
-    constexpr int W = NDIM == 2 ? kRec2D : kRec3D;
-    constexpr int NC = NDIM == 2 ? 15 : 21;
+    constexpr int W = RecShape<NDIM, DPP>::W;
+    constexpr int NC = RecShape<NDIM, DPP>::NC;
     constexpr int NCORN = NDIM == 2 ? 4 : 8;
     double g[TINTERP ? NC : 1];
 #pragma unroll
@@ -97,7 +103,7 @@ __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, do
 }
 
 struct Kappa {
-    double kpara, kperp, skpara, skperp, skpara_perp;
+    double knorm_para, kpara, kperp, skpara, skperp, skpara_perp;
     double dkxx_dx, dkyy_dy, dkzz_dz, dkxy_dx, dkxy_dy, dkxz_dx, dkxz_dz, dkyz_dy, dkyz_dz;
 };
 
@@ -122,6 +128,7 @@ __device__ __forceinline__ Kappa kappa(const MoverParams& m, double p, const dou
     double knorm;
     if (m.flags & MF_MOMENTUM_DEP) knorm = knorm_para * pow(p / m.p0, m.pindex);
     else knorm = knorm_para;
+    k.knorm_para = knorm_para;
     k.kpara = m.kpara0 * knorm;
     k.kperp = k.kpara * m.kret;
     k.skpara = sqrt(2.0 * k.kpara);
@@ -196,14 +203,69 @@ __device__ __forceinline__ bool in_acc_region(const MoverParams& m, double x, do
 
 __device__ __forceinline__ double fmin_seq(double a, double b) { return b < a ? b : a; }
 
+// Momentum diffusion added to dp_dt and dpp: calc_dpp_wave_scattering
+// (particle_module.f90:2918-2933) with va = B / sqrt(rho), and
+// calc_dpp_flow_shear (:2947-2979) on the shear tensor of push_particle_2d
+// (:3487-3500; sigma_zz = -divv/3, no xz / yz terms) or push_particle_3d
+// (:4772-4787).
+template <int NDIM>
+__device__ __forceinline__ void dpp_terms(const MoverParams& m, double p, const double* f,
+                                          const Kappa& k, double b, double bx, double by,
+                                          double bz, double divv, double& dp_dt, double& dpp) {
+    if (m.flags & MF_DPP_WAVE) {
+        const double rho = NDIM == 2 ? f[R2_RHO] : f[R3_RHO];
+        const double va = b / sqrt(rho);
+        if (m.flags & MF_MOMENTUM_DEP) dp_dt = dp_dt + (8.0 * p / (27.0 * k.kpara)) * (va * va);
+        else dp_dt = dp_dt + (4.0 * p / (9.0 * k.kpara)) * (va * va);
+        dpp = dpp + ((p * va) * (p * va)) / (9.0 * k.kpara);
+    }
+    if (m.flags & MF_DPP_SHEAR) {
+        double sxx, syy, szz, sxy, sxz, syz;
+        if (NDIM == 2) {
+            sxx = f[R2_DVXDX] - divv / 3.0;
+            syy = f[R2_DVYDY] - divv / 3.0;
+            szz = -(divv / 3.0);
+            sxy = (f[R2_DVXDY] + f[R2_DVYDX]) / 2.0;
+            sxz = 0.0;
+            syz = 0.0;
+        } else {
+            sxx = f[R3_DVXDX] - divv / 3.0;
+            syy = f[R3_DVYDY] - divv / 3.0;
+            szz = f[R3_DVZDZ] - divv / 3.0;
+            sxy = (f[R3_DVXDY] + f[R3_DVYDX]) / 2.0;
+            sxz = (f[R3_DVXDZ] + f[R3_DVZDX]) / 2.0;
+            syz = (f[R3_DVYDZ] + f[R3_DVZDY]) / 2.0;
+        }
+        double gshear;
+        if (m.flags & MF_WEAK_SCATTERING) {
+            double ib;
+            if (b < __DBL_EPSILON__) ib = 0.0; else ib = 1.0 / b;
+            double bbs = sxx * (bx * bx) + syy * (by * by) + szz * (bz * bz) +
+                         2.0 * (sxy * bx * by + sxz * bx * bz + syz * by * bz);
+            bbs = bbs * ib * ib;
+            gshear = bbs * bbs / 5.0;
+        } else {
+            gshear = 2.0 * (sxx * sxx + syy * syy + szz * szz +
+                            2.0 * (sxy * sxy + sxz * sxz + syz * syz)) / 15.0;
+        }
+        if (gshear > 0.0) {
+            dp_dt = dp_dt + (2.0 + m.pindex) * gshear * m.tau0 * k.knorm_para *
+                                pow(p, m.pindex - 1.0) * m.p0_2mpi;
+            dpp = dpp + gshear * m.tau0 * k.knorm_para * pow(p, m.pindex) * m.p0_2mpi;
+        }
+    }
+}
+
 // push_particle_2d (particle_module.f90:3358-3606) / push_particle_3d
-// (:4625-4907), Cartesian, uniform grid, dpp terms off.  r = (ran1, ran2,
-// ran3, ran_p).  d receives (deltax, deltay, deltaz, deltap); the 2D pusher
-// leaves deltaz at zero like the reference (its z drift is not undone).
-template <int NDIM, bool ACC>
+// (:4625-4907), Cartesian, uniform grid; the momentum-diffusion terms when FX
+// has FX_DPP.  r = (ran1, ran2, ran3, ran_p).  d receives (deltax, deltay,
+// deltaz, deltap); the 2D pusher leaves deltaz at zero like the reference (its
+// z drift is not undone).
+template <int NDIM, int FX>
 __device__ __forceinline__ void push(const MoverParams& m, State& s, const double* f,
                                      const Kappa& k, bool fixed_dt, const double r[4],
                                      double d[4]) {
+    constexpr bool ACC = (FX & FX_ACC) != 0;
     const double vx = NDIM == 2 ? f[R2_VX] : f[R3_VX];
     const double vy = NDIM == 2 ? f[R2_VY] : f[R3_VY];
     const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
@@ -252,7 +314,9 @@ __device__ __forceinline__ void push(const MoverParams& m, State& s, const doubl
         dz_dt = vz + vdz + k.dkxz_dx + k.dkyz_dy + k.dkzz_dz;
         divv = f[R3_DVXDX] + f[R3_DVYDY] + f[R3_DVZDZ];
     }
-    const double dp_dt = -(s.p * divv / 3.0);
+    double dp_dt = -(s.p * divv / 3.0);
+    double dpp = 0.0;
+    if constexpr ((FX & FX_DPP) != 0) dpp_terms<NDIM>(m, s.p, f, k, b, bx, by, bz, divv, dp_dt, dpp);
 
     if (!fixed_dt) {
         // the zero test omits dz_dt in 3D (particle_module.f90:4792-4794)
@@ -291,7 +355,6 @@ __device__ __forceinline__ void push(const MoverParams& m, State& s, const doubl
     s.y = s.y + deltay;
     s.z = s.z + deltaz;
     s.t = s.t + s.dt;
-    const double dpp = 0.0;
     double dp = dp_dt * s.dt + r[3] * sqrt(2.0 * dpp) * sdt;
     if (ACC) {
         if (in_acc_region<NDIM>(m, s.x, s.y, s.z)) s.p = s.p + dp;
@@ -323,16 +386,16 @@ __device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i,
 }
 
 // The full per-step chain: interpolation parameters, gather, kappa, push.
-template <int NDIM, bool TINTERP, bool ACC>
+template <int NDIM, bool TINTERP, int FX>
 __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
                                                   const double r[4], double d[4]) {
-    constexpr int NC = NDIM == 2 ? 15 : 21;
-    double f[NC];
+    constexpr bool DPP = (FX & FX_DPP) != 0;
+    double f[RecShape<NDIM, DPP>::NC];
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const double rt = TINTERP ? (s.t - m.t0) / m.dtf : 0.0;
-    gather<NDIM, TINTERP>(m, I, rt, f);
+    gather<NDIM, TINTERP, DPP>(m, I, rt, f);
     const Kappa k = kappa<NDIM>(m, s.p, f);
-    push<NDIM, ACC>(m, s, f, k, fixed_dt, r, d);
+    push<NDIM, FX>(m, s, f, k, fixed_dt, r, d);
 }
 
 }  // namespace f64

@@ -19,6 +19,7 @@ module gpat_hip
         init_tstamps_mhd, calc_tstamps_mhd, free_tstamps_mhd, &
         read_particle_boundary_conditions, read_particle_params, read_diagnostics_params, &
         set_drift_parameters, set_flag_check_drift_2d, set_precision, set_sort_steps, &
+        set_dpp_params, &
         init_particles, free_particles, init_field_data, free_field_data, &
         read_field_data_parallel, calc_fields_gradients, copy_fields, &
         inject_particles_spatial_uniform, particle_mover, split_particle, &
@@ -268,6 +269,18 @@ contains
         cfg%seed = seed
     end subroutine set_precision
 
+    !< set_dpp_params (particle_module.f90:260-272): momentum diffusion by wave
+    !< scattering and by flow shear in the push
+    subroutine set_dpp_params(dpp_wave_int, dpp_shear_int, weak_scattering_flag, &
+            tau0_scattering)
+        integer, intent(in) :: dpp_wave_int, dpp_shear_int, weak_scattering_flag
+        real(dp), intent(in) :: tau0_scattering
+        cfg%dpp_wave = merge(1, 0, dpp_wave_int == 1)
+        cfg%dpp_shear = merge(1, 0, dpp_shear_int == 1)
+        cfg%weak_scattering = merge(1, 0, weak_scattering_flag == 1)
+        cfg%tau0 = tau0_scattering
+    end subroutine set_dpp_params
+
     !< spatial sort cadence inside one mover call (steps per particle; 0: once)
     subroutine set_sort_steps(nsteps)
         integer, intent(in) :: nsteps
@@ -289,6 +302,7 @@ contains
         cfg%stream = 0
         cfg%sort_every = 1
         cfg%reserved0 = 0
+        cfg%reserved1 = 0
         call check_status(sp_ctx_create(cfg, ctx), "sp_ctx_create")
         nptl_current = 0
         leak = 0.0_dp

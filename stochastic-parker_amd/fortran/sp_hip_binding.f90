@@ -18,11 +18,12 @@ module sp_hip_binding
         sp_local_dist_dims, sp_escaped_distributions, sp_quick_check, sp_calibration_copy, &
         sp_error_message, &
         sp_set_variants, sp_inject_variants, sp_variant_distributions, sp_autotune_mover
-    public SP_OK, SP_PREC_F64, SP_PREC_F32
+    public SP_OK, SP_PREC_F64, SP_PREC_F32, SP_PREC_F32H
 
     integer(c_int), parameter :: SP_OK = 0
     integer(c_int32_t), parameter :: SP_PREC_F64 = 0
     integer(c_int32_t), parameter :: SP_PREC_F32 = 1
+    integer(c_int32_t), parameter :: SP_PREC_F32H = 2
 
     type, bind(C) :: sp_config
         integer(c_int32_t) :: ndim, nx, ny, nz
@@ -43,6 +44,8 @@ module sp_hip_binding
         integer(c_int64_t) :: seed
         integer(c_int32_t) :: stream, sort_every, sort_steps, reserved0
         integer(c_int64_t) :: nptl_max
+        integer(c_int32_t) :: dpp_wave, dpp_shear, weak_scattering, reserved1
+        real(c_double) :: tau0
     end type sp_config
 
     type, bind(C) :: sp_particles

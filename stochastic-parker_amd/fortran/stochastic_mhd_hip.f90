@@ -6,7 +6,8 @@
 !< particle_module / mhd_data_parallel / diagnostics procedures.
 !<
 !< GPU-only flags: -gp/--gpu_precision (f64 = reference arithmetic, default;
-!< f32 = FP32 gather/coefficients with FP64 positions and time),
+!< f32 = FP32 gather/coefficients with FP64 positions and time; f32h = f32 on
+!< 3D records with FP16 gradients),
 !< -rs/--rng_seed (Philox key) and -ss/--sort_steps (spatial re-sort of the
 !< particles every that many steps inside an interval; 0 = once per interval).
 !*******************************************************************************
@@ -30,7 +31,9 @@ program stochastic_hip
     integer :: num_fine_steps = 1, tmax_to_inject = 100000, ndim_field = 2, charge = -1
     integer :: sort_steps = 0
     integer :: uniform_grid = 0, spherical_coord = 0, check_drift_2d = 0
-    integer :: dpp_wave = 0, dpp_shear = 0, deltab_flag = 0, correlation_flag = 0
+    integer :: dpp_wave = 0, dpp_shear = 0, weak_scattering = 0
+    real(dp) :: tau0_scattering = 1.0_dp
+    integer :: deltab_flag = 0, correlation_flag = 0
     integer :: include_3rd_dim = 0, acc_by_surface = 0, size_mpi_sub = 1
     logical :: focused_transport = .false., nlgc = .true., local_dist = .false.
     logical :: inject_new_ptl = .false., inject_part_box = .false., restart_flag = .false.
@@ -55,7 +58,12 @@ program stochastic_hip
     call read_diagnostics_params('config/'//trim(conf_file), t_end - t_start, local_dist)
     call set_drift_parameters(drift_param1, drift_param2, charge)
     if (ndim_field == 2 .and. check_drift_2d == 1) call set_flag_check_drift_2d(check_drift_2d)
-    call set_precision(merge(1, 0, trim(gpu_precision) == 'f32'), rng_seed)
+    call set_dpp_params(dpp_wave, dpp_shear, weak_scattering, tau0_scattering)
+    select case (trim(gpu_precision))
+    case ('f32'); call set_precision(1, rng_seed)
+    case ('f32h'); call set_precision(2, rng_seed)
+    case default; call set_precision(0, rng_seed)
+    end select
     call set_sort_steps(sort_steps)
     call init_particles(nptl_max)
     call init_particle_distributions(local_dist, dump_escaped_dist)
@@ -122,7 +130,8 @@ contains
         if (nlgc) call stop_scope("-nl .true. (NLGC); pass -nl .false.")
         if (uniform_grid /= 1) call stop_scope("-ug 0 (non-uniform grid); pass -ug 1")
         if (spherical_coord /= 0) call stop_scope("-sc 1 (spherical coordinates)")
-        if (dpp_wave /= 0 .or. dpp_shear /= 0) call stop_scope("-dw/-ds (momentum diffusion)")
+        if (trim(gpu_precision) == 'f32h' .and. (dpp_wave /= 0 .or. dpp_shear /= 0)) &
+            call stop_scope("-dw/-ds with -gp f32h (use f32 or f64)")
         if (deltab_flag /= 0 .or. correlation_flag /= 0) call stop_scope("-db/-co maps")
         if (include_3rd_dim /= 0) call stop_scope("-i3 1")
         if (acc_by_surface /= 0) call stop_scope("-as 1")
@@ -218,6 +227,8 @@ contains
             case ('-ld', '--local_dist'); local_dist = to_logical(val)
             case ('-dw', '--dpp_wave'); dpp_wave = to_int(val)
             case ('-ds', '--dpp_shear'); dpp_shear = to_int(val)
+            case ('-ws', '--weak_scattering'); weak_scattering = to_int(val)
+            case ('-t0', '--tau0_scattering'); tau0_scattering = to_real(val)
             case ('-db', '--deltab_flag'); deltab_flag = to_int(val)
             case ('-co', '--correlation'); correlation_flag = to_int(val)
             case ('-nd', '--ndim_field'); ndim_field = to_int(val)
@@ -235,7 +246,7 @@ contains
             case ('-rs', '--rng_seed'); rng_seed = int(to_real(val), c_int64_t)
             case ('-ss', '--sort_steps'); sort_steps = to_int(val)
             case ('-sn', '-ib', '-iv', '-ir', '-iaj', '-jz', '-nn', '-db2', '-nb', '-dv', '-nv', &
-                  '-rm', '-nr', '-ajm', '-naj', '-t0', '-ws', '-pd', '-ptf', '-de', '-du', &
+                  '-rm', '-nr', '-ajm', '-naj', '-pd', '-ptf', '-de', '-du', &
                   '-sf1', '-sn1', '-s2e', '-ii', '-sf2', '-sn2')
                 ! accepted and ignored: options of paths this build does not run
             case default

@@ -51,6 +51,8 @@ class SpConfig(C.Structure):
         ("seed", C.c_uint64), ("stream", C.c_uint32), ("sort_every", C.c_int32),
         ("sort_steps", C.c_int32), ("reserved0", C.c_int32),
         ("nptl_max", C.c_int64),
+        ("dpp_wave", C.c_int32), ("dpp_shear", C.c_int32), ("weak_scattering", C.c_int32),
+        ("reserved1", C.c_int32), ("tau0", C.c_double),
     ]
 
 

@@ -25,7 +25,7 @@ def test_bind_c_layout_matches_header():
     _need(os.path.join(BIN, "abi_check"))
     out = subprocess.run([os.path.join(BIN, "abi_check")], capture_output=True, text=True,
                          check=True).stdout.split()
-    assert [int(v) for v in out] == [320, 56, 40, 176, 136, 32, 32]
+    assert [int(v) for v in out] == [344, 56, 40, 176, 136, 32, 32]
 
 
 @pytest.mark.parametrize("args,msg", [

@@ -349,3 +349,102 @@ def test_rng_modes_agree_in_distribution():
     for key in ("p", "x", "y"):
         assert stats.ks_2samp(mt[key], ph[key]).pvalue > 1e-3, key
     assert np.log(mt["p"]).std() > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# Momentum diffusion (calc_dpp_wave_scattering / calc_dpp_flow_shear).  The
+# push's momentum increment is dp = A dt + r sqrt(2 D) sqrt(dt); A and D are
+# read back from two pushes (r = 0 and r = 1).  An Ito SDE for the momentum
+# with diffusion D(p) and isotropic f(p) p^2 must carry the drift
+# A = (1/p^2) d(p^2 D)/dp (besides -p divV/3): checked by finite differences
+# of D, which pins drift and diffusion against each other.
+
+def momentum_rates(prm, ndim, f, p, dt=1e-8):
+    k = kappa(prm, p, f)
+    _, d0 = push(prm, ndim, f, k, p=p, dt=dt)
+    _, d1 = push(prm, ndim, f, k, p=p, dt=dt, rands=(0, 0, 0, 1.0))
+    return d0[3] / dt, ((d1[3] - d0[3]) / np.sqrt(dt)) ** 2 / 2.0
+
+
+def fp_drift(prm, ndim, f, p, rel=1e-4):
+    h = rel * p
+    dp_ = momentum_rates(prm, ndim, f, p + h)[1]
+    dm_ = momentum_rates(prm, ndim, f, p - h)[1]
+    return ((p + h) ** 2 * dp_ - (p - h) ** 2 * dm_) / (2 * h) / p ** 2
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("momdep", [1, 0])
+def test_dpp_wave_rates(ndim, momdep):
+    c = (cfg2d if ndim == 2 else cfg3d)(momentum_dependency=momdep, dpp_wave=1)
+    prm = oracle.params(c)
+    f = np.zeros(32)
+    f[3] = 4.0                       # rho: Alfven speed |B| / sqrt(rho) = 0.5
+    f[4:7] = (0.6, 0.0, 0.8)
+    f[7] = 1.0
+    for p in (0.7, 1.0, 3.0, 10.0):
+        a, d = momentum_rates(prm, ndim, f, p)
+        kpara = c["kpara0"] * (p / c["p0"]) ** (prm.pindex if momdep else 0.0)
+        # quasi-linear momentum diffusion by Alfven waves: D = p^2 va^2 / (9 kpara)
+        assert d == pytest.approx(p * p * 0.25 / (9.0 * kpara), rel=1e-6)
+        assert a == pytest.approx(fp_drift(prm, ndim, f, p), rel=1e-6)
+    # without the flag neither term is present
+    prm0 = oracle.params(dict(c, dpp_wave=0))
+    assert momentum_rates(prm0, ndim, f, 2.0) == (0.0, 0.0)
+
+
+SHEAR_SLOTS_3D = {"dvx_dy": 9, "dvx_dz": 10, "dvy_dx": 11, "dvy_dz": 13, "dvz_dx": 14, "dvz_dy": 15}
+
+
+@pytest.mark.parametrize("ndim,slot", [(2, "dvx_dy"), (2, "dvy_dx")] +
+                         [(3, s) for s in SHEAR_SLOTS_3D])
+def test_dpp_shear_simple_shear(ndim, slot):
+    """Strong scattering: Gamma = (2/15) sigma:sigma; a simple shear dv_i/dx_j = a
+    (i != j) has sigma:sigma = a^2 / 2, so Gamma = a^2 / 15 whatever the axes."""
+    tau0, a = 2.5, 0.8
+    c = (cfg2d if ndim == 2 else cfg3d)(dpp_shear=1, tau0=tau0)
+    prm = oracle.params(c)
+    f = np.zeros(32)
+    f[3] = 1.0
+    f[4:7] = (0.3, 0.4, 0.5)
+    f[7] = np.sqrt(0.5)
+    f[SHEAR_SLOTS_3D[slot]] = a
+    for p in (0.5, 1.0, 4.0):
+        acc, d = momentum_rates(prm, ndim, f, p)
+        gamma = a * a / 15.0
+        assert d == pytest.approx(gamma * tau0 * p ** prm.pindex * c["p0"] ** (2 - prm.pindex),
+                                  rel=1e-6)
+        assert acc == pytest.approx(fp_drift(prm, ndim, f, p), rel=1e-6)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_dpp_shear_compression_and_weak_scattering(ndim):
+    tau0, a = 1.5, 0.6
+    c = (cfg2d if ndim == 2 else cfg3d)(dpp_shear=1, tau0=tau0)
+    prm = oracle.params(c)
+    f = np.zeros(32)
+    f[3] = 1.0
+    f[4:7] = (1.0, 0.0, 0.0)
+    f[7] = 1.0
+    # pure compression along x: sigma = a diag(2/3, -1/3, -1/3), sigma:sigma = 2 a^2 / 3
+    f[8] = a
+    p = 2.0
+    acc, d = momentum_rates(prm, ndim, f, p)
+    dref = (2.0 / 15.0) * (2.0 * a * a / 3.0) * tau0 * p ** prm.pindex
+    assert d == pytest.approx(dref, rel=1e-6)
+    assert acc == pytest.approx(-p * a / 3.0 + (2 + prm.pindex) * dref / p, rel=1e-6)
+    # weak scattering: Gamma = (b.sigma.b)^2 / 5
+    cw = dict(c, weak_scattering=1)
+    prmw = oracle.params(cw)
+    _, dw = momentum_rates(prmw, ndim, f, p)       # b along x: b.sigma.b = 2a/3
+    assert dw == pytest.approx((2.0 * a / 3.0) ** 2 / 5.0 * tau0 * p ** prm.pindex, rel=1e-6)
+    g = np.zeros(32)
+    g[3] = 1.0
+    g[4:7] = (1.0, 1.0, 0.0)
+    g[7] = np.sqrt(2.0)
+    g[9] = a                                       # dvx/dy: b.sigma.b = a/2 at 45 degrees
+    _, dw = momentum_rates(prmw, ndim, g, p)
+    assert dw == pytest.approx((a / 2.0) ** 2 / 5.0 * tau0 * p ** prm.pindex, rel=1e-6)
+    g[4:7] = (1.0, 0.0, 0.0)
+    g[7] = 1.0                                     # b along x: b.sigma.b = 0, no term
+    assert momentum_rates(prmw, ndim, g, p) == (0.0, 0.0)
