@@ -96,8 +96,10 @@ typedef struct sp_config {
     int32_t dpp_wave;           /* 1: wave scattering (calc_dpp_wave_scattering, :2918-2933) */
     int32_t dpp_shear;          /* 1: flow shear (calc_dpp_flow_shear, :2947-2979) */
     int32_t weak_scattering;    /* 1: the weak-scattering form of the flow-shear rate */
-    int32_t reserved1;          /* must be 0 */
+    int32_t nlgc;               /* 1: kperp from NLGC (calc_spatial_diffusion_coefficients_nlgc,
+                                   particle_module.f90:2464-2771; -nl) */
     double tau0;                /* scattering time of particles with p0 (flow shear) */
+    double kperp_kpara;         /* NLGC kperp / kpara at p0 (-kk) */
 } sp_config;
 
 /* Structure-of-arrays view of particle_type (particle_module.f90:38-50) plus
@@ -199,7 +201,9 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
  * increments (ran1, ran2, ran3, ran_p per particle, interleaved), as one
  * iteration of the hot loop (particle_module.f90:1615-1690).  state rows are
  * (x, y, z, p, t, dt) per particle and are updated in place; deltas receives
- * (dx, dy, dz, dp) per particle.  fixed_dt is per particle (0/1). */
+ * (dx, dy, dz, dp) per particle.  fixed_dt is per particle (0/1).  With nlgc,
+ * row i takes its pitch-angle cosine mu from the context's particle i
+ * (n <= nptl_current). */
 int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
                  const double* rands, const int8_t* fixed_dt, double* deltas);
 

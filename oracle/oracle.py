@@ -44,7 +44,7 @@ class OrcParams(C.Structure):
         ("time_interp", C.c_int32),
         ("seed", C.c_uint64), ("stream", C.c_uint32),
         ("dpp_wave", C.c_int32), ("dpp_shear", C.c_int32), ("weak_scattering", C.c_int32),
-        ("tau0", C.c_double),
+        ("tau0", C.c_double), ("nlgc", C.c_int32), ("kperp_kpara", C.c_double),
     ]
 
 
@@ -102,6 +102,7 @@ def load():
                                             C.POINTER(C.c_int), _D]),
         "orc_interp_fields": (None, [P, _F, _F, C.POINTER(C.c_int), _D, C.c_double, _D]),
         "orc_kappa_standard": (None, [P, C.c_double, _D, C.POINTER(OrcKappa)]),
+        "orc_kappa_nlgc": (None, [P, C.c_double, C.c_double, _D, C.POINTER(OrcKappa)]),
         "orc_push_particle_2d": (None, [P, _D, _PP, _D, C.POINTER(OrcKappa), C.c_int, _D, _D]),
         "orc_push_particle_3d": (None, [P, _D, _PP, _D, C.POINTER(OrcKappa), C.c_int, _D, _D]),
         "orc_step_once": (None, [P, _F, _F, C.c_double, C.c_double, _PP, C.c_int, _D, _D]),
@@ -136,7 +137,7 @@ def params(cfg: dict, seed: int = 20261202, stream: int = 0) -> OrcParams:
     p = OrcParams()
     for name, _ in OrcParams._fields_:
         if name in ("pindex", "pcharge", "acc_region", "seed", "stream", "dpp_wave", "dpp_shear",
-                    "weak_scattering", "tau0"):
+                    "weak_scattering", "tau0", "nlgc", "kperp_kpara"):
             continue
         setattr(p, name, cfg[name])
     p.pindex = 3.0 - cfg["gamma_turb"]
@@ -148,6 +149,8 @@ def params(cfg: dict, seed: int = 20261202, stream: int = 0) -> OrcParams:
     p.dpp_shear = cfg.get("dpp_shear", 0)
     p.weak_scattering = cfg.get("weak_scattering", 0)
     p.tau0 = cfg.get("tau0", 0.0)
+    p.nlgc = cfg.get("nlgc", 0)
+    p.kperp_kpara = cfg.get("kperp_kpara", 0.0)
     p.seed = cfg.get("seed", seed)
     p.stream = cfg.get("stream", stream)
     return p

@@ -369,6 +369,91 @@ void orc_kappa_standard(const orc_params* prm, double p, const double f[32], orc
     }
 }
 
+/* calc_spatial_diffusion_coefficients_nlgc, particle_module.f90:2464-2771
+ * (Cartesian, Parker, no delta-B or correlation-length maps, no third
+ * dimension in 2D): kperp = kpara0 * kperp_kpara * B^((gamma-2)/3) *
+ * (p/p0)^((5-gamma)/3) * mu^2, and separate gradients of ln kpara and
+ * ln kperp (the latter a third of the former) in the divergence terms. */
+void orc_kappa_nlgc(const orc_params* prm, double p, double mu, const double f[32], orc_kappa* k)
+{
+    const double bx = f[4], by = f[5], bz = f[6];
+    const double b = sqrt(bx * bx + by * by + bz * bz);
+    const double gamma_turb = prm->gamma_turb;
+    double ib1;
+    if (b < DBL_EPSILON) ib1 = 1.0; else ib1 = 1.0 / b;
+    const double ib2 = ib1 * ib1;
+    const double ib3 = ib1 * ib2;
+
+    k->knorm_para = 1.0;
+    k->knorm_perp = 1.0;
+    if (prm->mag_dependency == 1) {
+        k->knorm_para = k->knorm_para * pow(b, gamma_turb - 2.0);
+        k->knorm_perp = k->knorm_perp * pow(b, (gamma_turb - 2.0) / 3.0);
+    }
+    double knorm_para, knorm_perp;
+    if (prm->momentum_dependency == 1) {
+        knorm_para = k->knorm_para * pow(p / prm->p0, prm->pindex);
+        knorm_perp = k->knorm_perp * pow(p / prm->p0, (5.0 - gamma_turb) / 3.0);
+    } else {
+        knorm_para = k->knorm_para;
+        knorm_perp = k->knorm_perp;
+    }
+    k->kpara = prm->kpara0 * knorm_para;
+    k->kperp = prm->kpara0 * prm->kperp_kpara * knorm_perp * (mu * mu);
+    k->skpara = sqrt(2.0 * k->kpara);
+    k->skperp = sqrt(2.0 * k->kperp);
+    k->skpara_perp = sqrt(2.0 * (k->kpara - k->kperp));
+
+    const double kpara = k->kpara, kperp = k->kperp;
+    const double kpp = kpara - kperp;
+    const int nd = prm->ndim;
+    const double dbx_dx = f[20], dbx_dy = f[21], dbx_dz = nd == 3 ? f[22] : 0.0;
+    const double dby_dx = f[23], dby_dy = f[24], dby_dz = nd == 3 ? f[25] : 0.0;
+    const double dbz_dx = f[26], dbz_dy = f[27], dbz_dz = nd == 3 ? f[28] : 0.0;
+    const double db_dx = f[29], db_dy = f[30], db_dz = nd == 3 ? f[31] : 0.0;
+    double dkpara_dx = 0.0, dkpara_dy = 0.0, dkpara_dz = 0.0;
+    double dkperp_dx = 0.0, dkperp_dy = 0.0, dkperp_dz = 0.0;
+    if (prm->mag_dependency == 1) {
+        dkpara_dx = db_dx * ib1 * (gamma_turb - 2.0);
+        dkpara_dy = db_dy * ib1 * (gamma_turb - 2.0);
+        dkperp_dx = db_dx * ib1 * (gamma_turb - 2.0) / 3.0;
+        dkperp_dy = db_dy * ib1 * (gamma_turb - 2.0) / 3.0;
+        if (nd == 3) {
+            dkpara_dz = db_dz * ib1 * (gamma_turb - 2.0);
+            dkperp_dz = db_dz * ib1 * (gamma_turb - 2.0) / 3.0;
+        }
+    }
+    k->dkxx_dx = kperp * dkperp_dx + (kpara * dkpara_dx - kperp * dkperp_dx) * (bx * bx) * ib2 +
+                 2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+    k->dkyy_dy = kperp * dkperp_dy + (kpara * dkpara_dy - kperp * dkperp_dy) * (by * by) * ib2 +
+                 2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+    k->dkxy_dx = (kpara * dkpara_dx - kperp * dkperp_dx) * bx * by * ib2 +
+                 kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+    k->dkxy_dy = (kpara * dkpara_dy - kperp * dkperp_dy) * bx * by * ib2 +
+                 kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+    k->kxx = kperp + kpp * bx * bx * ib2;
+    k->kyy = kperp + kpp * by * by * ib2;
+    k->kxy = kpp * bx * by * ib2;
+    if (nd == 3) {
+        k->dkzz_dz = kperp * dkperp_dz + (kpara * dkpara_dz - kperp * dkperp_dz) * (bz * bz) * ib2 +
+                     2.0 * kpp * bz * (dbz_dz * b - bz * db_dz) * ib3;
+        k->dkxz_dx = (kpara * dkpara_dx - kperp * dkperp_dx) * bx * bz * ib2 +
+                     kpp * ((dbx_dx * bz + bx * dbz_dx) * ib2 - 2.0 * bx * bz * db_dx * ib3);
+        k->dkxz_dz = (kpara * dkpara_dz - kperp * dkperp_dz) * bx * bz * ib2 +
+                     kpp * ((dbx_dz * bz + bx * dbz_dz) * ib2 - 2.0 * bx * bz * db_dz * ib3);
+        k->dkyz_dy = (kpara * dkpara_dy - kperp * dkperp_dy) * by * bz * ib2 +
+                     kpp * ((dby_dy * bz + by * dbz_dy) * ib2 - 2.0 * by * bz * db_dy * ib3);
+        k->dkyz_dz = (kpara * dkpara_dz - kperp * dkperp_dz) * by * bz * ib2 +
+                     kpp * ((dby_dz * bz + by * dbz_dz) * ib2 - 2.0 * by * bz * db_dz * ib3);
+        k->kzz = kperp + kpp * bz * bz * ib2;
+        k->kxz = kpp * bx * bz * ib2;
+        k->kyz = kpp * by * bz * ib2;
+    } else {
+        k->kzz = k->kxz = k->kyz = 0.0;
+        k->dkzz_dz = k->dkxz_dx = k->dkxz_dz = k->dkyz_dy = k->dkyz_dz = 0.0;
+    }
+}
+
 /* particle_in_acceleration_region, particle_module.f90:2885-2906 */
 static int in_acc_region(const orc_params* prm, const orc_particle* ptl)
 {
@@ -642,7 +727,8 @@ static void gather_kappa_push(const orc_params* prm, const float* farray1, const
     orc_kappa kap;
     orc_get_interp_paramters(prm, px, py, pz, pos, w);
     orc_interp_fields(prm, farray1, farray2, pos, w, rt, fields);
-    orc_kappa_standard(prm, ptl->p, fields, &kap);
+    if (prm->nlgc) orc_kappa_nlgc(prm, ptl->p, ptl->mu, fields, &kap);
+    else orc_kappa_standard(prm, ptl->p, fields, &kap);
     if (prm->ndim == 2) {
         double d[4];
         orc_push_particle_2d(prm, dtmm, ptl, fields, &kap, fixed_dt, rands, d);

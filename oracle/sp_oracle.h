@@ -74,6 +74,9 @@ typedef struct {
     /* momentum diffusion (set_dpp_params, particle_module.f90:260-272) */
     int32_t dpp_wave, dpp_shear, weak_scattering;
     double tau0;
+    /* NLGC kperp (calc_spatial_diffusion_coefficients_nlgc) */
+    int32_t nlgc;
+    double kperp_kpara;
 } orc_params;
 
 /* kappa_type, particle_module.f90:93-104 */
@@ -110,6 +113,8 @@ void orc_interp_fields(const orc_params* prm, const float* farray1, const float*
 /* ---- physics --------------------------------------------------------- */
 void orc_kappa_standard(const orc_params* prm, double p, const double fields[32],
                         orc_kappa* kappa);
+void orc_kappa_nlgc(const orc_params* prm, double p, double mu, const double fields[32],
+                    orc_kappa* k);
 /* One Parker step.  rands = (ran1, ran2, ran3, ran_p) already mapped to the
  * increment law; dtmm = (dt_min, dt_max).  Deltas are written to delta[4]
  * (x, y, z, p); for 2D delta[2] is left at 0 as in the reference. */

@@ -156,8 +156,9 @@ int validate(const sp_config* c) {
     if (c->sort_steps < 0 || c->reserved0 != 0) return fail(SP_ERR_ARG, "need sort_steps >= 0 and reserved0 == 0");
     if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
     if (c->dpp_wave < 0 || c->dpp_wave > 1 || c->dpp_shear < 0 || c->dpp_shear > 1 ||
-        c->weak_scattering < 0 || c->weak_scattering > 1 || c->reserved1 != 0)
-        return fail(SP_ERR_ARG, "dpp_wave, dpp_shear, weak_scattering must be 0 or 1 and reserved1 0");
+        c->weak_scattering < 0 || c->weak_scattering > 1 || c->nlgc < 0 || c->nlgc > 1)
+        return fail(SP_ERR_ARG, "dpp_wave, dpp_shear, weak_scattering and nlgc must be 0 or 1");
+    if (c->nlgc && !(c->kperp_kpara >= 0)) return fail(SP_ERR_ARG, "kperp_kpara must be >= 0");
     if (c->dpp_shear && !(c->tau0 >= 0)) return fail(SP_ERR_ARG, "tau0 must be >= 0");
     if ((c->dpp_wave || c->dpp_shear) && c->precision == SP_PREC_F32H)
         return fail(SP_ERR_UNSUPPORTED, "momentum diffusion needs SP_PREC_F64 or SP_PREC_F32 records");
@@ -166,7 +167,8 @@ int validate(const sp_config* c) {
 
 // Step features of a configuration (template argument of the mover).
 int step_features(const sp_config& c) {
-    return (c.acc_region_flag == 1 ? sp::FX_ACC : 0) | (c.dpp_wave || c.dpp_shear ? sp::FX_DPP : 0);
+    return (c.acc_region_flag == 1 ? sp::FX_ACC : 0) | (c.dpp_wave || c.dpp_shear ? sp::FX_DPP : 0) |
+           (c.nlgc ? sp::FX_NLGC : 0);
 }
 
 sp::DevParams make_params(const sp_config& c) {
@@ -284,6 +286,10 @@ sp::KappaVar kappa_var(double p0, double kpara0, double kret, double gamma_turb)
     k.pindex = 3.0 - gamma_turb;
     k.gamma_m2 = gamma_turb - 2.0;
     k.p0_2mpi = std::pow(p0, 2.0 - k.pindex);   // p0**(2.0-pindex) (:2975-2977)
+    k.perp_mag = (gamma_turb - 2.0) / 3.0;      // NLGC kperp exponents (:2499-2525)
+    k.perp_mom = (5.0 - gamma_turb) / 3.0;
+    k.f_perp_mag = (float)k.perp_mag;
+    k.f_perp_mom = (float)k.perp_mom;
     k.f_kpara0 = (float)kpara0;
     k.f_kret = (float)kret;
     k.f_pindex = (float)k.pindex;
@@ -356,6 +362,12 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.lx = f.lx; m.ly = f.ly; m.lz = f.lz;
     m.tau0 = f.tau0;
     m.p0_2mpi = kv.p0_2mpi;
+    m.kperp_kpara = f.kperp_kpara;
+    m.perp_mag = kv.perp_mag;
+    m.perp_mom = kv.perp_mom;
+    m.f_kperp_kpara = (float)f.kperp_kpara;
+    m.f_perp_mag = kv.f_perp_mag;
+    m.f_perp_mom = kv.f_perp_mom;
     m.f_tau0 = (float)f.tau0;
     m.f_p02 = (float)(f.p0 * f.p0);
     return m;
@@ -772,6 +784,8 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
     if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
     if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    if (c->cfg.nlgc && n > c->nptl)
+        return fail(SP_ERR_ARG, "with nlgc, row i takes mu of particle i: need n <= nptl_current");
     double *ds = nullptr, *dr = nullptr, *dd = nullptr;
     int8_t* df = nullptr;
     hipError_t e = hipMalloc(&ds, sizeof(double) * 6 * n);

@@ -99,6 +99,7 @@ __device__ __forceinline__ int clamp_cell(double fl, int ncell_max) {
 // Per-particle dynamic state carried through the steps of one interval.
 struct State {
     double x, y, z, p, t, dt;
+    double mu;                    // read only by NLGC steps (kperp ~ mu^2)
 };
 
 }  // namespace sp

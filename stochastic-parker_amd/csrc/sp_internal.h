@@ -50,8 +50,9 @@ __host__ __device__ inline int rec3_src(int s) {
 }
 
 // Step features chosen at compile time (template argument of the mover):
-// the acceleration-region gate and the momentum-diffusion terms.
-enum StepFeatures { FX_ACC = 1, FX_DPP = 2 };
+// the acceleration-region gate, the momentum-diffusion terms and the NLGC
+// perpendicular diffusion coefficient.
+enum StepFeatures { FX_ACC = 1, FX_DPP = 2, FX_NLGC = 4 };
 
 // Kernel-side copy of the configuration (passed by value).
 struct DevParams {
@@ -127,7 +128,9 @@ struct PtlBase {
 struct KappaVar {
     double kpara0, kret, pindex, gamma_m2;
     double p0_2mpi;               // p0**(2 - pindex), flow-shear momentum diffusion
+    double perp_mag, perp_mom;    // NLGC kperp exponents (gamma-2)/3 of B, (5-gamma)/3 of p/p0
     float f_kpara0, f_kret, f_pindex, f_gm2;
+    float f_perp_mag, f_perp_mom;
 };
 
 // Bit flags of MoverParams::flags
@@ -177,6 +180,9 @@ struct MoverParams {
     // momentum diffusion (template-enabled)
     double tau0, p0_2mpi;         // p0_2mpi = p0**(2 - pindex)
     float f_tau0, f_p02;          // FP32 path: tau0, p0 * p0
+    // NLGC kperp (template-enabled): kpara0 * kperp_kpara * B^perp_mag * (p/p0)^perp_mom * mu^2
+    double kperp_kpara, perp_mag, perp_mom;
+    float f_kperp_kpara, f_perp_mag, f_perp_mom;
 };
 
 // Records the calling thread's error message (sp_last_error) and returns code.

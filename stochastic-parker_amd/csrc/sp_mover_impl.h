@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
     bool exhausted = false;
     // start on this die's part of the range, then help the other parts
     int part = (int)(xcc_id() % (unsigned)kParts), parts_done = 0;
-    State s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    State s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double w = 0.0, dtarget = 0.0, dt_old = 0.0;
     double dl[4] = {0.0, 0.0, 0.0, 0.0};
     int cf = 0, nsp = 0, pending = PEND_NONE;
@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
                     s.p = pp[idx];
                     s.t = pt[idx];
                     s.dt = pdt[idx];
+                    if constexpr ((FX & FX_NLGC) != 0) s.mu = m.P.d(PD_MU)[idx];
                     w = m.P.d(PD_W)[idx];
                     cf = pflag[idx];
                     nsp = ppushed[idx];
@@ -138,6 +139,8 @@ __global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
                         m.kpara0 = kv.kpara0; m.kret = kv.kret;
                         m.pindex = kv.pindex; m.gamma_m2 = kv.gamma_m2;
                         m.p0_2mpi = kv.p0_2mpi;
+                        m.perp_mag = kv.perp_mag; m.perp_mom = kv.perp_mom;
+                        m.f_perp_mag = kv.f_perp_mag; m.f_perp_mom = kv.f_perp_mom;
                         m.f_kpara0 = kv.f_kpara0; m.f_kret = kv.f_kret;
                         m.f_pindex = kv.f_pindex; m.f_gm2 = kv.f_gm2;
                     }
@@ -289,7 +292,8 @@ __global__ void __launch_bounds__(256) step_once_kernel(MoverParams m, double* s
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
     State s{state[6 * i + 0], state[6 * i + 1], state[6 * i + 2],
-            state[6 * i + 3], state[6 * i + 4], state[6 * i + 5]};
+            state[6 * i + 3], state[6 * i + 4], state[6 * i + 5], 0.0};
+    if constexpr ((FX & FX_NLGC) != 0) s.mu = m.P.d(PD_MU)[i];
     double d[4] = {0.0, 0.0, 0.0, 0.0};
     Pol::template step_given<NDIM, TINTERP, FX>(m, s, fixed[i] != 0, rands + 4 * i, d);
     state[6 * i + 0] = s.x;
@@ -308,17 +312,23 @@ struct Variant {
     static constexpr int fx = X;
 };
 
+template <bool DPP, int N, int BASE, class F>
+auto dispatch_acc(bool ti, int fx, F&& f) {
+    const bool with_acc = (fx & FX_ACC) != 0;
+    if (ti) return with_acc ? f(Variant<N, true, BASE | FX_ACC>{}) : f(Variant<N, true, BASE>{});
+    return with_acc ? f(Variant<N, false, BASE | FX_ACC>{}) : f(Variant<N, false, BASE>{});
+}
+
 template <bool DPP, int N, class F>
 auto dispatch_fx(bool ti, int fx, F&& f) {
-    const bool with_acc = (fx & FX_ACC) != 0;
     if constexpr (DPP) {
         if (fx & FX_DPP) {
-            if (ti) return with_acc ? f(Variant<N, true, FX_ACC | FX_DPP>{}) : f(Variant<N, true, FX_DPP>{});
-            return with_acc ? f(Variant<N, false, FX_ACC | FX_DPP>{}) : f(Variant<N, false, FX_DPP>{});
+            if (fx & FX_NLGC) return dispatch_acc<DPP, N, FX_DPP | FX_NLGC>(ti, fx, f);
+            return dispatch_acc<DPP, N, FX_DPP>(ti, fx, f);
         }
     }
-    if (ti) return with_acc ? f(Variant<N, true, FX_ACC>{}) : f(Variant<N, true, 0>{});
-    return with_acc ? f(Variant<N, false, FX_ACC>{}) : f(Variant<N, false, 0>{});
+    if (fx & FX_NLGC) return dispatch_acc<DPP, N, FX_NLGC>(ti, fx, f);
+    return dispatch_acc<DPP, N, 0>(ti, fx, f);
 }
 
 // Calls f(Variant<...>{}) for the runtime (ndim, time_interp, step features)

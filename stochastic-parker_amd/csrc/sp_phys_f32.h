@@ -214,6 +214,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
                                            bool fixed_dt, const float r[4], double d[4]) {
     constexpr bool ACC = (FX & FX_ACC) != 0;
     constexpr bool DPP = (FX & FX_DPP) != 0;
+    constexpr bool NLGC = (FX & FX_NLGC) != 0;
     const float bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
     const float by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
     const float bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
@@ -232,11 +233,21 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     const float knorm_para = knorm;
     if (m.flags & MF_MOMENTUM_DEP) knorm *= __powf((float)(s.p * m.inv_p0), m.f_pindex);
     const float kpara = m.f_kpara0 * knorm;
-    const float kperp = kpara * m.f_kret;
-    // kpara - kperp as a product of non-negative factors: a contracted
-    // kpara - kpara * kret can come out as a negative rounding error when
-    // kret = 1, and its square root as NaN
-    const float kpp = kpara * (1.0f - m.f_kret);
+    float kperp, kpp;
+    if constexpr (NLGC) {
+        // kpara0 * kperp_kpara * B^((gamma-2)/3) * (p/p0)^((5-gamma)/3) * mu^2
+        float kn_perp = mag ? __powf(b, m.f_perp_mag) : 1.0f;
+        if (m.flags & MF_MOMENTUM_DEP) kn_perp *= __powf((float)(s.p * m.inv_p0), m.f_perp_mom);
+        const float muf = (float)s.mu;
+        kperp = m.f_kpara0 * m.f_kperp_kpara * kn_perp * (muf * muf);
+        kpp = kpara - kperp;
+    } else {
+        kperp = kpara * m.f_kret;
+        // kpara - kperp as a product of non-negative factors: a contracted
+        // kpara - kpara * kret can come out as a negative rounding error when
+        // kret = 1, and its square root as NaN
+        kpp = kpara * (1.0f - m.f_kret);
+    }
     const float skpara = fsqrt(2.0f * kpara);
     const float skperp = fsqrt(2.0f * kperp);
     const float skpp = fsqrt(2.0f * kpp);
@@ -258,10 +269,20 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         float dkdx = 0.f, dkdy = 0.f;
         if (mag) { dkdx = db_dx * ibk * m.f_gm2; dkdy = db_dy * ibk * m.f_gm2; }
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
-        dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
-        dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
-        dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
-        dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        if constexpr (NLGC) {
+            // d ln kperp = d ln kpara / 3 along B's magnitude
+            const float ex = dkdx * (1.0f / 3.0f), ey = dkdy * (1.0f / 3.0f);
+            const float ax = (kpara * dkdx - kperp * ex) * ib2, ay = (kpara * dkdy - kperp * ey) * ib2;
+            dkxx_dx = kperp * ex + ax * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kperp * ey + ay * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+            dkxy_dx = ax * bx * by + kb2 * (dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = ay * bx * by + kb2 * (dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        } else {
+            dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+            dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        }
         const float ibp2 = ib * ib, ibp3 = ib * ibp2;
         const float vdx = vdp * (dbz_dy * ibp2 - 2.0f * bz * db_dy * ibp3);
         const float vdy = vdp * (-(dbz_dx * ibp2) + 2.0f * bz * db_dx * ibp3);
@@ -277,18 +298,37 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         const float dby_dx = f[R3_DBYDX], dby_dy = f[R3_DBYDY], dby_dz = f[R3_DBYDZ];
         const float dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
         const float db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
-        float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
-        if (mag) { dkdx = db_dx * m.f_gm2; dkdy = db_dy * m.f_gm2; dkdz = db_dz * m.f_gm2; }
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
-        dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
-        dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
-        dkzz_dz = kperp * dkdz + kb2 * dkdz * bz * bz + kb3 * bz * (dbz_dz * b - bz * db_dz);
-        dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
-        dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
-        dkxz_dx = kb2 * (dkdx * bx * bz + dbx_dx * bz + bx * dbz_dx) - kb3 * bx * bz * db_dx;
-        dkxz_dz = kb2 * (dkdz * bx * bz + dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
-        dkyz_dy = kb2 * (dkdy * by * bz + dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
-        dkyz_dz = kb2 * (dkdz * by * bz + dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
+        if constexpr (NLGC) {
+            // NLGC keeps 1/B in the 3D logarithmic gradients (:2708-2715)
+            float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
+            if (mag) { dkdx = db_dx * ibk * m.f_gm2; dkdy = db_dy * ibk * m.f_gm2; dkdz = db_dz * ibk * m.f_gm2; }
+            const float ex = dkdx * (1.0f / 3.0f), ey = dkdy * (1.0f / 3.0f), ez = dkdz * (1.0f / 3.0f);
+            const float ax = (kpara * dkdx - kperp * ex) * ib2;
+            const float ay = (kpara * dkdy - kperp * ey) * ib2;
+            const float az = (kpara * dkdz - kperp * ez) * ib2;
+            dkxx_dx = kperp * ex + ax * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kperp * ey + ay * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+            dkzz_dz = kperp * ez + az * bz * bz + kb3 * bz * (dbz_dz * b - bz * db_dz);
+            dkxy_dx = ax * bx * by + kb2 * (dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = ay * bx * by + kb2 * (dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+            dkxz_dx = ax * bx * bz + kb2 * (dbx_dx * bz + bx * dbz_dx) - kb3 * bx * bz * db_dx;
+            dkxz_dz = az * bx * bz + kb2 * (dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
+            dkyz_dy = ay * by * bz + kb2 * (dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
+            dkyz_dz = az * by * bz + kb2 * (dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
+        } else {
+            float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
+            if (mag) { dkdx = db_dx * m.f_gm2; dkdy = db_dy * m.f_gm2; dkdz = db_dz * m.f_gm2; }
+            dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
+            dkzz_dz = kperp * dkdz + kb2 * dkdz * bz * bz + kb3 * bz * (dbz_dz * b - bz * db_dz);
+            dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+            dkxz_dx = kb2 * (dkdx * bx * bz + dbx_dx * bz + bx * dbz_dx) - kb3 * bx * bz * db_dx;
+            dkxz_dz = kb2 * (dkdz * bx * bz + dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
+            dkyz_dy = kb2 * (dkdy * by * bz + dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
+            dkyz_dz = kb2 * (dkdz * by * bz + dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
+        }
         bxn = bx * ib; byn = by * ib; bzn = bz * ib;
         const float bxy2 = bxn * bxn + byn * byn;
         bxyn = fsqrt(bxy2);

@@ -187,6 +187,93 @@ __device__ __forceinline__ Kappa kappa(const MoverParams& m, double p, const dou
     return k;
 }
 
+// calc_spatial_diffusion_coefficients_nlgc (particle_module.f90:2464-2771),
+// Cartesian Parker transport without delta-B / correlation-length maps: kperp
+// from the NLGC scaling, kpara0 * kperp_kpara * B^((gamma-2)/3) *
+// (p/p0)^((5-gamma)/3) * mu^2, and separate logarithmic gradients of kpara and
+// kperp in the divergence terms (both with 1/B, in 2D and 3D).
+template <int NDIM>
+__device__ __forceinline__ Kappa kappa_nlgc(const MoverParams& m, double p, double mu,
+                                            const double* f) {
+    Kappa k;
+    const double bx = NDIM == 2 ? f[R2_BX] : f[R3_BX];
+    const double by = NDIM == 2 ? f[R2_BY] : f[R3_BY];
+    const double bz = NDIM == 2 ? f[R2_BZ] : f[R3_BZ];
+    const double b = sqrt(bx * bx + by * by + bz * bz);
+    double ib1;
+    if (b < __DBL_EPSILON__) ib1 = 1.0; else ib1 = 1.0 / b;
+    const double ib2 = ib1 * ib1;
+    const double ib3 = ib1 * ib2;
+    const bool mag = (m.flags & MF_MAG_DEP) != 0;
+    double knorm_para = 1.0, knorm_perp = 1.0;
+    if (mag) {
+        knorm_para = knorm_para * pow(b, m.gamma_m2);
+        knorm_perp = knorm_perp * pow(b, m.perp_mag);
+    }
+    double kn_para, kn_perp;
+    if (m.flags & MF_MOMENTUM_DEP) {
+        kn_para = knorm_para * pow(p / m.p0, m.pindex);
+        kn_perp = knorm_perp * pow(p / m.p0, m.perp_mom);
+    } else {
+        kn_para = knorm_para;
+        kn_perp = knorm_perp;
+    }
+    k.knorm_para = knorm_para;
+    k.kpara = m.kpara0 * kn_para;
+    k.kperp = m.kpara0 * m.kperp_kpara * kn_perp * (mu * mu);
+    k.skpara = sqrt(2.0 * k.kpara);
+    k.skperp = sqrt(2.0 * k.kperp);
+    k.skpara_perp = sqrt(2.0 * (k.kpara - k.kperp));
+    const double kpara = k.kpara, kperp = k.kperp;
+    const double kpp = kpara - kperp;
+    if (NDIM == 2) {
+        const double dbx_dx = f[R2_DBXDX], dbx_dy = f[R2_DBXDY];
+        const double dby_dx = f[R2_DBYDX], dby_dy = f[R2_DBYDY];
+        const double db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
+        double dpa_x = 0.0, dpa_y = 0.0, dpe_x = 0.0, dpe_y = 0.0;
+        if (mag) {
+            dpa_x = db_dx * ib1 * m.gamma_m2;
+            dpa_y = db_dy * ib1 * m.gamma_m2;
+            dpe_x = db_dx * ib1 * m.gamma_m2 / 3.0;
+            dpe_y = db_dy * ib1 * m.gamma_m2 / 3.0;
+        }
+        const double ax = kpara * dpa_x - kperp * dpe_x;
+        const double ay = kpara * dpa_y - kperp * dpe_y;
+        k.dkxx_dx = kperp * dpe_x + ax * (bx * bx) * ib2 + 2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+        k.dkyy_dy = kperp * dpe_y + ay * (by * by) * ib2 + 2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+        k.dkxy_dx = ax * bx * by * ib2 + kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+        k.dkxy_dy = ay * bx * by * ib2 + kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+        k.dkzz_dz = k.dkxz_dx = k.dkxz_dz = k.dkyz_dy = k.dkyz_dz = 0.0;
+    } else {
+        const double dbx_dx = f[R3_DBXDX], dbx_dy = f[R3_DBXDY], dbx_dz = f[R3_DBXDZ];
+        const double dby_dx = f[R3_DBYDX], dby_dy = f[R3_DBYDY], dby_dz = f[R3_DBYDZ];
+        const double dbz_dx = f[R3_DBZDX], dbz_dy = f[R3_DBZDY], dbz_dz = f[R3_DBZDZ];
+        const double db_dx = f[R3_DBDX], db_dy = f[R3_DBDY], db_dz = f[R3_DBDZ];
+        double dpa_x = 0.0, dpa_y = 0.0, dpa_z = 0.0, dpe_x = 0.0, dpe_y = 0.0, dpe_z = 0.0;
+        if (mag) {
+            dpa_x = db_dx * ib1 * m.gamma_m2;
+            dpa_y = db_dy * ib1 * m.gamma_m2;
+            dpa_z = db_dz * ib1 * m.gamma_m2;
+            dpe_x = db_dx * ib1 * m.gamma_m2 / 3.0;
+            dpe_y = db_dy * ib1 * m.gamma_m2 / 3.0;
+            dpe_z = db_dz * ib1 * m.gamma_m2 / 3.0;
+        }
+        const double ax = kpara * dpa_x - kperp * dpe_x;
+        const double ay = kpara * dpa_y - kperp * dpe_y;
+        const double az = kpara * dpa_z - kperp * dpe_z;
+        k.dkxx_dx = kperp * dpe_x + ax * (bx * bx) * ib2 + 2.0 * kpp * bx * (dbx_dx * b - bx * db_dx) * ib3;
+        k.dkyy_dy = kperp * dpe_y + ay * (by * by) * ib2 + 2.0 * kpp * by * (dby_dy * b - by * db_dy) * ib3;
+        k.dkzz_dz = kperp * dpe_z + az * (bz * bz) * ib2 + 2.0 * kpp * bz * (dbz_dz * b - bz * db_dz) * ib3;
+        k.dkxy_dx = ax * bx * by * ib2 + kpp * ((dbx_dx * by + bx * dby_dx) * ib2 - 2.0 * bx * by * db_dx * ib3);
+        k.dkxy_dy = ay * bx * by * ib2 + kpp * ((dbx_dy * by + bx * dby_dy) * ib2 - 2.0 * bx * by * db_dy * ib3);
+        k.dkxz_dx = ax * bx * bz * ib2 + kpp * ((dbx_dx * bz + bx * dbz_dx) * ib2 - 2.0 * bx * bz * db_dx * ib3);
+        k.dkxz_dz = az * bx * bz * ib2 + kpp * ((dbx_dz * bz + bx * dbz_dz) * ib2 - 2.0 * bx * bz * db_dz * ib3);
+        k.dkyz_dy = ay * by * bz * ib2 + kpp * ((dby_dy * bz + by * dbz_dy) * ib2 - 2.0 * by * bz * db_dy * ib3);
+        k.dkyz_dz = az * by * bz * ib2 + kpp * ((dby_dz * bz + by * dbz_dz) * ib2 - 2.0 * by * bz * db_dz * ib3);
+    }
+    return k;
+}
+
 // particle_in_acceleration_region (particle_module.f90:2885-2906)
 template <int NDIM>
 __device__ __forceinline__ bool in_acc_region(const MoverParams& m, double x, double y, double z) {
@@ -394,7 +481,9 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const double rt = TINTERP ? (s.t - m.t0) / m.dtf : 0.0;
     gather<NDIM, TINTERP, DPP>(m, I, rt, f);
-    const Kappa k = kappa<NDIM>(m, s.p, f);
+    Kappa k;
+    if constexpr ((FX & FX_NLGC) != 0) k = kappa_nlgc<NDIM>(m, s.p, s.mu, f);
+    else k = kappa<NDIM>(m, s.p, f);
     push<NDIM, FX>(m, s, f, k, fixed_dt, r, d);
 }
 

@@ -19,7 +19,7 @@ module gpat_hip
         init_tstamps_mhd, calc_tstamps_mhd, free_tstamps_mhd, &
         read_particle_boundary_conditions, read_particle_params, read_diagnostics_params, &
         set_drift_parameters, set_flag_check_drift_2d, set_precision, set_sort_steps, &
-        set_dpp_params, &
+        set_dpp_params, set_nlgc, &
         init_particles, free_particles, init_field_data, free_field_data, &
         read_field_data_parallel, calc_fields_gradients, copy_fields, &
         inject_particles_spatial_uniform, particle_mover, split_particle, &
@@ -281,6 +281,16 @@ contains
         cfg%tau0 = tau0_scattering
     end subroutine set_dpp_params
 
+    !< The kperp model of particle_mover (particle_module.f90:1846-1851): NLGC
+    !< with kperp/kpara = kperp_kpara at p0, or the standard kret ratio.  The
+    !< context is created with it; particle_mover checks that its arguments agree.
+    subroutine set_nlgc(nlgc, kperp_kpara)
+        logical, intent(in) :: nlgc
+        real(dp), intent(in) :: kperp_kpara
+        cfg%nlgc = merge(1, 0, nlgc)
+        cfg%kperp_kpara = kperp_kpara
+    end subroutine set_nlgc
+
     !< spatial sort cadence inside one mover call (steps per particle; 0: once)
     subroutine set_sort_steps(nsteps)
         integer, intent(in) :: nsteps
@@ -302,7 +312,6 @@ contains
         cfg%stream = 0
         cfg%sort_every = 1
         cfg%reserved0 = 0
-        cfg%reserved1 = 0
         call check_status(sp_ctx_create(cfg, ctx), "sp_ctx_create")
         nptl_current = 0
         leak = 0.0_dp
@@ -379,7 +388,8 @@ contains
         type(sp_mover_stats) :: st
         real(dp) :: t0, dtf
         if (focused_transport) call unsupported("focused transport (-ft .true.)")
-        if (nlgc) call unsupported("NLGC perpendicular diffusion (-nl .true.)")
+        if (merge(1, 0, nlgc) /= cfg%nlgc .or. (nlgc .and. kperp_kpara /= cfg%kperp_kpara)) &
+            call unsupported("changing -nl / -kk after init_particles (set_nlgc)")
         t0 = tstamps_mhd(mhd_tframe)
         dtf = tstamps_mhd(mhd_tframe + 1) - t0
         call check_status(sp_particle_mover(ctx, t0, dtf, int(num_fine_steps, c_int32_t), &

@@ -44,8 +44,8 @@ module sp_hip_binding
         integer(c_int64_t) :: seed
         integer(c_int32_t) :: stream, sort_every, sort_steps, reserved0
         integer(c_int64_t) :: nptl_max
-        integer(c_int32_t) :: dpp_wave, dpp_shear, weak_scattering, reserved1
-        real(c_double) :: tau0
+        integer(c_int32_t) :: dpp_wave, dpp_shear, weak_scattering, nlgc
+        real(c_double) :: tau0, kperp_kpara
     end type sp_config
 
     type, bind(C) :: sp_particles

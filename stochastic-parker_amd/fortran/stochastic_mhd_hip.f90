@@ -59,6 +59,7 @@ program stochastic_hip
     call set_drift_parameters(drift_param1, drift_param2, charge)
     if (ndim_field == 2 .and. check_drift_2d == 1) call set_flag_check_drift_2d(check_drift_2d)
     call set_dpp_params(dpp_wave, dpp_shear, weak_scattering, tau0_scattering)
+    call set_nlgc(nlgc, kperp_kpara)
     select case (trim(gpu_precision))
     case ('f32'); call set_precision(1, rng_seed)
     case ('f32h'); call set_precision(2, rng_seed)
@@ -127,7 +128,6 @@ contains
     subroutine check_scope
         if (restart_flag) call stop_scope("-rf (restart)")
         if (focused_transport) call stop_scope("-ft .true. (focused transport)")
-        if (nlgc) call stop_scope("-nl .true. (NLGC); pass -nl .false.")
         if (uniform_grid /= 1) call stop_scope("-ug 0 (non-uniform grid); pass -ug 1")
         if (spherical_coord /= 0) call stop_scope("-sc 1 (spherical coordinates)")
         if (trim(gpu_precision) == 'f32h' .and. (dpp_wave /= 0 .or. dpp_shear /= 0)) &

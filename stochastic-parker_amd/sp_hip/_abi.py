@@ -52,7 +52,7 @@ class SpConfig(C.Structure):
         ("sort_steps", C.c_int32), ("reserved0", C.c_int32),
         ("nptl_max", C.c_int64),
         ("dpp_wave", C.c_int32), ("dpp_shear", C.c_int32), ("weak_scattering", C.c_int32),
-        ("reserved1", C.c_int32), ("tau0", C.c_double),
+        ("nlgc", C.c_int32), ("tau0", C.c_double), ("kperp_kpara", C.c_double),
     ]
 
 

@@ -37,7 +37,8 @@ CONFIG_DEFAULTS = dict(
     dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
     acc_region_flag=0, acc_region=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), time_interp=0,
     precision=_abi.SP_PREC_F64, seed=20261202, stream=0, sort_every=1, sort_steps=0, reserved0=0,
-    nptl_max=1000, dpp_wave=0, dpp_shear=0, weak_scattering=0, reserved1=0, tau0=0.0,
+    nptl_max=1000, dpp_wave=0, dpp_shear=0, weak_scattering=0, nlgc=0, tau0=0.0,
+    kperp_kpara=0.01,
 )
 
 

@@ -39,7 +39,7 @@ def test_abi_version():
 
 
 def test_struct_sizes_match_header_layout():
-    assert C.sizeof(_abi.SpConfig) == 344
+    assert C.sizeof(_abi.SpConfig) == 352
     assert C.sizeof(_abi.SpVariant) == 32
     assert C.sizeof(_abi.SpTuneResult) == 32
     assert C.sizeof(_abi.SpMoverStats) == 56

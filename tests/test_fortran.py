@@ -25,12 +25,12 @@ def test_bind_c_layout_matches_header():
     _need(os.path.join(BIN, "abi_check"))
     out = subprocess.run([os.path.join(BIN, "abi_check")], capture_output=True, text=True,
                          check=True).stdout.split()
-    assert [int(v) for v in out] == [344, 56, 40, 176, 136, 32, 32]
+    assert [int(v) for v in out] == [352, 56, 40, 176, 136, 32, 32]
 
 
 @pytest.mark.parametrize("args,msg", [
-    (["-dm", "x/"], "NLGC"),                       # the reference default -nl .true.
-    (["-dm", "x/", "-nl", ".false."], "non-uniform"),
+    (["-dm", "x/"], "non-uniform"),                # the reference default -ug 0
+    (["-dm", "x/", "-ug", "1", "-dw", "1", "-gp", "f32h"], "f32h"),
     (["-dm", "x/", "-nl", ".false.", "-ug", "1", "-ft", ".true."], "focused transport"),
     (["-dm", "x/", "-nl", ".false.", "-ug", "1", "-is", ".true."], "shock"),
     (["-nl", ".false."], "dir_mhd_data"),

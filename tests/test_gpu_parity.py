@@ -104,7 +104,7 @@ def test_injection_matches_oracle(dist_flag):
 
 
 def _mover_case(c, nodes, n, dtf, t0=0.0, nodes2=None, mixed=False, intervals=1, split=None,
-                nfine=1):
+                nfine=1, prep=None):
     pz, _ = make(c, nodes, nptl_max=4 * n, nodes2=nodes2)
     prm = oracle.params(c)
     fa = oracle.gradients(prm, nodes)
@@ -122,6 +122,12 @@ def _mover_case(c, nodes, n, dtf, t0=0.0, nodes2=None, mixed=False, intervals=1,
         ref["count_flag"] = flags
         g["p"] = np.where(neg, -0.5, g["p"])
         ref["p"] = np.where(neg, -0.5, ref["p"])
+        pz.set_particles(g)
+    if prep is not None:
+        # the same change to both particle sets before the run
+        g = pz.get_particles()
+        prep(g)
+        prep(ref)
         pz.set_particles(g)
     steps_ref = 0
     leak_ref = negp_ref = 0.0
