@@ -184,6 +184,32 @@ int sp_particles_counts(void* ctx, int64_t* nptl_current, int64_t* nptl_split,
 int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int32_t dist_flag,
                                         double particle_v0, double t0, double dtf,
                                         const double part_box[6], double power_index);
+/* Quantities of the threshold injections */
+#define SP_INJECT_JZ 1      /* |dBy/dx - dBx/dy|  (inject_particles_at_large_jz) */
+#define SP_INJECT_ABSJ 2    /* |curl B|           (inject_particles_at_large_absj) */
+#define SP_INJECT_DB2 3     /* not supported: needs the turbulence-amplitude map */
+#define SP_INJECT_DIVV 4    /* -div V             (inject_particles_at_large_divv) */
+#define SP_INJECT_RHO 5     /* rho                (inject_particles_at_large_rho) */
+/* get_ncells_large_jz / _absj / _divv / _rho (mhd_data_parallel.f90:2211-2499):
+ * interior nodes of frame 0 strictly inside part_box (axes beyond ndim
+ * untested) whose quantity exceeds threshold, in the reference's precision
+ * (|jz|, |J| in FP32).  The divV count reads each node's value two nodes
+ * below along every ghosted axis, as the reference's does. */
+int sp_ncells_above_threshold(void* ctx, int32_t quantity, double threshold,
+                              const double part_box[6], int64_t* ncells);
+/* inject_particles_at_large_jz / _absj / _divv / _rho (particle_module.f90:
+ * 785-1470), one rank: nptl_inject = int(nptl * ncells / ncells_norm), or nptl
+ * with inject_same_nptl (none when no cell passes).  Each particle draws
+ * positions over the domain until one lies in part_box and its quantity
+ * (frame 0, interpolated) reaches threshold; then mu and the momentum law as
+ * sp_inject_particles_spatial_uniform.  A particle needing more than 2^24 draws
+ * fails the call (SP_ERR_ARG) before anything is stored.  *nptl_injected
+ * (may be NULL) receives nptl_inject. */
+int sp_inject_particles_threshold(void* ctx, int32_t quantity, int64_t nptl, double dt,
+                                  int32_t dist_flag, double particle_v0, double t0, double dtf,
+                                  int32_t inject_same_nptl, double threshold,
+                                  int64_t ncells_norm, const double part_box[6],
+                                  double power_index, int64_t* nptl_injected);
 /* particle_mover (particle_module.f90:1846-1974) for Parker transport with the
  * standard kappa (nlgc = .false.), one rank: push every particle to t0 + dtf,
  * apply the exact-bounds boundary pass and remove non-INBOX particles. */

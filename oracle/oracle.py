@@ -114,6 +114,12 @@ def load():
                                                         C.c_double, C.c_double, _D, C.c_double,
                                                         C.POINTER(C.c_int32)]),
         "orc_split_particle": (None, [P, _PP, _I64, C.c_int64, C.c_double, C.c_double, _I64]),
+        "orc_ncells_above_threshold": (C.c_int64, [P, _F, C.c_int, C.c_double, _D]),
+        "orc_inject_particles_threshold": (C.c_int64, [P, _F, _PP, _I64, C.c_int64, C.c_int,
+                                                       C.c_int64, C.c_double, C.c_int, C.c_double,
+                                                       C.c_double, C.c_double, C.c_int, C.c_double,
+                                                       C.c_int64, _D, C.c_double,
+                                                       C.POINTER(C.c_int32)]),
         "orc_local_dist_size": (C.c_int64, [P, C.POINTER(OrcDistConfig), C.c_int]),
         "orc_calc_particle_distributions": (None, [P, C.POINTER(OrcDistConfig), _PP, C.c_int64,
                                                    _D, C.POINTER(_D)]),
@@ -240,6 +246,33 @@ def inject(prm, n_existing_arr, nptl_max, nptl, dt, dist_flag, v0, t0, dtf, box,
                                              nptl_max, nptl, dt, dist_flag, v0, t0, dtf, dptr(b),
                                              power_index, C.byref(tm))
     return buf[: n.value].copy(), tm.value
+
+
+THRESHOLD = {"jz": 1, "absj": 2, "divv": 4, "rho": 5}
+
+
+def ncells_above_threshold(prm, fa1, quantity, threshold, box):
+    lib = load()
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    return lib.orc_ncells_above_threshold(C.byref(prm), fptr(fa1), THRESHOLD[quantity], threshold,
+                                          dptr(b))
+
+
+def inject_threshold(prm, fa1, existing, nptl_max, quantity, nptl, dt, dist_flag, v0, t0, dtf,
+                     same, threshold, ncells_norm, box, power_index, tag_max=0):
+    """Returns (particles, tag_max, number injected)."""
+    lib = load()
+    buf = np.zeros(nptl_max, dtype=PARTICLE_DTYPE)
+    k = len(existing)
+    buf[:k] = existing
+    n = C.c_int64(k)
+    tm = C.c_int32(tag_max)
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    ninj = lib.orc_inject_particles_threshold(C.byref(prm), fptr(fa1), buf.ctypes.data_as(_PP),
+                                              C.byref(n), nptl_max, THRESHOLD[quantity], nptl, dt,
+                                              dist_flag, v0, t0, dtf, int(same), threshold,
+                                              ncells_norm, dptr(b), power_index, C.byref(tm))
+    return buf[: n.value].copy(), tm.value, ninj
 
 
 def split(prm, ptls, nptl_max, split_ratio, pmin_split):

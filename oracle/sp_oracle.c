@@ -954,6 +954,11 @@ void orc_remove_particles(orc_particle* ptls, int64_t* nptl_current, int64_t* np
     *nptl_current = n - nremoved;
 }
 
+static void inject_one_particle(const orc_params* prm, orc_particle* q, orc_drawstream* ds,
+                                double xt, double yt, double zt, double mu, int dist_flag,
+                                double particle_v0, double t0, double dtf, double dt,
+                                double power_index, int32_t* tag_max);
+
 /* inject_particles_spatial_uniform + inject_one_particle,
  * particle_module.f90:385-530 (whole domain on one rank).  Draw order per
  * particle: x, y, z, mu, [momentum draws], t, from the particle's own Philox
@@ -978,6 +983,20 @@ void orc_inject_particles_spatial_uniform(const orc_params* prm, orc_particle* p
         const double yt = ds_next(&ds) * (part_box[4] - part_box[1]) + part_box[1];
         const double zt = ds_next(&ds) * (part_box[5] - part_box[2]) + part_box[2];
         const double mu = mu_max * (2.0 * ds_next(&ds) - 1.0);
+        inject_one_particle(prm, q, &ds, xt, yt, zt, mu, dist_flag, particle_v0, t0, dtf, dt,
+                            power_index, tag_max);
+    }
+}
+
+/* inject_one_particle, particle_module.f90:385-441: momentum by the
+ * distribution law, then the time stamp, from the particle's draw stream. */
+static void inject_one_particle(const orc_params* prm, orc_particle* q, orc_drawstream* ds_,
+                                double xt, double yt, double zt, double mu, int dist_flag,
+                                double particle_v0, double t0, double dtf, double dt,
+                                double power_index, int32_t* tag_max)
+{
+    orc_drawstream ds = *ds_;
+    {   /* draws continue the caller's stream */
         memset(q, 0, sizeof(*q));
         q->x = xt; q->y = yt; q->z = zt;
         if (dist_flag == 0) {
@@ -1014,6 +1033,158 @@ void orc_inject_particles_spatial_uniform(const orc_params* prm, orc_particle* p
         q->tag_splitted = 1;
         q->rng_step = 0u;
     }
+    *ds_ = ds;
+}
+
+/* The interior-node quantity of get_ncells_large_jz / _absj / _divv / _rho
+ * (mhd_data_parallel.f90:2211-2499), in the precision the Fortran evaluates it:
+ * |jz| and |J| in FP32 (farray1 is real(sp)), divV summed in FP64.  (gi, gj,
+ * gk) is the 0-based ghost-inclusive node. */
+static double node_quantity(const orc_params* prm, const float* fa, int quantity, int gi, int gj,
+                            int gk)
+{
+    const int nxg = nxg_of(prm), nyg = nyg_of(prm);
+    switch (quantity) {
+    case ORC_THR_JZ: {
+        const float d = FA(fa, 23, gi, gj, gk) - FA(fa, 21, gi, gj, gk);
+        return (double)fabsf(d);
+    }
+    case ORC_THR_ABSJ: {
+        const float a = FA(fa, 25, gi, gj, gk) - FA(fa, 27, gi, gj, gk);
+        const float b = FA(fa, 26, gi, gj, gk) - FA(fa, 22, gi, gj, gk);
+        const float c = FA(fa, 21, gi, gj, gk) - FA(fa, 23, gi, gj, gk);
+        return (double)sqrtf(a * a + b * b + c * c);
+    }
+    case ORC_THR_DIVV: {
+        double d = (double)FA(fa, 8, gi, gj, gk);
+        if (prm->ndim > 1) {
+            d = d + (double)FA(fa, 12, gi, gj, gk);
+            if (prm->ndim > 2) d = d + (double)FA(fa, 16, gi, gj, gk);
+        }
+        return -d;
+    }
+    default:
+        return (double)FA(fa, 3, gi, gj, gk);
+    }
+}
+
+/* get_ncells_large_jz / _absj / _divv / _rho, mhd_data_parallel.f90:2211-2499,
+ * uniform Cartesian grid, one rank: interior nodes strictly inside part_box
+ * (axes beyond ndim not tested) whose quantity exceeds the threshold (-divV for
+ * divV).  The divV count reads node (i-2, j-2, k-2): the routine assigns the
+ * whole ghosted farray1 slice to an array allocated without ghosts, which
+ * Fortran's reallocation on assignment rebounds to start at 1 (:2423-2429);
+ * in 2D the z extent has no ghosts and no shift. */
+int64_t orc_ncells_above_threshold(const orc_params* prm, const float* farray1, int quantity,
+                                   double threshold, const double part_box[6])
+{
+    const int nz = prm->ndim > 2 ? prm->nz : 1, ny = prm->ndim > 1 ? prm->ny : 1;
+    int64_t n = 0;
+    for (int fk = 1; fk <= nz; ++fk) {
+        const double zp = prm->zmin + (fk - 1) * prm->dz;
+        const int inz = prm->ndim > 2 ? (zp > part_box[2] && zp < part_box[5]) : 1;
+        for (int fj = 1; fj <= ny; ++fj) {
+            const double yp = prm->ymin + (fj - 1) * prm->dy;
+            const int iny = prm->ndim > 1 ? (yp > part_box[1] && yp < part_box[4]) : 1;
+            for (int fi = 1; fi <= prm->nx; ++fi) {
+                const double xp = prm->xmin + (fi - 1) * prm->dx;
+                const int inx = xp > part_box[0] && xp < part_box[3];
+                if (!(inx && iny && inz)) continue;
+                int gi = fi + 1, gj = prm->ndim > 1 ? fj + 1 : 0, gk = prm->ndim > 2 ? fk + 1 : 0;
+                if (quantity == ORC_THR_DIVV) {
+                    gi -= 2;
+                    if (prm->ndim > 1) gj -= 2;
+                    if (prm->ndim > 2) gk -= 2;
+                }
+                if (node_quantity(prm, farray1, quantity, gi, gj, gk) > threshold) n++;
+            }
+        }
+    }
+    return n;
+}
+
+/* The quantity the threshold injections test at a trial position, from the
+ * interpolated fields (particle_module.f90:883-890 jz, :1035-1043 |J|,
+ * :1318-1323 divV, :1456 rho), in FP64. */
+static double point_quantity(const orc_params* prm, int quantity, const double f[32])
+{
+    switch (quantity) {
+    case ORC_THR_JZ:
+        return fabs(f[23] - f[21]);
+    case ORC_THR_ABSJ: {
+        const double a = f[25] - f[27], b = f[26] - f[22], c = f[21] - f[23];
+        return sqrt(a * a + b * b + c * c);
+    }
+    case ORC_THR_DIVV: {
+        double d = f[8];
+        if (prm->ndim > 1) {
+            d = d + f[12];
+            if (prm->ndim > 2) d = d + f[16];
+        }
+        return d;
+    }
+    default:
+        return f[3];
+    }
+}
+
+/* inject_particles_at_large_jz / _absj / _divv / _rho,
+ * particle_module.f90:785-1470, uniform Cartesian grid, one rank.  The count
+ * is int(nptl * ncells / ncells_g) with inject_same_nptl (one rank: nptl when
+ * some cell passes), else int(nptl * ncells / ncells_norm); no cell: none.
+ * Each particle draws x, y, z over the domain until the point lies in
+ * part_box (inclusive) and its interpolated quantity (frame 1) passes; a
+ * point outside the box scores -3 (jz, |J|), divV = 3 or rho = 0.  The loop
+ * tests its initial value first (jz, |J|: -2, divV: 2, rho: 0): when that
+ * already passes no point is drawn and the particle sits at the box's low
+ * corner.  Then mu, then inject_one_particle.  Returns the number injected. */
+int64_t orc_inject_particles_threshold(const orc_params* prm, const float* farray1,
+                                       orc_particle* ptls, int64_t* nptl_current,
+                                       int64_t nptl_max, int quantity, int64_t nptl, double dt,
+                                       int dist_flag, double particle_v0, double t0, double dtf,
+                                       int inject_same_nptl, double threshold,
+                                       int64_t ncells_norm, const double part_box[6],
+                                       double power_index, int32_t* tag_max)
+{
+    const double mu_max = (double)0.99f;
+    const int64_t ncells = orc_ncells_above_threshold(prm, farray1, quantity, threshold, part_box);
+    if (ncells == 0) return 0;
+    int64_t nptl_inject;
+    if (inject_same_nptl) nptl_inject = (int64_t)((double)nptl * ((double)ncells / (double)ncells));
+    else nptl_inject = (int64_t)((double)nptl * ((double)ncells / (double)ncells_norm));
+    const int divv = quantity == ORC_THR_DIVV;
+    const double v_init = divv ? 2.0 : (quantity == ORC_THR_RHO ? 0.0 : -2.0);
+    const double v_out = divv ? 3.0 : (quantity == ORC_THR_RHO ? 0.0 : -3.0);
+    const double zlo = prm->zmin, zhi = prm->zmax;
+    for (int64_t i = 0; i < nptl_inject; ++i) {
+        int64_t slot = *nptl_current + 1;
+        if (slot > nptl_max) slot = nptl_max;
+        *nptl_current = slot;
+        orc_drawstream ds;
+        ds_init(&ds, prm, (uint32_t)*tag_max);
+        double xt = part_box[0], yt = part_box[1], zt = part_box[2];
+        double v = v_init;
+        while ((divv ? -v : v) < threshold) {
+            xt = ds_next(&ds) * (prm->xmax - prm->xmin) + prm->xmin;
+            yt = ds_next(&ds) * (prm->ymax - prm->ymin) + prm->ymin;
+            zt = ds_next(&ds) * (zhi - zlo) + zlo;
+            if (xt >= part_box[0] && xt <= part_box[3] && yt >= part_box[1] && yt <= part_box[4] &&
+                zt >= part_box[2] && zt <= part_box[5]) {
+                int pos[3];
+                double w[8], f[32];
+                orc_get_interp_paramters(prm, (xt - prm->xmin) / prm->dx, (yt - prm->ymin) / prm->dy,
+                                         (zt - prm->zmin) / prm->dz, pos, w);
+                orc_interp_fields(prm, farray1, farray1, pos, w, 0.0, f);
+                v = point_quantity(prm, quantity, f);
+            } else {
+                v = v_out;
+            }
+        }
+        const double mu = mu_max * (2.0 * ds_next(&ds) - 1.0);
+        inject_one_particle(prm, &ptls[slot - 1], &ds, xt, yt, zt, mu, dist_flag, particle_v0, t0,
+                            dtf, dt, power_index, tag_max);
+    }
+    return nptl_inject;
 }
 
 /* x**n for an integer n, by right-to-left binary powering (the method of

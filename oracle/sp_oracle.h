@@ -149,6 +149,17 @@ void orc_particle_mover(const orc_params* prm, const float* farray1, const float
 
 void orc_remove_particles(orc_particle* ptls, int64_t* nptl_current, int64_t* nptl_escaped);
 
+/* quantities of the threshold injections */
+enum { ORC_THR_JZ = 1, ORC_THR_ABSJ = 2, ORC_THR_DIVV = 4, ORC_THR_RHO = 5 };
+int64_t orc_ncells_above_threshold(const orc_params* prm, const float* farray1, int quantity,
+                                   double threshold, const double part_box[6]);
+int64_t orc_inject_particles_threshold(const orc_params* prm, const float* farray1,
+                                       orc_particle* ptls, int64_t* nptl_current,
+                                       int64_t nptl_max, int quantity, int64_t nptl, double dt,
+                                       int dist_flag, double particle_v0, double t0, double dtf,
+                                       int inject_same_nptl, double threshold,
+                                       int64_t ncells_norm, const double part_box[6],
+                                       double power_index, int32_t* tag_max);
 void orc_inject_particles_spatial_uniform(const orc_params* prm, orc_particle* ptls,
                                           int64_t* nptl_current, int64_t nptl_max,
                                           int64_t nptl, double dt, int dist_flag,

@@ -629,6 +629,105 @@ int sp_inject_particles_spatial_uniform(void* ctx, int64_t nptl, double dt, int3
     return SP_OK;
 }
 
+sp::ThreshArgs thresh_args(Ctx* c, int quantity, double threshold, const double part_box[6]) {
+    sp::ThreshArgs ta{};
+    ta.quantity = quantity;
+    ta.threshold = threshold;
+    for (int i = 0; i < 6; ++i) ta.box[i] = part_box[i];
+    ta.idxh = 0.5 / c->cfg.dx;
+    ta.idyh = 0.5 / c->cfg.dy;
+    ta.idzh = 0.5 / c->cfg.dz;
+    ta.n8 = c->n8[0];
+    ta.max_trials = 1ll << 24;
+    return ta;
+}
+
+int check_threshold_call(Ctx* c, int quantity, const double* part_box) {
+    if (quantity == SP_INJECT_DB2)
+        return fail(SP_ERR_UNSUPPORTED, "db2 injection needs the turbulence-amplitude map");
+    if (quantity != SP_INJECT_JZ && quantity != SP_INJECT_ABSJ && quantity != SP_INJECT_DIVV &&
+        quantity != SP_INJECT_RHO)
+        return fail(SP_ERR_ARG, "quantity must be SP_INJECT_JZ, _ABSJ, _DIVV or _RHO");
+    if (!part_box) return fail(SP_ERR_ARG, "part_box is NULL");
+    if (!c->have_fields[0] || !c->n8[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not uploaded");
+    return SP_OK;
+}
+
+long long ncells_above(Ctx* c, const sp::ThreshArgs& ta, int& rc) {
+    rc = SP_OK;
+    unsigned long long* d = c->totals;
+    unsigned long long h = 0;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(h), c->st);
+    if (e == hipSuccess) e = sp::launch_threshold_count(c->dp, ta, d, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) rc = fail(SP_ERR_HIP, std::string("cell count failed: ") + hipGetErrorString(e));
+    return (long long)h;
+}
+
+int sp_ncells_above_threshold(void* ctx, int32_t quantity, double threshold, const double part_box[6],
+                              int64_t* ncells) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if ((rc = check_threshold_call(c, quantity, part_box)) != SP_OK) return rc;
+    if (!ncells) return fail(SP_ERR_ARG, "ncells is NULL");
+    const long long n = ncells_above(c, thresh_args(c, quantity, threshold, part_box), rc);
+    if (rc != SP_OK) return rc;
+    *ncells = n;
+    return SP_OK;
+}
+
+int sp_inject_particles_threshold(void* ctx, int32_t quantity, int64_t nptl, double dt,
+                                  int32_t dist_flag, double particle_v0, double t0, double dtf,
+                                  int32_t inject_same_nptl, double threshold, int64_t ncells_norm,
+                                  const double part_box[6], double power_index,
+                                  int64_t* nptl_injected) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if ((rc = check_threshold_call(c, quantity, part_box)) != SP_OK) return rc;
+    if (nptl < 0) return fail(SP_ERR_ARG, "nptl must be >= 0");
+    if (dist_flag < 0 || dist_flag > 2) return fail(SP_ERR_ARG, "dist_flag must be 0, 1 or 2");
+    if (!inject_same_nptl && ncells_norm < 1) return fail(SP_ERR_ARG, "ncells_norm must be >= 1");
+    sp::ThreshArgs ta = thresh_args(c, quantity, threshold, part_box);
+    const long long ncells = ncells_above(c, ta, rc);
+    if (rc != SP_OK) return rc;
+    // nptl_inject = int(nptl * mpi_sub_size * (dble(ncells) / dble(ncells_g | norm)))
+    long long ninj = 0;
+    if (ncells > 0) {
+        const double ratio = inject_same_nptl ? (double)ncells / (double)ncells
+                                              : (double)ncells / (double)ncells_norm;
+        const double v = (double)nptl * ratio;
+        if (!(v < 9.0e18)) return fail(SP_ERR_ARG, "nptl_inject overflows");
+        ninj = (long long)v;
+    }
+    if (nptl_injected) *nptl_injected = ninj;
+    if (ninj == 0) return SP_OK;
+    if ((long long)c->tag_max + ninj > 2147483647ll) return fail(SP_ERR_CAPACITY, "tag_max would overflow int32");
+    sp::InjectArgs ia{};
+    for (int i = 0; i < 6; ++i) ia.box[i] = part_box[i];
+    ia.dt = dt; ia.v0 = particle_v0; ia.t0 = t0; ia.dtf = dtf; ia.power_index = power_index;
+    ia.dist_flag = dist_flag;
+    unsigned long long* d = c->totals;
+    // pass 1 only draws; the particles are stored once every one has a position
+    for (int commit = 0; commit < 2; ++commit) {
+        ta.commit = commit;
+        unsigned long long fails = 0;
+        SP_HIP(hipMemsetAsync(d, 0, sizeof(fails), c->st));
+        SP_HIP(sp::launch_threshold_inject(c->dp, c->P[c->cur], c->nptl, ninj, c->nmax, c->tag_max, ia,
+                                           ta, d, c->st));
+        SP_HIP(hipMemcpyAsync(&fails, d, sizeof(fails), hipMemcpyDeviceToHost, c->st));
+        SP_HIP(hipStreamSynchronize(c->st));
+        if (fails)
+            return fail(SP_ERR_ARG, std::to_string(fails) + " particles found no point above the threshold in " +
+                                    std::to_string(ta.max_trials) + " draws");
+    }
+    c->nptl = std::min<long long>(c->nptl + ninj, c->nmax);
+    c->tag_max += (int)ninj;
+    return SP_OK;
+}
+
 // The push of one interval: launches of the persistent mover over the
 // current particles, with re-sorts between launches when a step budget is set.
 // bpc > 0 / sort_steps >= 0 override the launch settings (autotuner trials).

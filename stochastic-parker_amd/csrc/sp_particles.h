@@ -15,6 +15,18 @@ struct InjectArgs {
     int origin;                 // stored in each injected particle (variant index)
 };
 
+// Threshold injections (particle_module.f90:785-1470) and their cell
+// counters (mhd_data_parallel.f90:2211-2499).  quantity: SP_INJECT_*.
+struct ThreshArgs {
+    int quantity;
+    double threshold;
+    double box[6];                // part_box: xmin, ymin, zmin, xmax, ymax, zmax
+    double idxh, idyh, idzh;      // 0.5 / dx ... of the gradient stencil
+    const float* n8;              // frame-1 node fields (8 per node)
+    long long max_trials;         // per particle, then the particle counts as failed
+    int commit;                   // 0: only count failures; 1: store the particles
+};
+
 struct SplitArgs {
     double split_ratio, pmin_split, p0, pmax;
 };
@@ -60,6 +72,11 @@ hipError_t launch_spatial_sort(const Ptl& S, const Ptl& D, long long n, const So
                                bool copy_back, hipStream_t s);
 hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, long long nptl,
                          long long nmax, int tag0, const InjectArgs& ia, hipStream_t s);
+hipError_t launch_threshold_count(const DevParams& prm, const ThreshArgs& ta,
+                                  unsigned long long* count, hipStream_t s);
+hipError_t launch_threshold_inject(const DevParams& prm, const Ptl& P, long long start,
+                                   long long nptl, long long nmax, int tag0, const InjectArgs& ia,
+                                   const ThreshArgs& ta, unsigned long long* fails, hipStream_t s);
 hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,
                              unsigned* tile_cnt, unsigned* tile_esc, const DistArgs* da,
                              const EscArgs* ea, hipStream_t s);

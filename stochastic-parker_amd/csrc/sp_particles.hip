@@ -8,6 +8,7 @@
 // so the outcome is deterministic; the reference's remove_particles swaps
 // removed particles to the tail instead, which changes only the array order.
 #include "sp_device.h"
+#include "sp_nodes.h"
 #include "sp_particles.h"
 
 #include <algorithm>
@@ -40,6 +41,10 @@ struct DrawStream {
     }
 };
 
+__device__ void inject_one(const DevParams& prm, const Ptl& P, long long slot, DrawStream& ds,
+                           double xt, double yt, double zt, double mu, const InjectArgs& ia,
+                           int tag);
+
 __global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, long long start,
                                                         long long nptl, long long nmax, int tag0,
                                                         InjectArgs ia) {
@@ -57,6 +62,14 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, lo
     const double yt = ds.next() * (ia.box[4] - ia.box[1]) + ia.box[1];
     const double zt = ds.next() * (ia.box[5] - ia.box[2]) + ia.box[2];
     const double mu = mu_max * (2.0 * ds.next() - 1.0);
+    inject_one(prm, P, slot, ds, xt, yt, zt, mu, ia, tag0 + (int)i);
+}
+
+// inject_one_particle (particle_module.f90:385-441): momentum by the
+// distribution law, then the time stamp, continuing the particle's draws.
+__device__ void inject_one(const DevParams& prm, const Ptl& P, long long slot, DrawStream& ds,
+                           double xt, double yt, double zt, double mu, const InjectArgs& ia,
+                           int tag) {
     double p = prm.p0;
     if (ia.dist_flag == 0) {
         double ftest = 1.0, fxp = 0.5, ptmp = 0.0;
@@ -94,9 +107,182 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, lo
     P.origin[slot] = ia.origin;
     P.nsteps_tracked[slot] = 0;
     P.nsteps_pushed[slot] = 0;
-    P.tag_injected[slot] = tag0 + (int)i;
+    P.tag_injected[slot] = tag;
     P.tag_splitted[slot] = 1;
     P.rng_step[slot] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Threshold injections.  The node quantity of the cell counters, in the
+// precision the Fortran evaluates it (farray1 is real(sp)): |jz| and |J| in
+// FP32, divV summed in FP64 (mhd_data_parallel.f90:2236-2238, :2306-2312,
+// :2423-2429, :2488).
+__device__ double node_quantity(const DevParams& prm, const ThreshArgs& ta, long long node) {
+    float f[32];
+    node_components(prm, ta.idxh, ta.idyh, ta.idzh, ta.n8, node, f);
+    switch (ta.quantity) {
+    case SP_INJECT_JZ:
+        return (double)fabsf(f[23] - f[21]);
+    case SP_INJECT_ABSJ: {
+        const float a = f[25] - f[27], b = f[26] - f[22], c = f[21] - f[23];
+        return (double)sqrtf(a * a + b * b + c * c);
+    }
+    case SP_INJECT_DIVV: {
+        double d = (double)f[8];
+        if (prm.ndim > 1) {
+            d = d + (double)f[12];
+            if (prm.ndim > 2) d = d + (double)f[16];
+        }
+        return -d;
+    }
+    default:
+        return (double)f[3];
+    }
+}
+
+// get_ncells_large_* (mhd_data_parallel.f90:2211-2499): interior nodes
+// strictly inside the box whose quantity exceeds the threshold.  The divV
+// counter reads the node two below in each ghosted axis: that routine assigns
+// the whole ghosted farray1 slice to an array allocated without ghosts, and
+// reallocation on assignment rebounds it to start at 1 (:2423-2446).
+__global__ void __launch_bounds__(kBlock) threshold_count_kernel(DevParams prm, ThreshArgs ta,
+                                                                 unsigned long long* count) {
+    const int nx = prm.nxg - 4;
+    const int ny = prm.ndim > 1 ? prm.nyg - 4 : 1;
+    const int nz = prm.ndim > 2 ? prm.nzg - 4 : 1;
+    const long long total = (long long)nx * ny * nz;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned c = 0;
+    if (t < total) {
+        const int fi = (int)(t % nx) + 1;
+        const int fj = (int)((t / nx) % ny) + 1;
+        const int fk = (int)(t / ((long long)nx * ny)) + 1;
+        const double xp = prm.xmin + (fi - 1) * prm.dx;
+        const double yp = prm.ymin + (fj - 1) * prm.dy;
+        const double zp = prm.zmin + (fk - 1) * prm.dz;
+        bool in = xp > ta.box[0] && xp < ta.box[3];
+        if (prm.ndim > 1) in = in && yp > ta.box[1] && yp < ta.box[4];
+        if (prm.ndim > 2) in = in && zp > ta.box[2] && zp < ta.box[5];
+        if (in) {
+            const int sh = ta.quantity == SP_INJECT_DIVV ? 2 : 0;
+            const int gi = fi + 1 - sh;
+            const int gj = prm.ndim > 1 ? fj + 1 - sh : 0;
+            const int gk = prm.ndim > 2 ? fk + 1 - sh : 0;
+            const long long node = gi + (long long)prm.nxg * (gj + (long long)prm.nyg * gk);
+            c = node_quantity(prm, ta, node) > ta.threshold ? 1u : 0u;
+        }
+    }
+    const unsigned long long m = __ballot(c != 0u);
+    if ((threadIdx.x & (kWave - 1)) == 0 && m) atomicAdd(count, (unsigned long long)__popcll(m));
+}
+
+// The quantity at a trial point from the frame-1 fields interpolated as
+// interp_fields does with rt = 0 (corners k, j, i with i fastest, FP32 node
+// values times FP64 weights), in FP64 (particle_module.f90:883-890 jz,
+// :1035-1043 |J|, :1318-1323 divV, :1456 rho).
+__device__ double point_quantity(const DevParams& prm, const ThreshArgs& ta, double x, double y,
+                                 double z) {
+    const double px = (x - prm.xmin) / prm.dx;
+    const double py = (y - prm.ymin) / prm.dy;
+    const double fx = floor(px), fy = floor(py);
+    const double rx = (px - (fx + 1.0)) + 1.0, ry = (py - (fy + 1.0)) + 1.0;
+    const int ix = clamp_cell(fx, prm.nxg - 4) + 2, iy = clamp_cell(fy, prm.nyg - 4) + 2;
+    int iz = 0;
+    double rz = 0.0;
+    if (prm.ndim > 2) {
+        const double pz = (z - prm.zmin) / prm.dz;
+        const double fz = floor(pz);
+        rz = (pz - (fz + 1.0)) + 1.0;
+        iz = clamp_cell(fz, prm.nzg - 4) + 2;
+    }
+    const double rx1 = 1.0 - rx, ry1 = 1.0 - ry, rz1 = 1.0 - rz;
+    const double w[8] = {rx1 * ry1 * rz1, rx * ry1 * rz1, rx1 * ry * rz1, rx * ry * rz1,
+                         rx1 * ry1 * rz,  rx * ry1 * rz,  rx1 * ry * rz,  rx * ry * rz};
+    // components used: rho, dvx/dx, dvy/dy, dvz/dz and the six off-diagonal dB_i/dx_j
+    const int comp[10] = {3, 8, 12, 16, 21, 22, 23, 25, 26, 27};
+    double acc[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) acc[c] = 0.0;
+    const int nk = prm.ndim > 2 ? 2 : 1;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i) {
+                const long long node = (ix + i) + (long long)prm.nxg * ((iy + j) + (long long)prm.nyg * (iz + k));
+                float f[32];
+                node_components(prm, ta.idxh, ta.idyh, ta.idzh, ta.n8, node, f);
+                const double wq = w[k * 4 + j * 2 + i];
+#pragma unroll
+                for (int c = 0; c < 10; ++c) acc[c] = acc[c] + (double)f[comp[c]] * wq;
+            }
+    const double rho = acc[0], dvx_dx = acc[1], dvy_dy = acc[2], dvz_dz = acc[3];
+    const double dbx_dy = acc[4], dbx_dz = acc[5], dby_dx = acc[6], dby_dz = acc[7];
+    const double dbz_dx = acc[8], dbz_dy = acc[9];
+    switch (ta.quantity) {
+    case SP_INJECT_JZ:
+        return fabs(dby_dx - dbx_dy);
+    case SP_INJECT_ABSJ: {
+        const double a = dby_dz - dbz_dy, b = dbz_dx - dbx_dz, c = dbx_dy - dby_dx;
+        return sqrt(a * a + b * b + c * c);
+    }
+    case SP_INJECT_DIVV: {
+        double d = dvx_dx;
+        if (prm.ndim > 1) {
+            d = d + dvy_dy;
+            if (prm.ndim > 2) d = d + dvz_dz;
+        }
+        return d;
+    }
+    default:
+        return rho;
+    }
+}
+
+// inject_particles_at_large_{jz,absj,divv,rho} (particle_module.f90:785-1470),
+// one rank: particle i (tag tag0 + i) draws x, y, z over the domain until the
+// point lies in the box (inclusive) and its quantity passes; a point outside
+// the box scores -3 (jz, |J|), divV = 3 or rho = 0.  The loop tests its initial
+// value first (jz, |J|: -2, divV: 2, rho: 0); when that passes, no point is
+// drawn and the particle sits at the box's low corner.  Then mu and
+// inject_one_particle.  A particle still failing after max_trials draws
+// counts in *fails; with commit = 0 nothing is stored.
+__global__ void __launch_bounds__(kBlock) threshold_inject_kernel(DevParams prm, Ptl P,
+                                                                  long long start, long long nptl,
+                                                                  long long nmax, int tag0,
+                                                                  InjectArgs ia, ThreshArgs ta,
+                                                                  unsigned long long* fails) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nptl) return;
+    long long slot = start + i;
+    bool store = ta.commit != 0;
+    if (slot >= nmax) {
+        if (i != nptl - 1) store = false;
+        slot = nmax - 1;
+    }
+    DrawStream ds{(uint32_t)(tag0 + (int)i), 0u, prm.key0, prm.key1, prm.stream, U4{0, 0, 0, 0}, 4};
+    const bool divv = ta.quantity == SP_INJECT_DIVV;
+    const double v_init = divv ? 2.0 : (ta.quantity == SP_INJECT_RHO ? 0.0 : -2.0);
+    const double v_out = divv ? 3.0 : (ta.quantity == SP_INJECT_RHO ? 0.0 : -3.0);
+    double xt = ta.box[0], yt = ta.box[1], zt = ta.box[2];
+    double v = v_init;
+    long long trials = 0;
+    while ((divv ? -v : v) < ta.threshold && trials < ta.max_trials) {
+        xt = ds.next() * (prm.xmax - prm.xmin) + prm.xmin;
+        yt = ds.next() * (prm.ymax - prm.ymin) + prm.ymin;
+        zt = ds.next() * (prm.zmax - prm.zmin) + prm.zmin;
+        if (xt >= ta.box[0] && xt <= ta.box[3] && yt >= ta.box[1] && yt <= ta.box[4] &&
+            zt >= ta.box[2] && zt <= ta.box[5])
+            v = point_quantity(prm, ta, xt, yt, zt);
+        else
+            v = v_out;
+        ++trials;
+    }
+    if ((divv ? -v : v) < ta.threshold) {
+        atomicAdd(fails, 1ull);
+        return;
+    }
+    if (!store) return;
+    const double mu = (double)0.99f * (2.0 * ds.next() - 1.0);
+    inject_one(prm, P, slot, ds, xt, yt, zt, mu, ia, tag0 + (int)i);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,6 +666,29 @@ hipError_t launch_inject(const DevParams& prm, const Ptl& P, long long start, lo
 }
 
 long long num_tiles(long long n) { return (n + kTile - 1) / kTile; }
+
+hipError_t launch_threshold_count(const DevParams& prm, const ThreshArgs& ta,
+                                  unsigned long long* count, hipStream_t s) {
+    const long long nx = prm.nxg - 4;
+    const long long ny = prm.ndim > 1 ? prm.nyg - 4 : 1;
+    const long long nz = prm.ndim > 2 ? prm.nzg - 4 : 1;
+    const long long total = nx * ny * nz;
+    const long long grid = (total + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(threshold_count_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s, prm, ta,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t launch_threshold_inject(const DevParams& prm, const Ptl& P, long long start,
+                                   long long nptl, long long nmax, int tag0, const InjectArgs& ia,
+                                   const ThreshArgs& ta, unsigned long long* fails, hipStream_t s) {
+    if (nptl <= 0) return hipSuccess;
+    const long long grid = (nptl + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(threshold_inject_kernel, dim3((unsigned)grid), dim3(kBlock), 0, s, prm, P,
+                       start, nptl, nmax, tag0, ia, ta, fails);
+    return hipGetLastError();
+}
 
 hipError_t launch_tile_count(const Ptl& P, long long n, int mode, const SplitArgs& sa,
                              unsigned* tile_cnt, unsigned* tile_esc, const DistArgs* da,

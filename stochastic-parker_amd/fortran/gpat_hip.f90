@@ -23,6 +23,8 @@ module gpat_hip
         init_particles, free_particles, init_field_data, free_field_data, &
         read_field_data_parallel, calc_fields_gradients, copy_fields, &
         inject_particles_spatial_uniform, particle_mover, split_particle, &
+        inject_particles_at_large_jz, inject_particles_at_large_absj, &
+        inject_particles_at_large_divv, inject_particles_at_large_rho, &
         init_particle_distributions, distributions_diagnostics, quick_check, &
         get_pmax_global, nptl_current, leak, leak_negp, steps_total, kernel_ms_total, &
         fglobal, pbins_edges_global, check_status
@@ -378,6 +380,73 @@ contains
         call check_status(sp_particles_counts(ctx, nptl_current, ns, ne, tag_max), &
             "sp_particles_counts")
     end subroutine inject_particles_spatial_uniform
+
+    !< inject_particles_at_large_jz (particle_module.f90:785-903)
+    subroutine inject_particles_at_large_jz(nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, jz_min, ncells_large_jz_norm, part_box, power_index)
+        integer, intent(in) :: nptl, dist_flag, ct_mhd, ncells_large_jz_norm
+        logical, intent(in) :: inject_same_nptl
+        real(dp), intent(in) :: dt, jz_min, power_index, particle_v0
+        real(dp), intent(in) :: part_box(6)
+        call inject_threshold(SP_INJECT_JZ, "jz", nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, jz_min, ncells_large_jz_norm, part_box, power_index)
+    end subroutine inject_particles_at_large_jz
+
+    !< inject_particles_at_large_absj (particle_module.f90:922-1056)
+    subroutine inject_particles_at_large_absj(nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, absj_min, ncells_large_absj_norm, part_box, power_index)
+        integer, intent(in) :: nptl, dist_flag, ct_mhd, ncells_large_absj_norm
+        logical, intent(in) :: inject_same_nptl
+        real(dp), intent(in) :: dt, absj_min, power_index, particle_v0
+        real(dp), intent(in) :: part_box(6)
+        call inject_threshold(SP_INJECT_ABSJ, "absj", nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, absj_min, ncells_large_absj_norm, part_box, power_index)
+    end subroutine inject_particles_at_large_absj
+
+    !< inject_particles_at_large_divv (particle_module.f90:1205-1337)
+    subroutine inject_particles_at_large_divv(nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, divv_min, ncells_large_divv_norm, part_box, power_index)
+        integer, intent(in) :: nptl, dist_flag, ct_mhd, ncells_large_divv_norm
+        logical, intent(in) :: inject_same_nptl
+        real(dp), intent(in) :: dt, divv_min, power_index, particle_v0
+        real(dp), intent(in) :: part_box(6)
+        call inject_threshold(SP_INJECT_DIVV, "divv", nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, divv_min, ncells_large_divv_norm, part_box, power_index)
+    end subroutine inject_particles_at_large_divv
+
+    !< inject_particles_at_large_rho (particle_module.f90:1356-1468)
+    subroutine inject_particles_at_large_rho(nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, rho_min, ncells_large_rho_norm, part_box, power_index)
+        integer, intent(in) :: nptl, dist_flag, ct_mhd, ncells_large_rho_norm
+        logical, intent(in) :: inject_same_nptl
+        real(dp), intent(in) :: dt, rho_min, power_index, particle_v0
+        real(dp), intent(in) :: part_box(6)
+        call inject_threshold(SP_INJECT_RHO, "rho", nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, rho_min, ncells_large_rho_norm, part_box, power_index)
+    end subroutine inject_particles_at_large_rho
+
+    subroutine inject_threshold(quantity, what, nptl, dt, dist_flag, particle_v0, ct_mhd, &
+            inject_same_nptl, threshold, ncells_norm, part_box, power_index)
+        integer(c_int32_t), intent(in) :: quantity
+        character(*), intent(in) :: what
+        integer, intent(in) :: nptl, dist_flag, ct_mhd, ncells_norm
+        logical, intent(in) :: inject_same_nptl
+        real(dp), intent(in) :: dt, threshold, power_index, particle_v0
+        real(dp), intent(in) :: part_box(6)
+        real(dp) :: t0, dtf
+        integer(c_int32_t) :: tag_max
+        integer(c_int64_t) :: ns, ne, ninj
+        t0 = tstamps_mhd(ct_mhd)
+        dtf = tstamps_mhd(ct_mhd + 1) - t0
+        call check_status(sp_inject_particles_threshold(ctx, quantity, int(nptl, c_int64_t), dt, &
+            int(dist_flag, c_int32_t), particle_v0, t0, dtf, merge(1, 0, inject_same_nptl), &
+            threshold, int(ncells_norm, c_int64_t), part_box, power_index, ninj), &
+            "sp_inject_particles_threshold")
+        call check_status(sp_particles_counts(ctx, nptl_current, ns, ne, tag_max), &
+            "sp_particles_counts")
+        write(*, "(A,A,A,I0,A)") "Finished injecting particles where ", what, " is large (", &
+            ninj, ")"
+    end subroutine inject_threshold
 
     !< particle_mover (particle_module.f90:1846-1974)
     subroutine particle_mover(focused_transport, nlgc, kperp_kpara, nsteps_interval, &

@@ -13,17 +13,21 @@ module sp_hip_binding
     public sp_ctx_create, sp_ctx_destroy, sp_last_error, sp_abi_version, &
         sp_fields_upload, sp_fields_gradients, sp_fields_copy, &
         sp_particles_upload, sp_particles_download, sp_particles_counts, &
-        sp_inject_particles_spatial_uniform, sp_particle_mover, sp_sort_particles, &
+        sp_inject_particles_spatial_uniform, sp_ncells_above_threshold, &
+        sp_inject_particles_threshold, sp_particle_mover, sp_sort_particles, &
         sp_split_particle, sp_step_once, sp_distributions_init, sp_distributions, &
         sp_local_dist_dims, sp_escaped_distributions, sp_quick_check, sp_calibration_copy, &
         sp_error_message, &
         sp_set_variants, sp_inject_variants, sp_variant_distributions, sp_autotune_mover
     public SP_OK, SP_PREC_F64, SP_PREC_F32, SP_PREC_F32H
+    public SP_INJECT_JZ, SP_INJECT_ABSJ, SP_INJECT_DIVV, SP_INJECT_RHO
 
     integer(c_int), parameter :: SP_OK = 0
     integer(c_int32_t), parameter :: SP_PREC_F64 = 0
     integer(c_int32_t), parameter :: SP_PREC_F32 = 1
     integer(c_int32_t), parameter :: SP_PREC_F32H = 2
+    integer(c_int32_t), parameter :: SP_INJECT_JZ = 1, SP_INJECT_ABSJ = 2
+    integer(c_int32_t), parameter :: SP_INJECT_DIVV = 4, SP_INJECT_RHO = 5
 
     type, bind(C) :: sp_config
         integer(c_int32_t) :: ndim, nx, ny, nz
@@ -158,6 +162,35 @@ module sp_hip_binding
             real(c_double), intent(in) :: part_box(6)
             real(c_double), value :: power_index
         end function sp_inject_particles_spatial_uniform
+
+        integer(c_int) function sp_ncells_above_threshold(ctx, quantity, threshold, part_box, &
+                ncells) bind(C, name="sp_ncells_above_threshold")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: quantity
+            real(c_double), value :: threshold
+            real(c_double), intent(in) :: part_box(6)
+            integer(c_int64_t), intent(out) :: ncells
+        end function sp_ncells_above_threshold
+
+        integer(c_int) function sp_inject_particles_threshold(ctx, quantity, nptl, dt, &
+                dist_flag, particle_v0, t0, dtf, inject_same_nptl, threshold, ncells_norm, &
+                part_box, power_index, nptl_injected) &
+                bind(C, name="sp_inject_particles_threshold")
+            import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: quantity
+            integer(c_int64_t), value :: nptl
+            real(c_double), value :: dt
+            integer(c_int32_t), value :: dist_flag
+            real(c_double), value :: particle_v0, t0, dtf
+            integer(c_int32_t), value :: inject_same_nptl
+            real(c_double), value :: threshold
+            integer(c_int64_t), value :: ncells_norm
+            real(c_double), intent(in) :: part_box(6)
+            real(c_double), value :: power_index
+            integer(c_int64_t), intent(out) :: nptl_injected
+        end function sp_inject_particles_threshold
 
         integer(c_int) function sp_particle_mover(ctx, t0, dtf, num_fine_steps, &
                 nsteps_interval, stats) bind(C, name="sp_particle_mover")

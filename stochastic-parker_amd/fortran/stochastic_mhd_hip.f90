@@ -39,6 +39,11 @@ program stochastic_hip
     logical :: inject_new_ptl = .false., inject_part_box = .false., restart_flag = .false.
     logical :: dump_escaped_dist = .false., track_particle_flag = .false.
     logical :: inject_at_shock = .false., inject_large_jz = .false., varying_dt_mhd = .false.
+    logical :: inject_same_nptl = .false., inject_large_absj = .false., inject_large_db2 = .false.
+    logical :: inject_large_divv = .false., inject_large_rho = .false.
+    real(dp) :: jz_min = 100.0_dp, absj_min = 5.0_dp, divv_min = 10.0_dp, rho_min = 5.0_dp
+    integer :: ncells_large_jz_norm = 800, ncells_large_absj_norm = 800
+    integer :: ncells_large_divv_norm = 800, ncells_large_rho_norm = 800
     real(dp) :: part_box(6)
     real(dp) :: step1, step2, start, finish
     integer :: tf, tmin
@@ -93,8 +98,26 @@ program stochastic_hip
         endif
         if (tf == t_start + 1 .or. inject_new_ptl) then
             if (tf <= tmax_to_inject) then
-                call inject_particles_spatial_uniform(nptl, dt, dist_flag, particle_v0, &
-                    tf - t_start, part_box, power_index)
+                if (inject_large_jz) then
+                    call inject_particles_at_large_jz(nptl, dt, dist_flag, particle_v0, &
+                        tf - t_start, inject_same_nptl, jz_min, ncells_large_jz_norm, &
+                        part_box, power_index)
+                else if (inject_large_absj) then
+                    call inject_particles_at_large_absj(nptl, dt, dist_flag, particle_v0, &
+                        tf - t_start, inject_same_nptl, absj_min, ncells_large_absj_norm, &
+                        part_box, power_index)
+                else if (inject_large_divv) then
+                    call inject_particles_at_large_divv(nptl, dt, dist_flag, particle_v0, &
+                        tf - t_start, inject_same_nptl, divv_min, ncells_large_divv_norm, &
+                        part_box, power_index)
+                else if (inject_large_rho) then
+                    call inject_particles_at_large_rho(nptl, dt, dist_flag, particle_v0, &
+                        tf - t_start, inject_same_nptl, rho_min, ncells_large_rho_norm, &
+                        part_box, power_index)
+                else
+                    call inject_particles_spatial_uniform(nptl, dt, dist_flag, particle_v0, &
+                        tf - t_start, part_box, power_index)
+                endif
             endif
         endif
         if (tf == t_start + 1) then
@@ -136,7 +159,8 @@ contains
         if (include_3rd_dim /= 0) call stop_scope("-i3 1")
         if (acc_by_surface /= 0) call stop_scope("-as 1")
         if (track_particle_flag) call stop_scope("-tf .true. (particle tracking)")
-        if (inject_at_shock .or. inject_large_jz) call stop_scope("threshold/shock injection")
+        if (inject_at_shock) call stop_scope("-is .true. (shock injection)")
+        if (inject_large_db2) call stop_scope("-ib .true. (needs the turbulence-amplitude map)")
         if (varying_dt_mhd) call stop_scope("-vdt .true. (time_stamps.dat)")
         if (ndim_field /= 2 .and. ndim_field /= 3) call stop_scope("-nd other than 2 or 3")
         if (size_mpi_sub /= 1) write(*, "(A)") " -sm ignored: one GPU per run"
@@ -214,6 +238,19 @@ contains
             case ('-is', '--inject_at_shock'); inject_at_shock = to_logical(val)
             case ('-in', '--inject_new_ptl'); inject_new_ptl = to_logical(val)
             case ('-ij', '--inject_large_jz'); inject_large_jz = to_logical(val)
+            case ('-sn', '--inject_same_nptl'); inject_same_nptl = to_logical(val)
+            case ('-jz', '--jz_min'); jz_min = to_real(val)
+            case ('-nn', '--ncells_large_jz_norm'); ncells_large_jz_norm = to_int(val)
+            case ('-iaj', '--inject_large_absj'); inject_large_absj = to_logical(val)
+            case ('-ajm', '--absj_min'); absj_min = to_real(val)
+            case ('-naj', '--ncells_large_absj_norm'); ncells_large_absj_norm = to_int(val)
+            case ('-ib', '--inject_large_db2'); inject_large_db2 = to_logical(val)
+            case ('-iv', '--inject_large_divv'); inject_large_divv = to_logical(val)
+            case ('-dv', '--divv_min'); divv_min = to_real(val)
+            case ('-nv', '--ncells_large_divv_norm'); ncells_large_divv_norm = to_int(val)
+            case ('-ir', '--inject_large_rho'); inject_large_rho = to_logical(val)
+            case ('-rm', '--rho_min'); rho_min = to_real(val)
+            case ('-nr', '--ncells_large_rho_norm'); ncells_large_rho_norm = to_int(val)
             case ('-tti', '--tmax_to_inject'); tmax_to_inject = to_int(val)
             case ('-ip', '--inject_part_box'); inject_part_box = to_logical(val)
             case ('-xs', '--ptl_xmin'); ptl_xmin = to_real(val)
@@ -245,8 +282,7 @@ contains
             case ('-gp', '--gpu_precision'); gpu_precision = val
             case ('-rs', '--rng_seed'); rng_seed = int(to_real(val), c_int64_t)
             case ('-ss', '--sort_steps'); sort_steps = to_int(val)
-            case ('-sn', '-ib', '-iv', '-ir', '-iaj', '-jz', '-nn', '-db2', '-nb', '-dv', '-nv', &
-                  '-rm', '-nr', '-ajm', '-naj', '-pd', '-ptf', '-de', '-du', &
+            case ('-db2', '-nb', '-pd', '-ptf', '-de', '-du', &
                   '-sf1', '-sn1', '-s2e', '-ii', '-sf2', '-sn2')
                 ! accepted and ignored: options of paths this build does not run
             case default

@@ -17,6 +17,7 @@ SP_OK = 0
 SP_PREC_F64 = 0
 SP_PREC_F32 = 1
 SP_PREC_F32H = 2
+SP_INJECT = {"jz": 1, "absj": 2, "db2": 3, "divv": 4, "rho": 5}
 COUNT_FLAG_INBOX = 1
 COUNT_FLAG_OTHERS = 0
 
@@ -139,6 +140,10 @@ EXPORTS = {
     "sp_distributions": [_P, _D, _D, _D, _D, _D],
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],
     "sp_escaped_distributions": [_P, _D, _D, _D, _D, _D],
+    "sp_ncells_above_threshold": [_P, C.c_int32, C.c_double, _D, C.POINTER(C.c_int64)],
+    "sp_inject_particles_threshold": [_P, C.c_int32, C.c_int64, C.c_double, C.c_int32, C.c_double,
+                                      C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int64,
+                                      _D, C.c_double, C.POINTER(C.c_int64)],
     "sp_quick_check": [_P, _D],
     "sp_calibration_copy": [_P, C.c_int64, _D],
     "sp_autotune_mover": [_P, C.c_double, C.c_double, C.c_int32, C.c_int32,

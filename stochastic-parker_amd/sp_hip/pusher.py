@@ -157,6 +157,28 @@ class ParkerPusher:
         self._call("sp_inject_particles_spatial_uniform", int(nptl), float(dt), int(dist_flag),
                    float(particle_v0), float(t0), float(dtf), dptr(box), float(power_index))
 
+    def ncells_above_threshold(self, quantity, threshold, part_box) -> int:
+        """get_ncells_large_{jz,absj,divv,rho}; quantity is one of 'jz', 'absj',
+        'divv', 'rho'."""
+        box = np.ascontiguousarray(part_box, dtype=np.float64)
+        n = C.c_int64(0)
+        self._call("sp_ncells_above_threshold", _abi.SP_INJECT[quantity], float(threshold),
+                   dptr(box), C.byref(n))
+        return n.value
+
+    def inject_particles_threshold(self, quantity, nptl, dt, dist_flag, particle_v0, t0, dtf,
+                                   inject_same_nptl, threshold, ncells_norm, part_box,
+                                   power_index=7.0) -> int:
+        """inject_particles_at_large_{jz,absj,divv,rho}; returns the number
+        injected."""
+        box = np.ascontiguousarray(part_box, dtype=np.float64)
+        n = C.c_int64(0)
+        self._call("sp_inject_particles_threshold", _abi.SP_INJECT[quantity], int(nptl), float(dt),
+                   int(dist_flag), float(particle_v0), float(t0), float(dtf),
+                   int(bool(inject_same_nptl)), float(threshold), int(ncells_norm), dptr(box),
+                   float(power_index), C.byref(n))
+        return n.value
+
     def particle_mover(self, t0, dtf, num_fine_steps=1, nsteps_interval=10) -> dict:
         st = SpMoverStats()
         self._call("sp_particle_mover", float(t0), float(dtf), int(num_fine_steps),

@@ -94,8 +94,9 @@ def write_case(tmp, n, dtf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_driver_end_to_end_matches_host_api(tmp_path, precision):
+@pytest.mark.parametrize("precision,inject", [("f64", None), ("f32", None), ("f64", "divv"),
+                                              ("f32", "jz")])
+def test_driver_end_to_end_matches_host_api(tmp_path, precision, inject):
     _need(DRIVER)
     n, npart, dtf = 64, 3000, 0.05
     nodes = write_case(tmp_path, n, dtf)
@@ -103,6 +104,12 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
             "-st", "1", "-df", "1", "-sf", "1", "-sr", "1.3", "-ps", "1.05", "-nl", ".false.",
             "-ug", "1", "-nd", "2", "-ld", ".true.", "-dd", "data/", "-dt", "1e-7",
             "-gp", precision, "-ded", ".true."]
+    # threshold injections: -iv/-dv (-div V > 0.2) or -ij/-jz (|jz| > 2), same count
+    thresholds = {"divv": 0.2, "jz": 2.0}
+    if inject == "divv":
+        args += ["-iv", ".true.", "-dv", "0.2", "-sn", ".true."]
+    elif inject == "jz":
+        args += ["-ij", ".true.", "-jz", "2.0", "-sn", ".true."]
     print("driver:", " ".join(args[1:]), flush=True)
     r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -120,7 +127,11 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
         pz.upload_fields(nodes)
         pz.calc_fields_gradients()
         pz.init_particle_distributions(**dist, escaped_dist=1)
-        pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, 0.0, dtf, box)
+        if inject is None:
+            pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, 0.0, dtf, box)
+        else:
+            assert pz.inject_particles_threshold(inject, npart, 1e-7, 1, 5.0, 0.0, dtf, True,
+                                                 thresholds[inject], 800, box) == npart
         fg0, loc0 = pz.calc_particle_distributions()
         rows.append((0, pz.quick_check()))
         for tf in (1, 2):
@@ -158,7 +169,8 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision):
     assert struct.unpack_from("<i", raw, off)[0] == 4
     fesc = np.frombuffer(raw, dtype="<f8", count=fe.size, offset=off + 4)
     np.testing.assert_array_equal(fesc, fe.ravel())
-    assert fe.sum() > 0
+    if inject is None:
+        assert fe.sum() > 0   # the jz / divV regions lie away from the open y faces
     off += 4 + 8 * fe.size
     kset, size = struct.unpack_from("<iq", raw, off)
     flat = np.concatenate([esc[0]["x"].ravel(), esc[0]["y"].ravel()])

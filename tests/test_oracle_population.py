@@ -232,3 +232,140 @@ def test_escaped_weight_equals_leak_through_open_faces():
     per_face = fe.reshape(4, -1).sum(1)
     for f in range(1, 5):
         assert per_face[f - 1] == esc["weight"][esc["count_flag"] == -f].sum()
+
+
+# ---------------------------------------------------------------------------
+# Threshold injections and their cell counters
+
+def node_quantities(c, fa, quantity):
+    """numpy restatement of the counters' node quantity over interior nodes
+    (C order (z, y, x)); |jz| and |J| in FP32 as the Fortran's real(sp) farray."""
+    nd = c["ndim"]
+    sl = (slice(2, -2) if nd == 3 else slice(0, 1), slice(2, -2), slice(2, -2))
+    if quantity == "divv":
+        # the counter's array is rebound to start at 1: node (i, j, k) reads
+        # (i-2, j-2, k-2) of the ghosted array (no z ghosts in 2D)
+        sl = (slice(0, -4) if nd == 3 else slice(0, 1), slice(0, -4), slice(0, -4))
+        d = fa[sl + (8,)].astype(np.float64) + fa[sl + (12,)].astype(np.float64)
+        if nd == 3:
+            d = d + fa[sl + (16,)].astype(np.float64)
+        return -d
+    f = fa[sl]
+    if quantity == "jz":
+        return np.abs(f[..., 23] - f[..., 21]).astype(np.float64)
+    if quantity == "absj":
+        a, b, cc = f[..., 25] - f[..., 27], f[..., 26] - f[..., 22], f[..., 21] - f[..., 23]
+        return np.sqrt(a * a + b * b + cc * cc).astype(np.float64)
+    return f[..., 3].astype(np.float64)
+
+
+def in_box_nodes(c, box):
+    nd = c["ndim"]
+    x = c["xmin"] + np.arange(c["nx"]) * c["dx"]
+    y = c["ymin"] + np.arange(c["ny"]) * c["dy"]
+    z = c["zmin"] + np.arange(c["nz"]) * c["dz"]
+    ix = (x > box[0]) & (x < box[3])
+    iy = (y > box[1]) & (y < box[4])
+    iz = (z > box[2]) & (z < box[5]) if nd == 3 else np.ones(1, bool)
+    return iz[:, None, None] & iy[None, :, None] & ix[None, None, :]
+
+
+def interp_component(c, fa, comp, x, y, z):
+    """Bilinear / trilinear interpolation of one farray component (numpy)."""
+    nd = c["ndim"]
+    px, py = (x - c["xmin"]) / c["dx"], (y - c["ymin"]) / c["dy"]
+    i, j = np.floor(px).astype(int) + 2, np.floor(py).astype(int) + 2
+    rx, ry = px - np.floor(px), py - np.floor(py)
+    if nd == 2:
+        g = fa[0, :, :, comp].astype(np.float64)
+        return (g[j, i] * (1 - rx) * (1 - ry) + g[j, i + 1] * rx * (1 - ry) +
+                g[j + 1, i] * (1 - rx) * ry + g[j + 1, i + 1] * rx * ry)
+    pz = (z - c["zmin"]) / c["dz"]
+    k, rz = np.floor(pz).astype(int) + 2, pz - np.floor(pz)
+    g = fa[..., comp].astype(np.float64)
+    out = 0.0
+    for dk, wk in ((0, 1 - rz), (1, rz)):
+        for dj, wj in ((0, 1 - ry), (1, ry)):
+            for di, wi in ((0, 1 - rx), (1, rx)):
+                out = out + g[k + dk, j + dj, i + di] * wk * wj * wi
+    return out
+
+
+def point_quantities(c, fa, quantity, x, y, z):
+    q = lambda k: interp_component(c, fa, k, x, y, z)
+    if quantity == "jz":
+        return np.abs(q(23) - q(21))
+    if quantity == "absj":
+        return np.sqrt((q(25) - q(27)) ** 2 + (q(26) - q(22)) ** 2 + (q(21) - q(23)) ** 2)
+    if quantity == "divv":
+        d = q(8) + q(12) + (q(16) if c["ndim"] == 3 else 0.0)
+        return -d
+    return q(3)
+
+
+def threshold_case(ndim):
+    from helpers import cfg3d, nodes_for
+    c = cfg2d(32) if ndim == 2 else cfg3d(12)
+    nodes = nodes_for(c)
+    # a density that varies, so that the rho injection has something to select
+    zz, yy, xx = np.meshgrid(*[np.arange(s) for s in nodes.shape[:3]], indexing="ij")
+    nodes[..., 3] = (1.0 + 0.5 * np.sin(0.3 * xx + 0.2 * yy + 0.1 * zz)).astype(np.float32)
+    prm = oracle.params(c)
+    return c, nodes, prm, oracle.gradients(prm, nodes)
+
+
+THRESHOLDS = {"jz": 3.0, "absj": 3.0, "divv": 0.3, "rho": 1.2}
+TBOX = (0.1, 0.2, 0.05, 0.85, 0.9, 0.95)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("quantity", ["jz", "absj", "divv", "rho"])
+def test_threshold_cell_counts_match_numpy(ndim, quantity):
+    c, _, prm, fa = threshold_case(ndim)
+    thr = THRESHOLDS[quantity]
+    n = oracle.ncells_above_threshold(prm, fa, quantity, thr, TBOX)
+    ref = int(np.sum((node_quantities(c, fa, quantity) > thr) & in_box_nodes(c, TBOX)))
+    assert n == ref and 0 < n < in_box_nodes(c, TBOX).sum()
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("quantity", ["jz", "absj", "divv", "rho"])
+def test_threshold_injection_places_particles_above_threshold(ndim, quantity):
+    c, _, prm, fa = threshold_case(ndim)
+    thr = THRESHOLDS[quantity]
+    empty = np.zeros(0, oracle.PARTICLE_DTYPE)
+    a, tag_max, ninj = oracle.inject_threshold(prm, fa, empty, 5000, quantity, 2000, 1e-7, 1, 5.0,
+                                               0.0, 0.1, True, thr, 1, TBOX, 7.0)
+    assert ninj == len(a) == 2000 and tag_max == 2000
+    for k, lo, hi in (("x", 0, 3), ("y", 1, 4), ("z", 2, 5)):
+        assert np.all((a[k] >= TBOX[lo]) & (a[k] <= TBOX[hi]))
+    v = point_quantities(c, fa, quantity, a["x"], a["y"], a["z"])
+    assert np.all(v >= thr * (1 - 1e-12))
+    # the accepted points cover the region where the quantity passes: the
+    # fraction of uniform box points that pass matches the density of hits
+    rng = np.random.default_rng(3)
+    u = [rng.uniform(TBOX[i], TBOX[i + 3], 20000) for i in range(3)]
+    if ndim == 2:
+        u[2] = np.full(20000, 0.5)
+    frac = np.mean(point_quantities(c, fa, quantity, *u) >= thr)
+    assert 0.0 < frac < 1.0
+    # with ncells_norm, nptl_inject = int(nptl * ncells / ncells_norm)
+    ncells = oracle.ncells_above_threshold(prm, fa, quantity, thr, TBOX)
+    _, _, n2 = oracle.inject_threshold(prm, fa, empty, 5000, quantity, 1000, 1e-7, 1, 5.0, 0.0,
+                                       0.1, False, thr, 4 * ncells, TBOX, 7.0)
+    assert n2 == 250
+
+
+def test_threshold_injection_edge_cases():
+    c, _, prm, fa = threshold_case(2)
+    empty = np.zeros(0, oracle.PARTICLE_DTYPE)
+    # no node passes: nothing is injected
+    _, _, n = oracle.inject_threshold(prm, fa, empty, 100, "jz", 50, 1e-7, 1, 5.0, 0.0, 0.1, True,
+                                      1e9, 1, TBOX, 7.0)
+    assert n == 0
+    # a threshold the loop's initial value already passes (rho: 0 >= -1): no
+    # point is drawn and every particle sits at the box's low corner
+    a, _, n = oracle.inject_threshold(prm, fa, empty, 100, "rho", 20, 1e-7, 1, 5.0, 0.0, 0.1, True,
+                                      -1.0, 1, TBOX, 7.0)
+    assert n == 20
+    assert np.all(a["x"] == TBOX[0]) and np.all(a["y"] == TBOX[1]) and np.all(a["z"] == TBOX[2])
