@@ -68,8 +68,13 @@ __device__ __forceinline__ void pin_constants(MoverParams& m) {
     pin_vgpr(m.lx); pin_vgpr(m.ly); pin_vgpr(m.lz);
 }
 
+// At most two resident waves per SIMD: the mover runs one or two 256-lane
+// blocks per CU (fewer particles in flight keep their field records in L2),
+// so the compiler may spend registers on keeping the gather's loads in flight
+// rather than on a higher occupancy that is never used (3D: +13 %).
 template <class Pol, int NDIM, bool TINTERP, int FX>
-__global__ void __launch_bounds__(256) mover_kernel(MoverParams m0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+mover_kernel(MoverParams m0) {
     MoverParams m = m0;
     // 2D steps are instruction-bound and gain from this; 3D steps are
     // memory-bound and lose occupancy headroom to the extra VGPRs

@@ -4,8 +4,9 @@
 // :4625-4907) with the gather, the coefficient arithmetic and the increments
 // in FP32; position, momentum and elapsed time stay FP64 accumulators.
 // Arithmetic is regrouped for the hardware: reciprocals and square roots use
-// the single-instruction v_rcp_f32 / v_rsq_f32 / v_sqrt_f32, the momentum
-// power is exp2(pindex * log2(p/p0)), and contraction into FMA is allowed.
+// the single-instruction v_rcp_f32 / v_rsq_f32 / v_sqrt_f32, powers are
+// exp2(y * log2(x)) on v_log_f32 / v_exp_f32, and contraction into FMA is
+// allowed.
 #pragma once
 
 #include "sp_device.h"
@@ -16,6 +17,12 @@ namespace f32 {
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// x^y = 2^(y log2 x), x > 0, on v_log_f32 / v_exp_f32 (about 1 ulp each): three
+// instructions where the library powf spends about 120 on double-float
+// arithmetic
+__device__ __forceinline__ float fpow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
 
 struct Interp {
     int ix, iy, iz;
@@ -198,7 +205,7 @@ __device__ __forceinline__ void dpp_terms(const MoverParams& m, float pf, const 
                                        2.0f * (sxy * sxy + sxz * sxz + syz * syz));
         }
         if (gshear > 0.0f) {
-            const float q = __powf(pf * (float)m.inv_p0, m.f_pindex);
+            const float q = fpow(pf * (float)m.inv_p0, m.f_pindex);
             const float dd = gshear * m.f_tau0 * knorm_para * m.f_p02 * q;
             dp_dt += (2.0f + m.f_pindex) * dd * frcp(pf);
             dpp += dd;
@@ -229,15 +236,15 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
 
     // --- kappa ---------------------------------------------------------
     float knorm = 1.0f;
-    if (mag) knorm = __powf(b, m.f_gm2);
+    if (mag) knorm = fpow(b, m.f_gm2);
     const float knorm_para = knorm;
-    if (m.flags & MF_MOMENTUM_DEP) knorm *= __powf((float)(s.p * m.inv_p0), m.f_pindex);
+    if (m.flags & MF_MOMENTUM_DEP) knorm *= fpow((float)(s.p * m.inv_p0), m.f_pindex);
     const float kpara = m.f_kpara0 * knorm;
     float kperp, kpp;
     if constexpr (NLGC) {
         // kpara0 * kperp_kpara * B^((gamma-2)/3) * (p/p0)^((5-gamma)/3) * mu^2
-        float kn_perp = mag ? __powf(b, m.f_perp_mag) : 1.0f;
-        if (m.flags & MF_MOMENTUM_DEP) kn_perp *= __powf((float)(s.p * m.inv_p0), m.f_perp_mom);
+        float kn_perp = mag ? fpow(b, m.f_perp_mag) : 1.0f;
+        if (m.flags & MF_MOMENTUM_DEP) kn_perp *= fpow((float)(s.p * m.inv_p0), m.f_perp_mom);
         const float muf = (float)s.mu;
         kperp = m.f_kpara0 * m.f_kperp_kpara * kn_perp * (muf * muf);
         kpp = kpara - kperp;
