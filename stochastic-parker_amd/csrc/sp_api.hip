@@ -55,6 +55,7 @@ struct Ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     long long nnodes = 0;
     int recw = 0;
+    long long nrec = 0;      // records per frame (3D: whole 8^3 bricks)
     float* n8[2] = {nullptr, nullptr};
     float* rec[2] = {nullptr, nullptr};
     bool have_fields[2] = {false, false};
@@ -196,6 +197,9 @@ sp::DevParams make_params(const sp_config& c) {
     d.time_interp = c.time_interp;
     d.precision = c.precision;
     d.ext_rec = (c.dpp_wave || c.dpp_shear) ? 1 : 0;
+    d.bnx = (d.nxg + 7) / 8;
+    d.bny = (d.nyg + 7) / 8;
+    d.bnz = (d.nzg + 7) / 8;
     for (int i = 0; i < 6; ++i) d.acc_region[i] = c.acc_region[i];
     d.key0 = (uint32_t)c.seed;
     d.key1 = (uint32_t)(c.seed >> 32);
@@ -309,6 +313,7 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.rec2 = f.time_interp ? c->rec[1] : c->rec[0];
     m.ctr = c->ctr;
     m.nxg = d.nxg; m.nyg = d.nyg; m.nzg = d.nzg;
+    m.bnx = d.bnx; m.bny = d.bny;
     m.nsteps_interval = nsteps_interval;
     m.flags = (f.pbcx == 1 ? sp::MF_OPEN_X : 0) | (f.pbcy == 1 ? sp::MF_OPEN_Y : 0) |
               (f.ndim == 3 && f.pbcz == 1 ? sp::MF_OPEN_Z : 0) |
@@ -429,6 +434,7 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
     if (c->dp.ext_rec) c->recw = cfg->ndim == 2 ? sp::kRec2DX : sp::kRec3DX;
     else c->recw = cfg->ndim == 2 ? sp::kRec2D : (cfg->precision == SP_PREC_F32H ? sp::kRec3DH : sp::kRec3D);
+    c->nrec = cfg->ndim == 3 ? (long long)c->dp.bnx * c->dp.bny * c->dp.bnz * 512 : c->nnodes;
     auto bail = [&](int code) {
         destroy(c);
         return code;
@@ -500,7 +506,7 @@ int sp_fields_gradients(void* ctx, int32_t slot, float* farray32) {
     if (!c) return rc;
     if (slot < 0 || slot > 1) return fail(SP_ERR_ARG, "slot must be 0 or 1");
     if (!c->n8[slot]) return fail(SP_ERR_STATE, "fields of this slot were not uploaded");
-    if (!c->rec[slot]) SP_HIP(hipMalloc(&c->rec[slot], sizeof(float) * c->recw * c->nnodes));
+    if (!c->rec[slot]) SP_HIP(hipMalloc(&c->rec[slot], sizeof(float) * c->recw * c->nrec));
     float* fa = nullptr;
     if (farray32) SP_HIP(hipMalloc(&fa, sizeof(float) * 32 * c->nnodes));
     hipError_t e = hipSuccess;
@@ -541,9 +547,9 @@ int sp_fields_copy(void* ctx) {
     if (!c) return rc;
     if (!c->cfg.time_interp) return fail(SP_ERR_STATE, "copy_fields needs time_interp");
     if (!c->have_fields[1]) return fail(SP_ERR_STATE, "frame 2 is not prepared");
-    if (!c->rec[0]) SP_HIP(hipMalloc(&c->rec[0], sizeof(float) * c->recw * c->nnodes));
+    if (!c->rec[0]) SP_HIP(hipMalloc(&c->rec[0], sizeof(float) * c->recw * c->nrec));
     if (!c->n8[0]) SP_HIP(hipMalloc(&c->n8[0], sizeof(float) * 8 * c->nnodes));
-    SP_HIP(hipMemcpyAsync(c->rec[0], c->rec[1], sizeof(float) * c->recw * c->nnodes, hipMemcpyDeviceToDevice, c->st));
+    SP_HIP(hipMemcpyAsync(c->rec[0], c->rec[1], sizeof(float) * c->recw * c->nrec, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipMemcpyAsync(c->n8[0], c->n8[1], sizeof(float) * 8 * c->nnodes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
     c->gscale[0] = c->gscale[1];

@@ -43,8 +43,9 @@ __global__ void __launch_bounds__(256) gradients_kernel(DevParams prm, double id
         if (prm.ext_rec) pack_record<kRec2DX, 18, 2>(out, rec + node * kRec2DX);
         else pack_record<kRec2D, 15, 2>(out, rec + node * kRec2D);
     } else {
-        if (prm.ext_rec) pack_record<kRec3DX, 28, 3>(out, rec + node * kRec3DX);
-        else pack_record<kRec3D, 21, 3>(out, rec + node * kRec3D);
+        const long long r = brick_record(i, j, k, prm.bnx, prm.bny);
+        if (prm.ext_rec) pack_record<kRec3DX, 28, 3>(out, rec + r * kRec3DX);
+        else pack_record<kRec3D, 21, 3>(out, rec + r * kRec3D);
     }
     if (fa32) {
 #pragma unroll
@@ -95,7 +96,10 @@ __global__ void __launch_bounds__(256) pack_half_kernel(DevParams prm, double id
                     ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
     }
     wd[14] = wd[15] = 0u;
-    uint4* r4 = reinterpret_cast<uint4*>(rec + node * kRec3DH);
+    const int i = (int)(node % prm.nxg);
+    const int j = (int)((node / prm.nxg) % prm.nyg);
+    const int k = (int)(node / ((long long)prm.nxg * prm.nyg));
+    uint4* r4 = reinterpret_cast<uint4*>(rec + brick_record(i, j, k, prm.bnx, prm.bny) * kRec3DH);
 #pragma unroll
     for (int u = 0; u < kRec3DH / 4; ++u) r4[u] = make_uint4(wd[4 * u], wd[4 * u + 1], wd[4 * u + 2], wd[4 * u + 3]);
     if (fa32) {

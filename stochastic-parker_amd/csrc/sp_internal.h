@@ -54,6 +54,15 @@ __host__ __device__ inline int rec3_src(int s) {
 // perpendicular diffusion coefficient.
 enum StepFeatures { FX_ACC = 1, FX_DPP = 2, FX_NLGC = 4 };
 
+// 3D node records are stored in bricks of 8 x 8 x 8 nodes (bricks x-fastest,
+// nodes x-fastest inside a brick), 32-56 KB each: the 8 corners of a cell share
+// one brick unless the cell straddles a brick face, so a gather touches one or
+// a few neighbouring pages instead of four rows up to a plane apart.
+__host__ __device__ inline long long brick_record(int ix, int iy, int iz, int bnx, int bny) {
+    const long long b = ((long long)(iz >> 3) * bny + (iy >> 3)) * bnx + (ix >> 3);
+    return (b << 9) | (long long)(((iz & 7) << 6) | ((iy & 7) << 3) | (ix & 7));
+}
+
 // Kernel-side copy of the configuration (passed by value).
 struct DevParams {
     int ndim;
@@ -69,6 +78,7 @@ struct DevParams {
     double drift1, drift2;
     int check_drift_2d, acc_region_flag, time_interp, precision;
     int ext_rec;                   // records carry the momentum-diffusion slots
+    int bnx, bny, bnz;             // 3D record bricks per axis
     double acc_region[6];
     uint32_t key0, key1, stream;
 };
@@ -154,6 +164,7 @@ struct MoverParams {
     const float* rec2;
     Counters* ctr;
     int nxg, nyg, nzg, nsteps_interval;
+    int bnx, bny;                 // 3D record bricks along x and y
     int flags;
     uint32_t key0, key1, stream;
     double t0, dtf, dt_fine, dt_limit;

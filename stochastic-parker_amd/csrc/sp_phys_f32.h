@@ -81,7 +81,8 @@ __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, fl
 #pragma unroll
     for (int q = 0; q < NCORN; ++q) {
         const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-        const long long node = base + i * sx + j * sy + k * sz;
+        const long long node = NDIM == 3 ? brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny)
+                                         : base + i * sx + j * sy + k * sz;
         const float4* r4 = reinterpret_cast<const float4*>(m.rec1 + node * W);
         float v[W];
 #pragma unroll
@@ -117,8 +118,6 @@ template <bool TINTERP>
 __device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& I, float rt,
                                             float* __restrict__ f) {
     constexpr int NC = 21;
-    const long long sx = 1, sy = m.nxg, sz = (long long)m.nxg * m.nyg;
-    const long long base = (long long)I.ix * sx + (long long)I.iy * sy + (long long)I.iz * sz;
     float g[TINTERP ? NC : 1];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -132,7 +131,7 @@ __device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& 
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-            const long long node = base + i * sx + j * sy + k * sz;
+            const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
             const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * kRec3DH);
             uint32_t wd[14];
 #pragma unroll

@@ -74,7 +74,8 @@ __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, do
 #pragma unroll
     for (int q = 0; q < NCORN; ++q) {
         const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-        const long long node = base + i + j * sy + k * sz;
+        const long long node = NDIM == 3 ? brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny)
+                                         : base + i + j * sy + k * sz;
         const float4* r4 = reinterpret_cast<const float4*>(m.rec1 + node * W);
         float v[W];
 #pragma unroll
