@@ -107,11 +107,11 @@ __device__ __forceinline__ void stage(uint16_t (*S)[kKT + kLdsPad], const uint16
     }
 }
 
-// 128 x BN output tile per 256-lane block; wave (wm, wn) owns 64 x BN/2 as
-// 2 x BN/64 MFMA tiles of 32 x 32.
-template <int EPI, int BN>
+// BM x BN output tile per 256-lane block; wave (wm, wn) owns BM/2 x BN/2 as
+// BM/64 x BN/64 MFMA tiles of 32 x 32.
+template <int EPI, int BN, int BM>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-    constexpr int BM = 128;
+    constexpr int TM = BM / 64;   // 32-row MFMA tiles per wave
     constexpr int TN = BN / 64;   // 32-column MFMA tiles per wave
     __shared__ __align__(16) uint16_t As[BM][kKT + kLdsPad];
     __shared__ __align__(16) uint16_t Bs[BN][kKT + kLdsPad];
@@ -121,9 +121,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     // split K: block z covers [kb, ke) and writes its own partial (EPI_PLAIN)
     const int kb = g.kchunk ? (int)blockIdx.z * g.kchunk : 0;
     const int ke = g.kchunk ? min(g.K, kb + g.kchunk) : g.K;
-    v16f acc[2][TN];
+    v16f acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = v16f{};
     for (int k0 = kb; k0 < ke; k0 += kKT) {
@@ -132,15 +132,15 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         __syncthreads();
 #pragma unroll
         for (int kc = 0; kc < kKT; kc += 16) {
-            v8bf a[2], b[TN];
+            v8bf a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const v8bf*>(&As[wm * 64 + i * 32 + (lane & 31)][kc + 8 * (lane >> 5)]);
+            for (int i = 0; i < TM; ++i)
+                a[i] = *reinterpret_cast<const v8bf*>(&As[wm * (BM / 2) + i * 32 + (lane & 31)][kc + 8 * (lane >> 5)]);
 #pragma unroll
             for (int j = 0; j < TN; ++j)
                 b[j] = *reinterpret_cast<const v8bf*>(&Bs[wn * (BN / 2) + j * 32 + (lane & 31)][kc + 8 * (lane >> 5)]);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
@@ -148,13 +148,13 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < TM; ++i) {
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             const int col = n0 + wn * (BN / 2) + j * 32 + (lane & 31);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const int row = m0 + wm * (BM / 2) + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
                 if (row >= g.M || col >= g.N) continue;
                 const long long o = (long long)row * g.N + col;
                 const float v = acc[i][j][r];
@@ -176,15 +176,22 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     }
 }
 
+// Tile shape: the largest of 128 x 128, 128 x 64, 64 x 64 that still gives
+// 256 blocks (one per CU); small problems (the 4096-row training batches)
+// take the smallest tiles so that more CUs work on them.  Every output element
+// accumulates K in the same order whatever the tile, so results are identical.
 template <int EPI>
 hipError_t gemm(const GemmArgs& g, hipStream_t s, int splits = 1) {
-    if (g.N > 64) {
-        const dim3 grid((unsigned)((g.N + 127) / 128), (unsigned)((g.M + 127) / 128), (unsigned)splits);
-        hipLaunchKernelGGL((gemm_bf16_kernel<EPI, 128>), grid, dim3(256), 0, s, g);
-    } else {
-        const dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 127) / 128), (unsigned)splits);
-        hipLaunchKernelGGL((gemm_bf16_kernel<EPI, 64>), grid, dim3(256), 0, s, g);
-    }
+    auto blocks = [&](int bm, int bn) {
+        return (long long)((g.N + bn - 1) / bn) * ((g.M + bm - 1) / bm) * splits;
+    };
+    auto launch = [&](auto kern, int bm, int bn) {
+        const dim3 grid((unsigned)((g.N + bn - 1) / bn), (unsigned)((g.M + bm - 1) / bm), (unsigned)splits);
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, g);
+    };
+    if (g.N > 64 && blocks(128, 128) >= 256) launch(gemm_bf16_kernel<EPI, 128, 128>, 128, 128);
+    else if (blocks(128, 64) >= 256) launch(gemm_bf16_kernel<EPI, 64, 128>, 128, 64);
+    else launch(gemm_bf16_kernel<EPI, 64, 64>, 64, 64);
     return hipGetLastError();
 }
 
