@@ -292,7 +292,9 @@ int sp_mlp_get_params(void* mlp, float* params);
 int sp_mlp_set_params(void* mlp, const float* params);
 /* y (rows x nout) = network(x (rows x nin)), row-major host arrays. */
 int sp_mlp_forward(void* mlp, const float* x, int64_t rows, float* y);
-/* Device time (HIP events) of the layer kernels of the last sp_mlp_forward. */
+/* Device time (HIP events) of the layer kernels of the last sp_mlp_forward on
+ * this network, or of the forward launches of the last ensemble call whose
+ * first member it is. */
 int sp_mlp_last_forward_ms(void* mlp, double* ms);
 /* Mean squared error of network(x) against t and its gradient with respect
  * to every parameter (rows <= batch_max); no update.  grads may be NULL. */
@@ -306,6 +308,10 @@ int sp_mlp_train_step(void* mlp, const float* x, const float* t, int64_t rows, d
  * (rows) = mean over members of the L1 distance of their spectrum from spec. */
 int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, int64_t rows,
                             double floor, float* spec, float* spread);
+/* As sp_mlp_ensemble_spectra with x, spec and spread in device memory (HBM);
+ * returns once the results are written. */
+int sp_mlp_ensemble_spectra_device(void* const* nets, int32_t members, const float* x,
+                                   int64_t rows, double floor, float* spec, float* spread);
 /* Copies a training set (rows x nin inputs, rows x nout targets) to the device. */
 int sp_mlp_set_data(void* mlp, const float* x, const float* t, int64_t rows);
 /* nsteps AdamW steps on device-drawn mini-batches of `batch` rows (with

@@ -10,10 +10,11 @@
 // and every back-propagated pre-activation gradient; all products accumulate
 // in FP32.
 //
-// GEMM: 128 x 128 (or 128 x 64) output tiles per 256-lane block, four waves of
-// 64 x 64 (64 x 32), K in steps of 32 staged through LDS as k-contiguous BF16
-// rows with 16-byte loads, v_mfma_f32_32x32x16_bf16 on 2 x 2 (2 x 1) tiles per
-// wave.  Operand lane layout (checked
+// GEMM (training): 128 x 128, 128 x 64 or 64 x 64 output tiles per 256-lane
+// block, four waves of a quarter tile each, K in steps of 32 staged through LDS
+// as k-contiguous BF16 rows with 16-byte loads, v_mfma_f32_32x32x16_bf16 on the
+// wave's 32 x 32 tiles.  Inference of the usual shapes runs all layers in one
+// fused kernel instead (mlp_infer_kernel).  Operand lane layout (checked
 // on gfx950): lane l holds row/column l % 32, k = 8 (l / 32) .. + 7; result
 // register j of lane l is row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32.
 #include <hip/hip_runtime.h>
@@ -37,14 +38,21 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 constexpr int kKT = 32;
 constexpr int kLdsPad = 8;   // row stride 40 halves = 80 B
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ uint16_t bf16_rne(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Two values rounded as bf16_rne, lo in the low half (one v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t bf16x2_rne(float lo, float hi) {
+    typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((v2f){lo, hi}, v2bf));
+}
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+// 1 / (1 + e^-z) from the hardware exp2 and reciprocal (about 1 ulp each,
+// far below the BF16 rounding that follows); z -> -inf gives 0, z -> inf 1.
+__device__ __forceinline__ float sigmoid(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
 
 enum Epilogue : int {
     EPI_HIDDEN = 0,   // z = acc + b: store z (FP32) and bf16(silu(z))
@@ -195,6 +203,136 @@ hipError_t gemm(const GemmArgs& g, hipStream_t s, int splits = 1) {
     return hipGetLastError();
 }
 
+// ---- fused inference ------------------------------------------------------
+// One block carries 128 rows through every layer of one network: the BF16
+// activations stay in LDS (128 x (H + 8) halves, 66 KiB at H = 256, two
+// blocks per CU), weights come from L2 in MFMA fragment order, and only the
+// FP32 output row goes to HBM.  Each output element accumulates K in steps of 16 in the same
+// order as gemm_bf16_kernel, with the same bias, SiLU and rounding, so the
+// outputs equal the layer-by-layer GEMM path bit for bit.
+constexpr int kFusedRows = 128;
+constexpr int kFusedMaxMembers = 8;
+constexpr int kFusedMaxLayers = 16;
+constexpr int kFusedKin = 16;     // the input layer's K, zero-padded
+
+// Fragment order of a K x N weight matrix (K a multiple of 16): for 32-column
+// tile ct and k-step ks, the 64 lanes' B operands of one MFMA lie in 1 KiB,
+// lane l holding column 32 ct + l % 32, k = 16 ks + 8 (l / 32) .. + 7.
+__host__ __device__ __forceinline__ long long frag_index(int k, int c, int K) {
+    const int ct = c >> 5, ks = k >> 4, ln = (c & 31) + 32 * ((k >> 3) & 1);
+    return (((long long)ct * (K >> 4) + ks) * 64 + ln) * 8 + (k & 7);
+}
+
+// out (Kpad x N, fragment order) = W (K x N row-major BF16), zero for k >= K.
+__global__ void frag_kernel(const uint16_t* w, int K, int N, int Kpad, uint16_t* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)Kpad * N) return;
+    const int k = (int)(i / N), c = (int)(i % N);
+    out[frag_index(k, c, Kpad)] = k < K ? w[(long long)k * N + c] : (uint16_t)0;
+}
+
+struct FusedArgs {
+    const float* x;                          // rows x nin, FP32
+    int rows, nin, nhh;                      // nhh: hidden-to-hidden layers
+    const uint16_t* wf[kFusedMaxMembers];    // fragment-ordered weights per member
+    const float* p[kFusedMaxMembers];        // FP32 parameter vector per member
+    float* y[kFusedMaxMembers];              // rows x nout outputs per member
+    long long boff[kFusedMaxLayers];         // bias offsets in the parameter vector
+};
+
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+
+// Hidden layer K -> H on the block's activations (in place), computed as
+// Z^T = W^T X^T: the weights are the A operand, the activations the B
+// operand, so a lane's result registers hold four consecutive features of one
+// row and go back to LDS as 8-byte writes.  Wave w (of H / 32) computes the
+// 32-feature tile w for all four 32-row tiles.
+template <int H, int K>
+__device__ __forceinline__ void fused_hidden(uint16_t* act, const uint16_t* wl, const float* bias,
+                                             int lane, int w) {
+    constexpr int LD = H + 8;
+    const v8bf* wv = reinterpret_cast<const v8bf*>(wl);
+    v16f acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = v16f{};
+#pragma unroll 4
+    for (int ks = 0; ks < K / 16; ++ks) {
+        const v8bf a = wv[(w * (K / 16) + ks) * 64 + lane];
+        v8bf b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            b[j] = *reinterpret_cast<const v8bf*>(&act[(j * 32 + (lane & 31)) * LD + ks * 16 + 8 * (lane >> 5)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+    }
+    __syncthreads();   // every wave has read the layer's input
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int m = w * 32 + 8 * g + 4 * (lane >> 5);   // first of 4 features
+        const float b0 = bias[m], b1 = bias[m + 1], b2 = bias[m + 2], b3 = bias[m + 3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint2 v;
+            v.x = bf16x2_rne(silu(acc[j][4 * g] + b0), silu(acc[j][4 * g + 1] + b1));
+            v.y = bf16x2_rne(silu(acc[j][4 * g + 2] + b2), silu(acc[j][4 * g + 3] + b3));
+            *reinterpret_cast<uint2*>(&act[(j * 32 + (lane & 31)) * LD + m]) = v;
+        }
+    }
+    __syncthreads();
+}
+
+// Network nin -> H (x 1 + nhh, SiLU) -> NOUT for rows [128 bx, 128 bx + 128)
+// of member blockIdx.y, H / 32 waves per block.  In the output layer the
+// 32 x 32 tiles (NOUT / 32 feature tiles x 4 row tiles) go round-robin to the
+// waves, and each lane writes four outputs of its row per 16-byte store.
+template <int H, int NOUT>
+__global__ void __launch_bounds__(2 * H, 2) mlp_infer_kernel(FusedArgs a) {
+    constexpr int R = kFusedRows, LD = H + 8, W = H / 32, T = (NOUT / 32) * 4, TPW = (T + W - 1) / W;
+    __shared__ __align__(16) uint16_t act[R * LD];
+    const int m = blockIdx.y;
+    const long long r0 = (long long)blockIdx.x * R;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint16_t* wf = a.wf[m];
+    const float* p = a.p[m];
+    for (int i = threadIdx.x; i < R * kFusedKin; i += 64 * W) {
+        const int r = i / kFusedKin, c = i % kFusedKin;
+        const float v = (r0 + r < a.rows && c < a.nin) ? a.x[(r0 + r) * a.nin + c] : 0.0f;
+        act[r * LD + c] = bf16_rne(v);
+    }
+    __syncthreads();
+    fused_hidden<H, kFusedKin>(act, wf, p + a.boff[0], lane, w);
+    const uint16_t* wl = wf + H * kFusedKin;
+    for (int l = 1; l <= a.nhh; ++l, wl += H * H) fused_hidden<H, H>(act, wl, p + a.boff[l], lane, w);
+    const v8bf* wv = reinterpret_cast<const v8bf*>(wl);
+    const float* bias = p + a.boff[a.nhh + 1];
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+        const int t = w + W * q;
+        if (t >= T) break;   // uniform per wave
+        const int nt = t & 3, mt = t >> 2;
+        v16f acc = v16f{};
+#pragma unroll 4
+        for (int ks = 0; ks < H / 16; ++ks) {
+            const v8bf b = *reinterpret_cast<const v8bf*>(&act[(nt * 32 + (lane & 31)) * LD + ks * 16 + 8 * (lane >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[(mt * (H / 16) + ks) * 64 + lane], b, acc, 0, 0, 0);
+        }
+        const long long row = r0 + nt * 32 + (lane & 31);
+        if (row < a.rows) {
+            float* y = a.y[m] + row * NOUT;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int c = mt * 32 + 8 * g + 4 * (lane >> 5);
+                float4 v;
+                v.x = acc[4 * g] + bias[c];
+                v.y = acc[4 * g + 1] + bias[c + 1];
+                v.z = acc[4 * g + 2] + bias[c + 2];
+                v.w = acc[4 * g + 3] + bias[c + 3];
+                *reinterpret_cast<float4*>(&y[c]) = v;
+            }
+        }
+    }
+}
+
 // out[i] = sum over s < splits of part[s n + i], in order of s (deterministic).
 __global__ void split_sum_kernel(const float* part, int splits, long long n, float* out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -260,23 +398,32 @@ struct EnsembleArgs {
     float* spread;
 };
 
-__global__ void ensemble_kernel(EnsembleArgs e, int rows) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+// One wave per row: lane c takes columns c, c + 64, ...; the spread is the
+// wave's sum of the lanes' member distances (a fixed butterfly order).
+__global__ void __launch_bounds__(256) ensemble_kernel(EnsembleArgs e, int rows) {
+    const int lane = threadIdx.x & 63;
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;   // per wave: r is uniform across its lanes
     const float inv = 1.0f / (float)e.members;
     float dist = 0.0f;
-    for (int c = 0; c < e.nout; ++c) {
-        float s[8];
+    for (int c = lane; c < e.nout; c += 64) {
+        float sv[8];
         float mean = 0.0f;
-        for (int i = 0; i < e.members; ++i) {
-            s[i] = fmaxf(exp10f(e.y[i][(long long)r * e.nout + c]) - e.floor, 0.0f);
-            mean += s[i];
-        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < e.members) {
+                sv[i] = fmaxf(exp10f(e.y[i][r * e.nout + c]) - e.floor, 0.0f);
+                mean += sv[i];
+            }
         mean *= inv;
-        for (int i = 0; i < e.members; ++i) dist += fabsf(s[i] - mean);
-        e.spec[(long long)r * e.nout + c] = mean;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (i < e.members) dist += fabsf(sv[i] - mean);
+        e.spec[r * e.nout + c] = mean;
     }
-    e.spread[r] = dist * inv;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dist += __shfl_xor(dist, o, 64);
+    if (lane == 0) e.spread[r] = dist * inv;
 }
 
 // out[0] = scale * sum(part[0 .. np)), one block.
@@ -418,10 +565,12 @@ struct Mlp {
     uint32_t* bidx = nullptr;
     float* wpart = nullptr;             // split-K partial weight gradients
     hipEvent_t ev[2] = {nullptr, nullptr};
-    double fwd_ms = 0.0;                // device time of the last sp_mlp_forward's layers
+    double fwd_ms = 0.0;                // device time of the last forward's layer kernels
     sp::mlp::TrainState* ts = nullptr;  // device-side training loop state
     float* ens_spec = nullptr;          // ensemble spectra and spreads (batch_max rows)
     float* ens_spread = nullptr;
+    uint16_t* wf = nullptr;             // fragment-ordered BF16 weights (fused inference)
+    std::vector<hipEvent_t> tev;        // per-chunk timing events of the fused launches
     long long step = 0;                 // AdamW step count
 };
 
@@ -440,8 +589,10 @@ void destroy(Mlp* n) {
     for (auto* q : n->z) dfree(q);
     dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
     dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
-    dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread); dfree(n->ts);
+    dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread); dfree(n->ts); dfree(n->wf);
     for (auto& e : n->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : n->tev)
         if (e) (void)hipEventDestroy(e);
     if (n->st) (void)hipStreamDestroy(n->st);
     delete n;
@@ -565,15 +716,16 @@ int adamw(Mlp* n, double lr) {
 }
 
 // Host rows -> xh[0] (bf16) and, when given, targets -> n->t.
+// x and t may be host or device pointers.
 int load_rows(Mlp* n, const float* x, const float* t, int rows) {
     const long long nx = (long long)rows * n->cfg.nin;
-    MLP_HIP(hipMemcpyAsync(n->xin, x, sizeof(float) * (size_t)nx, hipMemcpyHostToDevice, n->st));
+    MLP_HIP(hipMemcpyAsync(n->xin, x, sizeof(float) * (size_t)nx, hipMemcpyDefault, n->st));
     hipLaunchKernelGGL(sp::mlp::to_bf16_kernel, dim3(blocks(nx)), dim3(256), 0, n->st, n->xin,
                        n->xh[0], nx);
     MLP_HIP(hipGetLastError());
     if (t)
         MLP_HIP(hipMemcpyAsync(n->t, t, sizeof(float) * (size_t)rows * n->cfg.nout,
-                               hipMemcpyHostToDevice, n->st));
+                               hipMemcpyDefault, n->st));
     return SP_OK;
 }
 
@@ -584,6 +736,167 @@ Mlp* live(void* q, int& rc) {
     }
     rc = SP_OK;
     return static_cast<Mlp*>(q);
+}
+
+// Shapes the fused inference kernel is built for; SP_MLP_UNFUSED=1 sends
+// inference through the layer-by-layer GEMMs instead.
+bool fused_ok(const Mlp* n) {
+    const char* env = std::getenv("SP_MLP_UNFUSED");
+    if (env && env[0] == '1') return false;
+    const sp_mlp_config& c = n->cfg;
+    return c.nin <= sp::mlp::kFusedKin && (c.nhidden == 128 || c.nhidden == 256) &&
+           (c.nout == 32 || c.nout == 64 || c.nout == 128) && c.nlayers >= 1 &&
+           n->L <= sp::mlp::kFusedMaxLayers;
+}
+
+bool same_shape(const Mlp* a, const Mlp* b) {
+    return a->cfg.nin == b->cfg.nin && a->cfg.nhidden == b->cfg.nhidden &&
+           a->cfg.nlayers == b->cfg.nlayers && a->cfg.nout == b->cfg.nout;
+}
+
+// Rewrites n->wf (fragment order, input layer padded to K = 16) from the BF16
+// weights on stream s.
+int refresh_frag(Mlp* n, hipStream_t s) {
+    long long total = 0;
+    for (int l = 0; l < n->L; ++l) total += (long long)(l == 0 ? sp::mlp::kFusedKin : n->K[l]) * n->N[l];
+    if (!n->wf) MLP_HIP(hipMalloc(&n->wf, 2 * (size_t)total));
+    long long off = 0;
+    for (int l = 0; l < n->L; ++l) {
+        const int kpad = l == 0 ? sp::mlp::kFusedKin : n->K[l];
+        const long long cnt = (long long)kpad * n->N[l];
+        hipLaunchKernelGGL(sp::mlp::frag_kernel, dim3(blocks(cnt)), dim3(256), 0, s,
+                           n->pb + n->woff[l], n->K[l], n->N[l], kpad, n->wf + off);
+        MLP_HIP(hipGetLastError());
+        off += cnt;
+    }
+    return SP_OK;
+}
+
+template <int H, int NOUT>
+void launch_fused(const sp::mlp::FusedArgs& a, int members, hipStream_t s) {
+    const dim3 grid((unsigned)((a.rows + sp::mlp::kFusedRows - 1) / sp::mlp::kFusedRows), (unsigned)members);
+    hipLaunchKernelGGL((sp::mlp::mlp_infer_kernel<H, NOUT>), grid, dim3(2 * H), 0, s, a);
+}
+
+// ms[i]->y (rows x nout) = network i (x), all members in one launch on s.
+// Needs fused_ok and equal shapes, and every member's wf refreshed.
+int fused_forward(Mlp* const* ms, int members, const float* x, int rows, hipStream_t s) {
+    const Mlp* h = ms[0];
+    sp::mlp::FusedArgs a{};
+    a.x = x;
+    a.rows = rows;
+    a.nin = h->cfg.nin;
+    a.nhh = h->L - 2;
+    for (int i = 0; i < members; ++i) {
+        a.wf[i] = ms[i]->wf;
+        a.p[i] = ms[i]->p;
+        a.y[i] = ms[i]->y;
+    }
+    for (int l = 0; l < h->L; ++l) a.boff[l] = h->boff[l];
+    if (rows == 0) return SP_OK;
+    const int H = h->cfg.nhidden, NO = h->cfg.nout;
+    if (H == 256 && NO == 64) launch_fused<256, 64>(a, members, s);
+    else if (H == 256 && NO == 32) launch_fused<256, 32>(a, members, s);
+    else if (H == 256 && NO == 128) launch_fused<256, 128>(a, members, s);
+    else if (H == 128 && NO == 64) launch_fused<128, 64>(a, members, s);
+    else if (H == 128 && NO == 32) launch_fused<128, 32>(a, members, s);
+    else launch_fused<128, 128>(a, members, s);
+    MLP_HIP(hipGetLastError());
+    return SP_OK;
+}
+
+int ensure_events(Mlp* n, size_t count) {
+    while (n->tev.size() < count) {
+        hipEvent_t e = nullptr;
+        MLP_HIP(hipEventCreate(&e));
+        n->tev.push_back(e);
+    }
+    return SP_OK;
+}
+
+// Ensemble spectra of rows of x (host or device pointers throughout; dev
+// selects which).  Chunks of batch_max rows; with the fused kernel all
+// members of a chunk run in one launch on the first member's stream.
+int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, double floor,
+             float* spec, float* spread, bool dev) {
+    if (!nets || members < 1 || members > 8) return sp::set_error(SP_ERR_ARG, "members must be in [1, 8]");
+    if (rows < 0 || (rows > 0 && (!x || !spec || !spread))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
+    std::vector<Mlp*> ms;
+    int rc;
+    for (int i = 0; i < members; ++i) {
+        Mlp* n = live(nets[i], rc);
+        if (!n) return rc;
+        if (n->cfg.nin != static_cast<Mlp*>(nets[0])->cfg.nin || n->cfg.nout != static_cast<Mlp*>(nets[0])->cfg.nout)
+            return sp::set_error(SP_ERR_ARG, "ensemble members differ in nin or nout");
+        ms.push_back(n);
+    }
+    Mlp* h = ms[0];
+    int chunk = h->bmax;
+    bool fused = true;
+    for (auto* n : ms) {
+        chunk = std::min(chunk, n->bmax);
+        fused = fused && fused_ok(n) && same_shape(n, h);
+    }
+    const int nin = h->cfg.nin, nout = h->cfg.nout;
+    if (!dev && !h->ens_spec) {
+        MLP_HIP(hipMalloc(&h->ens_spec, 4 * (size_t)h->bmax * nout));
+        MLP_HIP(hipMalloc(&h->ens_spread, 4 * (size_t)h->bmax));
+    }
+    const int64_t nchunks = (rows + chunk - 1) / chunk;
+    if (fused) {
+        for (auto* n : ms) {
+            MLP_HIP(hipStreamSynchronize(n->st));
+            rc = refresh_frag(n, h->st);
+            if (rc != SP_OK) return rc;
+        }
+        rc = ensure_events(h, 2 * (size_t)nchunks);
+        if (rc != SP_OK) return rc;
+    }
+    for (int64_t c = 0; c < nchunks; ++c) {
+        const int64_t r0 = c * chunk;
+        const int r = (int)std::min<int64_t>(chunk, rows - r0);
+        if (fused) {
+            const float* xc = x + r0 * nin;
+            if (!dev) {
+                MLP_HIP(hipMemcpyAsync(h->xin, xc, 4 * (size_t)r * nin, hipMemcpyHostToDevice, h->st));
+                xc = h->xin;
+            }
+            MLP_HIP(hipEventRecord(h->tev[(size_t)(2 * c)], h->st));
+            rc = fused_forward(ms.data(), members, xc, r, h->st);
+            if (rc != SP_OK) return rc;
+            MLP_HIP(hipEventRecord(h->tev[(size_t)(2 * c + 1)], h->st));
+        } else {
+            for (auto* n : ms) {
+                rc = load_rows(n, x + r0 * nin, nullptr, r);
+                if (rc == SP_OK) rc = forward(n, r, false);
+                if (rc != SP_OK) return rc;
+                MLP_HIP(hipStreamSynchronize(n->st));
+            }
+        }
+        sp::mlp::EnsembleArgs e{};
+        for (int i = 0; i < members; ++i) e.y[i] = ms[(size_t)i]->y;
+        e.members = members;
+        e.nout = nout;
+        e.floor = (float)floor;
+        e.spec = dev ? spec + r0 * nout : h->ens_spec;
+        e.spread = dev ? spread + r0 : h->ens_spread;
+        hipLaunchKernelGGL(sp::mlp::ensemble_kernel, dim3(blocks(r, 4)), dim3(256), 0, h->st, e, r);
+        MLP_HIP(hipGetLastError());
+        if (!dev) {
+            MLP_HIP(hipMemcpyAsync(spec + r0 * nout, h->ens_spec, 4 * (size_t)r * nout, hipMemcpyDeviceToHost, h->st));
+            MLP_HIP(hipMemcpyAsync(spread + r0, h->ens_spread, 4 * (size_t)r, hipMemcpyDeviceToHost, h->st));
+            MLP_HIP(hipStreamSynchronize(h->st));
+        }
+    }
+    MLP_HIP(hipStreamSynchronize(h->st));
+    h->fwd_ms = 0.0;
+    if (fused)
+        for (int64_t c = 0; c < nchunks; ++c) {
+            float ms_ = 0.f;
+            MLP_HIP(hipEventElapsedTime(&ms_, h->tev[(size_t)(2 * c)], h->tev[(size_t)(2 * c + 1)]));
+            h->fwd_ms += ms_;
+        }
+    return SP_OK;
 }
 
 }  // namespace
@@ -709,12 +1022,17 @@ int sp_mlp_forward(void* q, const float* x, int64_t rows, float* y) {
     if (!n) return rc;
     if (rows < 0 || (rows > 0 && (!x || !y))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
     n->fwd_ms = 0.0;
+    const bool fused = fused_ok(n);
+    if (fused) {
+        rc = refresh_frag(n, n->st);
+        if (rc != SP_OK) return rc;
+    }
     for (int64_t r0 = 0; r0 < rows; r0 += n->bmax) {
         const int r = (int)std::min<int64_t>(n->bmax, rows - r0);
         rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
         if (rc != SP_OK) return rc;
         MLP_HIP(hipEventRecord(n->ev[0], n->st));
-        rc = forward(n, r, false);
+        rc = fused ? fused_forward(&n, 1, n->xin, r, n->st) : forward(n, r, false);
         if (rc != SP_OK) return rc;
         MLP_HIP(hipEventRecord(n->ev[1], n->st));
         MLP_HIP(hipMemcpyAsync(y + r0 * n->cfg.nout, n->y, 4 * (size_t)r * n->cfg.nout,
@@ -739,47 +1057,12 @@ int sp_mlp_last_forward_ms(void* q, double* ms) {
 
 int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, int64_t rows,
                             double floor, float* spec, float* spread) {
-    if (!nets || members < 1 || members > 8) return sp::set_error(SP_ERR_ARG, "members must be in [1, 8]");
-    if (rows < 0 || (rows > 0 && (!x || !spec || !spread))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
-    std::vector<Mlp*> ms;
-    int rc;
-    for (int i = 0; i < members; ++i) {
-        Mlp* n = live(nets[i], rc);
-        if (!n) return rc;
-        if (n->cfg.nin != static_cast<Mlp*>(nets[0])->cfg.nin || n->cfg.nout != static_cast<Mlp*>(nets[0])->cfg.nout)
-            return sp::set_error(SP_ERR_ARG, "ensemble members differ in nin or nout");
-        ms.push_back(n);
-    }
-    Mlp* h = ms[0];
-    int chunk = h->bmax;
-    for (auto* n : ms) chunk = std::min(chunk, n->bmax);
-    const int nout = h->cfg.nout;
-    if (!h->ens_spec) {
-        MLP_HIP(hipMalloc(&h->ens_spec, 4 * (size_t)h->bmax * nout));
-        MLP_HIP(hipMalloc(&h->ens_spread, 4 * (size_t)h->bmax));
-    }
-    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
-        const int r = (int)std::min<int64_t>(chunk, rows - r0);
-        for (auto* n : ms) {
-            rc = load_rows(n, x + r0 * n->cfg.nin, nullptr, r);
-            if (rc == SP_OK) rc = forward(n, r, false);
-            if (rc != SP_OK) return rc;
-            MLP_HIP(hipStreamSynchronize(n->st));
-        }
-        sp::mlp::EnsembleArgs e{};
-        for (int i = 0; i < members; ++i) e.y[i] = ms[(size_t)i]->y;
-        e.members = members;
-        e.nout = nout;
-        e.floor = (float)floor;
-        e.spec = h->ens_spec;
-        e.spread = h->ens_spread;
-        hipLaunchKernelGGL(sp::mlp::ensemble_kernel, dim3(blocks(r)), dim3(256), 0, h->st, e, r);
-        MLP_HIP(hipGetLastError());
-        MLP_HIP(hipMemcpyAsync(spec + r0 * nout, h->ens_spec, 4 * (size_t)r * nout, hipMemcpyDeviceToHost, h->st));
-        MLP_HIP(hipMemcpyAsync(spread + r0, h->ens_spread, 4 * (size_t)r, hipMemcpyDeviceToHost, h->st));
-        MLP_HIP(hipStreamSynchronize(h->st));
-    }
-    return SP_OK;
+    return ensemble(nets, members, x, rows, floor, spec, spread, false);
+}
+
+int sp_mlp_ensemble_spectra_device(void* const* nets, int32_t members, const float* x, int64_t rows,
+                                   double floor, float* spec, float* spread) {
+    return ensemble(nets, members, x, rows, floor, spec, spread, true);
 }
 
 int sp_mlp_gradients(void* q, const float* x, const float* t, int64_t rows, float* grads, double* loss) {

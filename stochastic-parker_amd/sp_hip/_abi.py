@@ -163,6 +163,8 @@ EXPORTS = {
     "sp_mlp_train_step": [_P, _F, _F, C.c_int64, C.c_double, _D],
     "sp_mlp_ensemble_spectra": [C.POINTER(C.c_void_p), C.c_int32, _F, C.c_int64, C.c_double, _F,
                                 _F],
+    "sp_mlp_ensemble_spectra_device": [C.POINTER(C.c_void_p), C.c_int32, C.c_void_p, C.c_int64,
+                                       C.c_double, C.c_void_p, C.c_void_p],
     "sp_mlp_set_data": [_P, _F, _F, C.c_int64],
     "sp_mlp_train": [_P, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_uint64, _F],
 }

@@ -255,7 +255,28 @@ class Surrogate:
         f = Mlp._f
         check(lib, "sp_mlp_ensemble_spectra",
               lib.sp_mlp_ensemble_spectra(hs, len(self.nets), f(x), n, FLOOR, f(spec), f(spread)))
-        return spec, self.c0 + self.c1 * spread
+        return spec, self.estimate(spread)
+
+    def predict_device(self, x, spec, spread, rows=None):
+        """predict() on rows held in device memory: x (rows, nin) float32,
+        spec (rows, nout) and spread (rows,) float32 are device arrays
+        (sp_hip.device.DeviceArray) that receive the members' mean spectrum
+        and their L1 spread; the error estimate is estimate(spread)."""
+        nin, nout = self.nets[0].shape[0], self.nets[0].shape[3]
+        n = x.shape[0] if rows is None else int(rows)
+        if (len(x.shape) != 2 or x.shape[1] != nin or x.shape[0] < n or spec.shape[0] < n or
+                spec.shape[1:] != (nout,) or spread.shape[0] < n or
+                any(a.dtype != np.float32 for a in (x, spec, spread))):
+            raise ValueError("device arrays of the wrong shape or dtype")
+        hs = (C.c_void_p * len(self.nets))(*[m.h for m in self.nets])
+        lib = self.nets[0].lib
+        check(lib, "sp_mlp_ensemble_spectra_device",
+              lib.sp_mlp_ensemble_spectra_device(hs, len(self.nets), x.ptr, n, FLOOR, spec.ptr,
+                                                 spread.ptr))
+
+    def estimate(self, spread):
+        """Calibrated error estimate c0 + c1 * spread."""
+        return self.c0 + self.c1 * np.asarray(spread)
 
     def calibrate(self, x, f_true, quantile=0.9):
         """Fits the error estimate on held-out rows: err ~ c0 + c1 spread by

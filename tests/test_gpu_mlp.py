@@ -109,3 +109,37 @@ def test_training_graph_replays_match_eager(monkeypatch):
             np.testing.assert_array_equal(lg, le)
             np.testing.assert_array_equal(g.get_params(), e.get_params())
         assert lg[-1] < lg[0]
+
+
+@pytest.mark.parametrize("shape", [SHAPE, (5, 128, 2, 32), (6, 256, 1, 128)])
+def test_fused_inference_equals_layer_gemms(monkeypatch, shape):
+    """The one-kernel inference path gives the layer-by-layer GEMM path's
+    outputs bit for bit (ragged row counts, several chunks)."""
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1.0, 1.0, (1000, shape[0])).astype(np.float32)
+    with Mlp(*shape, seed=12, batch_max=384) as net:
+        y_fused = net.forward(x)
+        fwd_ms = net.last_forward_ms()
+        monkeypatch.setenv("SP_MLP_UNFUSED", "1")
+        y_gemm = net.forward(x)
+    assert fwd_ms > 0
+    np.testing.assert_array_equal(y_fused, y_gemm)
+
+
+def test_ensemble_device_arrays_equal_host_arrays(monkeypatch):
+    from sp_hip.device import DeviceArray
+    from sp_hip.surrogate import Surrogate
+    x, _ = _data(2500, seed=8)
+    with Surrogate(members=2, seed=4, batch_max=1024) as s:
+        s.c0, s.c1 = 0.01, 2.0
+        spec_h, est_h = s.predict(x)
+        with DeviceArray.from_host(x) as xd, DeviceArray((len(x), SHAPE[3])) as sd, \
+                DeviceArray(len(x)) as rd:
+            s.predict_device(xd, sd, rd)
+            spec_d, spread_d = sd.to_host(), rd.to_host()
+        monkeypatch.setenv("SP_MLP_UNFUSED", "1")
+        spec_g, est_g = s.predict(x)
+    np.testing.assert_array_equal(spec_d, spec_h)
+    np.testing.assert_array_equal(s.estimate(spread_d), est_h)
+    np.testing.assert_array_equal(spec_g, spec_h)
+    np.testing.assert_array_equal(est_g, est_h)
