@@ -51,8 +51,9 @@ __global__ void __launch_bounds__(kBlock) inject_kernel(DevParams prm, Ptl P, lo
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nptl) return;
     long long slot = start + i;
-    if (slot >= nmax) {
-        // past nptl_max every new particle lands in the last slot; the last one stays
+    if (slot >= nmax - 1) {
+        // past nptl_max every new particle lands in the last slot, so that slot
+        // keeps the batch's last particle; no other thread writes it
         if (i != nptl - 1) return;
         slot = nmax - 1;
     }
@@ -254,7 +255,8 @@ __global__ void __launch_bounds__(kBlock) threshold_inject_kernel(DevParams prm,
     if (i >= nptl) return;
     long long slot = start + i;
     bool store = ta.commit != 0;
-    if (slot >= nmax) {
+    if (slot >= nmax - 1) {
+        // the last slot keeps the batch's last particle (as in inject_kernel)
         if (i != nptl - 1) store = false;
         slot = nmax - 1;
     }
