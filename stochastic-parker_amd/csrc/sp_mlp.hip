@@ -343,11 +343,20 @@ __global__ void __launch_bounds__(2 * H, 2) mlp_infer_kernel(FusedArgs a) {
 }
 
 // out[i] = sum over s < splits of part[s n + i], in order of s (deterministic).
+// Loads go out eight at a time; the additions keep the order of s.
 __global__ void split_sum_kernel(const float* part, int splits, long long n, float* out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float acc = 0.0f;
-    for (int s = 0; s < splits; ++s) acc += part[(long long)s * n + i];
+    int s = 0;
+    for (; s + 8 <= splits; s += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(long long)(s + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; s < splits; ++s) acc += part[(long long)s * n + i];
     out[i] = acc;
 }
 
@@ -456,18 +465,43 @@ __global__ void zero_kernel(float* a, long long n) {
 }
 
 // Column sums of a BF16 M x N matrix (bias gradients), one block per column.
-__global__ void __launch_bounds__(256) colsum_kernel(const uint16_t* gmat, int M, int N, float* out) {
-    __shared__ float red[256];
-    const int c = blockIdx.x;
-    float acc = 0.0f;
-    for (int r = threadIdx.x; r < M; r += 256) acc += bf16_to_f(gmat[(long long)r * N + c]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+// Partial column sums of a rows x N BF16 matrix: block (bx, by) sums columns
+// [64 bx, 64 bx + 64) over rows [by chunk, (by + 1) chunk).  A thread takes
+// eight columns (one 16-byte load per row) and every 32nd row; the 32 row
+// phases are then added in a fixed order.  part[by N + c].
+constexpr int kColParts = 16;
+__global__ void __launch_bounds__(256) colsum_part_kernel(const uint16_t* gmat, int M, int N,
+                                                          int chunk, float* part) {
+    __shared__ float red[32][65];
+    const int cl = (threadIdx.x & 7) * 8, c0 = blockIdx.x * 64 + cl, ph = threadIdx.x >> 3;
+    const int r0 = blockIdx.y * chunk, r1 = min(M, r0 + chunk);
+    const bool vec = (N % 8) == 0 && c0 + 8 <= N;
+    float acc[8] = {};
+#pragma unroll 8
+    for (int r = r0 + ph; r < r1; r += 32) {
+        if (vec) {
+            const uint4 v = *reinterpret_cast<const uint4*>(gmat + (long long)r * N + c0);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[2 * u] += __uint_as_float(w[u] << 16);
+                acc[2 * u + 1] += __uint_as_float(w[u] & 0xFFFF0000u);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (c0 + u < N) acc[u] += bf16_to_f(gmat[(long long)r * N + c0 + u]);
+        }
     }
-    if (threadIdx.x == 0) out[c] = red[0];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) red[ph][cl + u] = acc[u];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int c = blockIdx.x * 64 + threadIdx.x;
+        float t = 0.0f;
+        for (int q = 0; q < 32; ++q) t += red[q][threadIdx.x];
+        if (c < N) part[(long long)blockIdx.y * N + c] = t;
+    }
 }
 
 // State of a device-side training loop: t counts the AdamW steps taken; the
@@ -695,8 +729,16 @@ int backward(Mlp* n, int rows, float* loss_out, const sp::mlp::TrainState* ts = 
         hipLaunchKernelGGL(sp::mlp::split_sum_kernel, dim3(blocks(mn)), dim3(256), 0, n->st,
                            n->wpart, splits, mn, n->g + n->woff[l]);
         MLP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(sp::mlp::colsum_kernel, dim3((unsigned)n->N[l]), dim3(256), 0, n->st,
-                           n->gh[cur], rows, n->N[l], n->g + n->boff[l]);
+        // bias gradient: column sums of the output gradient in kColParts row
+        // chunks, then summed in order of the chunk
+        const int chunk = (rows + sp::mlp::kColParts - 1) / sp::mlp::kColParts;
+        float* bpart = n->wpart + (long long)splits * mn;
+        hipLaunchKernelGGL(sp::mlp::colsum_part_kernel,
+                           dim3((unsigned)((n->N[l] + 63) / 64), (unsigned)sp::mlp::kColParts), dim3(256),
+                           0, n->st, n->gh[cur], rows, n->N[l], chunk, bpart);
+        MLP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sp::mlp::split_sum_kernel, dim3(blocks(n->N[l])), dim3(256), 0, n->st,
+                           bpart, sp::mlp::kColParts, (long long)n->N[l], n->g + n->boff[l]);
         MLP_HIP(hipGetLastError());
         if (l > 0) {
             sp::mlp::GemmArgs d{};
@@ -970,7 +1012,8 @@ int sp_mlp_create(const sp_mlp_config* cfg, void** out) {
     if (e == hipSuccess) e = hipMalloc(&n->bidx, 4 * B);
     long long wmaxmn = 0;
     for (int l = 0; l < n->L; ++l) wmaxmn = std::max(wmaxmn, (long long)n->K[l] * n->N[l]);
-    if (e == hipSuccess) e = hipMalloc(&n->wpart, 4 * (size_t)kMaxSplits * (size_t)wmaxmn);
+    if (e == hipSuccess)
+        e = hipMalloc(&n->wpart, 4 * ((size_t)kMaxSplits * (size_t)wmaxmn + (size_t)sp::mlp::kColParts * wmax));
     if (e == hipSuccess) e = hipMalloc(&n->loss_hist, 4 * 16);
     if (e != hipSuccess) return fail_all(e, "device allocation for the network");
     n->loss_cap = 16;
