@@ -416,32 +416,64 @@ struct EnsembleArgs {
     float* spread;
 };
 
-// One wave per row: lane c takes columns c, c + 64, ...; the spread is the
-// wave's sum of the lanes' member distances (a fixed butterfly order).
+// Sixteen lanes per row, four consecutive columns per lane and pass (16-byte
+// loads and stores when nout % 4 == 0); a lane's member distances add up in
+// column order, then the 16 lanes' sums by a fixed butterfly.
+__device__ __forceinline__ void ensemble_column(const EnsembleArgs& e, long long o, float inv, float& dist) {
+    float sv[8];
+    float mean = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (i < e.members) {
+            sv[i] = fmaxf(exp10f(e.y[i][o]) - e.floor, 0.0f);
+            mean += sv[i];
+        }
+    mean *= inv;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if (i < e.members) dist += fabsf(sv[i] - mean);
+    e.spec[o] = mean;
+}
+
 __global__ void __launch_bounds__(256) ensemble_kernel(EnsembleArgs e, int rows) {
-    const int lane = threadIdx.x & 63;
-    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= rows) return;   // per wave: r is uniform across its lanes
+    const int l16 = threadIdx.x & 15;
+    const long long r = (long long)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (r >= rows) return;   // uniform per 16-lane group, the shuffle width below
     const float inv = 1.0f / (float)e.members;
+    const long long base = r * e.nout;
     float dist = 0.0f;
-    for (int c = lane; c < e.nout; c += 64) {
-        float sv[8];
-        float mean = 0.0f;
+    for (int c = 4 * l16; c < e.nout; c += 64) {
+        if ((e.nout & 3) == 0) {
+            float4 yv[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < e.members) {
-                sv[i] = fmaxf(exp10f(e.y[i][r * e.nout + c]) - e.floor, 0.0f);
-                mean += sv[i];
+            for (int i = 0; i < 8; ++i)
+                if (i < e.members) yv[i] = *reinterpret_cast<const float4*>(e.y[i] + base + c);
+            float4 mv;
+            float* mp = &mv.x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float sv[8];
+                float mean = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (i < e.members) {
+                        sv[i] = fmaxf(exp10f((&yv[i].x)[k]) - e.floor, 0.0f);
+                        mean += sv[i];
+                    }
+                mean *= inv;
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (i < e.members) dist += fabsf(sv[i] - mean);
+                mp[k] = mean;
             }
-        mean *= inv;
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (i < e.members) dist += fabsf(sv[i] - mean);
-        e.spec[r * e.nout + c] = mean;
+            *reinterpret_cast<float4*>(e.spec + base + c) = mv;
+        } else {
+            for (int k = c; k < min(c + 4, e.nout); ++k) ensemble_column(e, base + k, inv, dist);
+        }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) dist += __shfl_xor(dist, o, 64);
-    if (lane == 0) e.spread[r] = dist * inv;
+    for (int o = 8; o > 0; o >>= 1) dist += __shfl_xor(dist, o, 16);
+    if (l16 == 0) e.spread[r] = dist * inv;
 }
 
 // out[0] = scale * sum(part[0 .. np)), one block.
@@ -613,6 +645,7 @@ struct Mlp {
     float* ens_spec = nullptr;          // ensemble spectra and spreads (batch_max rows)
     float* ens_spread = nullptr;
     uint16_t* wf = nullptr;             // fragment-ordered BF16 weights (fused inference)
+    uint64_t pver = 1, fver = 0;        // versions of the BF16 weights and of wf
     std::vector<hipEvent_t> tev;        // per-chunk timing events of the fused launches
     long long step = 0;                 // AdamW step count
 };
@@ -756,6 +789,7 @@ int backward(Mlp* n, int rows, float* loss_out, const sp::mlp::TrainState* ts = 
 
 int adamw(Mlp* n, double lr) {
     n->step += 1;
+    n->pver += 1;
     const double c1 = 1.0 - std::pow(n->cfg.beta1, (double)n->step);
     const double c2 = 1.0 - std::pow(n->cfg.beta2, (double)n->step);
     hipLaunchKernelGGL(sp::mlp::adamw_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->p, n->g,
@@ -806,8 +840,10 @@ bool same_shape(const Mlp* a, const Mlp* b) {
 }
 
 // Rewrites n->wf (fragment order, input layer padded to K = 16) from the BF16
-// weights on stream s.
+// weights on stream s, unless they have not changed since the last rewrite
+// (pver counts the updates of the BF16 weights).
 int refresh_frag(Mlp* n, hipStream_t s) {
+    if (n->wf && n->fver == n->pver) return SP_OK;
     long long total = 0;
     for (int l = 0; l < n->L; ++l) total += (long long)(l == 0 ? sp::mlp::kFusedKin : n->K[l]) * n->N[l];
     if (!n->wf) MLP_HIP(hipMalloc(&n->wf, 2 * (size_t)total));
@@ -820,6 +856,7 @@ int refresh_frag(Mlp* n, hipStream_t s) {
         MLP_HIP(hipGetLastError());
         off += cnt;
     }
+    n->fver = n->pver;
     return SP_OK;
 }
 
@@ -932,7 +969,7 @@ int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, d
         e.floor = (float)floor;
         e.spec = dev ? spec + r0 * nout : h->ens_spec;
         e.spread = dev ? spread + r0 : h->ens_spread;
-        hipLaunchKernelGGL(sp::mlp::ensemble_kernel, dim3(blocks(r, 4)), dim3(256), 0, h->st, e, r);
+        hipLaunchKernelGGL(sp::mlp::ensemble_kernel, dim3(blocks(r, 16)), dim3(256), 0, h->st, e, r);
         MLP_HIP(hipGetLastError());
         if (!dev) {
             MLP_HIP(hipMemcpyAsync(spec + r0 * nout, h->ens_spec, 4 * (size_t)r * nout, hipMemcpyDeviceToHost, h->st));
@@ -1057,6 +1094,7 @@ int sp_mlp_set_params(void* q, const float* in) {
     Mlp* n = live(q, rc);
     if (!n) return rc;
     if (!in) return sp::set_error(SP_ERR_ARG, "in is NULL");
+    n->pver += 1;
     MLP_HIP(hipMemcpyAsync(n->p, in, 4 * (size_t)n->P, hipMemcpyHostToDevice, n->st));
     hipLaunchKernelGGL(sp::mlp::to_bf16_kernel, dim3(blocks(n->P)), dim3(256), 0, n->st, n->p, n->pb,
                        n->P);
@@ -1188,6 +1226,7 @@ int sp_mlp_train(void* q, int64_t nsteps, int32_t batch, double lr0, double lr_m
     hs.lr_min = (float)lr_min;
     hs.b1 = (float)n->cfg.beta1;
     hs.b2 = (float)n->cfg.beta2;
+    n->pver += 1;
     MLP_HIP(hipMemcpyAsync(n->ts, &hs, sizeof(hs), hipMemcpyHostToDevice, n->st));
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     // One training step: every kernel reads the step from n->ts, so the same

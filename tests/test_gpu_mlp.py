@@ -143,3 +143,23 @@ def test_ensemble_device_arrays_equal_host_arrays(monkeypatch):
     np.testing.assert_array_equal(s.estimate(spread_d), est_h)
     np.testing.assert_array_equal(spec_g, spec_h)
     np.testing.assert_array_equal(est_g, est_h)
+
+
+def test_fused_inference_follows_parameter_updates(monkeypatch):
+    """The fragment-ordered weights of the fused path are rewritten after
+    every kind of parameter update (AdamW step, training loop, set_params)."""
+    x, t = _data(600, seed=9)
+    with _net(seed=13, batch_max=512) as net:
+        ys = [net.forward(x)]
+        net.train_step(x[:256], t[:256], 1e-2)
+        ys.append(net.forward(x))
+        net.set_data(x, t)
+        net.train(5, batch=256, lr0=1e-2, lr_min=1e-3, seed=2)
+        ys.append(net.forward(x))
+        net.set_params(net.get_params() * 0.5)
+        ys.append(net.forward(x))
+        monkeypatch.setenv("SP_MLP_UNFUSED", "1")
+        y_gemm = net.forward(x)
+    for a, b in zip(ys, ys[1:]):
+        assert not np.array_equal(a, b)
+    np.testing.assert_array_equal(ys[-1], y_gemm)
