@@ -308,8 +308,8 @@ int sp_mlp_train_step(void* mlp, const float* x, const float* t, int64_t rows, d
  * (rows) = mean over members of the L1 distance of their spectrum from spec. */
 int sp_mlp_ensemble_spectra(void* const* nets, int32_t members, const float* x, int64_t rows,
                             double floor, float* spec, float* spread);
-/* As sp_mlp_ensemble_spectra with x, spec and spread in device memory (HBM);
- * returns once the results are written. */
+/* As sp_mlp_ensemble_spectra with x, spec and spread in device memory (HBM;
+ * other pointers give SP_ERR_ARG); returns once the results are written. */
 int sp_mlp_ensemble_spectra_device(void* const* nets, int32_t members, const float* x,
                                    int64_t rows, double floor, float* spec, float* spread);
 /* Copies a training set (rows x nin inputs, rows x nout targets) to the device. */

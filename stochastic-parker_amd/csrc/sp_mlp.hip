@@ -906,10 +906,22 @@ int ensure_events(Mlp* n, size_t count) {
 // Ensemble spectra of rows of x (host or device pointers throughout; dev
 // selects which).  Chunks of batch_max rows; with the fused kernel all
 // members of a chunk run in one launch on the first member's stream.
+// True for memory that kernels may read: device or managed allocations.
+bool device_memory(const void* q) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, q) != hipSuccess) {
+        (void)hipGetLastError();   // an unknown host pointer; clear the error
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
 int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, double floor,
              float* spec, float* spread, bool dev) {
     if (!nets || members < 1 || members > 8) return sp::set_error(SP_ERR_ARG, "members must be in [1, 8]");
     if (rows < 0 || (rows > 0 && (!x || !spec || !spread))) return sp::set_error(SP_ERR_ARG, "bad rows or NULL buffers");
+    if (dev && rows > 0 && !(device_memory(x) && device_memory(spec) && device_memory(spread)))
+        return sp::set_error(SP_ERR_ARG, "x, spec and spread must be device memory");
     std::vector<Mlp*> ms;
     int rc;
     for (int i = 0; i < members; ++i) {
