@@ -128,19 +128,26 @@ __device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& 
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? f : g;
+        // all eight corners' loads go out before any is used
+        uint4 raw[8][4];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
             const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
             const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * kRec3DH);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) raw[q][u] = r4[u];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
             uint32_t wd[14];
 #pragma unroll
             for (int u = 0; u < 3; ++u) {
-                const uint4 t = r4[u];
+                const uint4 t = raw[q][u];
                 wd[4 * u] = t.x; wd[4 * u + 1] = t.y; wd[4 * u + 2] = t.z; wd[4 * u + 3] = t.w;
             }
             {
-                const uint4 t = r4[3];
+                const uint4 t = raw[q][3];
                 wd[12] = t.x; wd[13] = t.y;
             }
             const float wq = I.w[q];
