@@ -247,50 +247,78 @@ struct FusedArgs {
 
 __device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 
-// Hidden layer K -> H on the block's activations (in place), computed as
-// Z^T = W^T X^T: the weights are the A operand, the activations the B
-// operand, so a lane's result registers hold four consecutive features of one
-// row and go back to LDS as 8-byte writes.  Wave w (of H / 32) computes the
-// 32-feature tile w for all four 32-row tiles.
-template <int H, int K, int NT>
-__device__ __forceinline__ void fused_hidden(uint16_t* act, const uint16_t* wl, const float* bias,
-                                             int lane, int w) {
-    constexpr int LD = H + 8;
+// One phase of the pipelined hidden layers (128-row blocks), computed as
+// Z^T = W^T X^T: the fragment-ordered weights are the MFMA's A operand, the
+// activations in LDS the B operand, so register 4 g + q of a lane holds
+// feature w 32 + 8 g + 4 (lane / 32) + q of row lane % 32 of its 32-row tile,
+// and the SiLU outputs go back to LDS as 8-byte writes of four features.
+// The phase runs the MFMA work of rows [64 hm, 64 hm + 64) of a K -> H layer
+// into acc (this wave's 32 features, started from the bias) and, spread over
+// its k steps, the epilogue of prev, rows [64 he, 64 he + 64) of the layer
+// before (EPI).  Weight fragments are requested 4 k steps ahead, activations
+// 1 step ahead.
+template <int H, int K, bool EPI>
+__device__ __forceinline__ void phase(uint16_t* act, const uint16_t* wl, const float* bias, int hm,
+                                      v16f acc[2], int he, const v16f prev[2], int lane, int w) {
+    constexpr int LD = H + 8, KS = K / 16, PF = KS < 4 ? KS : 4;
     const v8bf* wv = reinterpret_cast<const v8bf*>(wl);
-    // register 4 g + q of a lane is feature w 32 + 8 g + 4 (lane / 32) + q;
-    // the bias starts the accumulation
-    v16f acc[NT];
 #pragma unroll
     for (int g = 0; g < 4; ++g)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float b0 = bias[w * 32 + 8 * g + 4 * (lane >> 5) + q];
-#pragma unroll
-            for (int j = 0; j < NT; ++j) acc[j][4 * g + q] = b0;
+            acc[0][4 * g + q] = b0;
+            acc[1][4 * g + q] = b0;
         }
-#pragma unroll 4
-    for (int ks = 0; ks < K / 16; ++ks) {
-        const v8bf a = wv[(w * (K / 16) + ks) * 64 + lane];
-        v8bf b[NT];
+    auto bload = [&](int ks, int j) {
+        return *reinterpret_cast<const v8bf*>(
+            &act[((2 * hm + j) * 32 + (lane & 31)) * LD + ks * 16 + 8 * (lane >> 5)]);
+    };
+    v8bf wq[PF];
 #pragma unroll
-        for (int j = 0; j < NT; ++j)
-            b[j] = *reinterpret_cast<const v8bf*>(&act[(j * 32 + (lane & 31)) * LD + ks * 16 + 8 * (lane >> 5)]);
+    for (int i = 0; i < PF; ++i) wq[i] = wv[(w * KS + i) * 64 + lane];
+    v8bf b0 = bload(0, 0), b1 = bload(0, 1);
 #pragma unroll
-        for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[j], 0, 0, 0);
+    for (int ks = 0; ks < KS; ++ks) {
+        const v8bf av = wq[ks % PF];
+        if (ks + PF < KS) wq[ks % PF] = wv[(w * KS + ks + PF) * 64 + lane];
+        const v8bf c0 = b0, c1 = b1;
+        if (ks + 1 < KS) {
+            b0 = bload(ks + 1, 0);
+            b1 = bload(ks + 1, 1);
+        }
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, c0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, c1, acc[1], 0, 0, 0);
+        // epilogue share of this k step: (g, j) pairs spread evenly over KS
+        if (EPI) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if ((e * KS) / 8 != ks) continue;
+                const int g = e >> 1, j = e & 1;
+                const int m = w * 32 + 8 * g + 4 * (lane >> 5);
+                uint2 v;
+                v.x = bf16x2_rne(silu(prev[j][4 * g]), silu(prev[j][4 * g + 1]));
+                v.y = bf16x2_rne(silu(prev[j][4 * g + 2]), silu(prev[j][4 * g + 3]));
+                *reinterpret_cast<uint2*>(&act[((2 * he + j) * 32 + (lane & 31)) * LD + m]) = v;
+            }
+        }
     }
-    __syncthreads();   // every wave has read the layer's input
+}
+
+template <int H>
+__device__ __forceinline__ void half_epilogue(uint16_t* act, const v16f acc[2], int hf, int lane, int w) {
+    constexpr int LD = H + 8;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-        const int m = w * 32 + 8 * g + 4 * (lane >> 5);   // first of 4 features
+        const int m = w * 32 + 8 * g + 4 * (lane >> 5);
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
+        for (int j = 0; j < 2; ++j) {
             uint2 v;
             v.x = bf16x2_rne(silu(acc[j][4 * g]), silu(acc[j][4 * g + 1]));
             v.y = bf16x2_rne(silu(acc[j][4 * g + 2]), silu(acc[j][4 * g + 3]));
-            *reinterpret_cast<uint2*>(&act[(j * 32 + (lane & 31)) * LD + m]) = v;
+            *reinterpret_cast<uint2*>(&act[((2 * hf + j) * 32 + (lane & 31)) * LD + m]) = v;
         }
     }
-    __syncthreads();
 }
 
 // Network nin -> H (x 1 + nhh, SiLU) -> NOUT for rows [128 bx, 128 bx + 128)
@@ -312,9 +340,33 @@ __global__ void __launch_bounds__(2 * H, 2) mlp_infer_kernel(FusedArgs a) {
         act[r * LD + c] = bf16_rne(v);
     }
     __syncthreads();
-    fused_hidden<H, kFusedKin, NT>(act, wf, p + a.boff[0], lane, w);
+    static_assert(NT == 4, "the hidden-layer pipeline works on two 64-row halves");
     const uint16_t* wl = wf + H * kFusedKin;
-    for (int l = 1; l <= a.nhh; ++l, wl += H * H) fused_hidden<H, H, NT>(act, wl, p + a.boff[l], lane, w);
+    {
+        // The hidden layers as a pipeline over the block's two 64-row halves:
+        // while a wave's matrix cores take one half of layer l, its vector
+        // ALUs apply the SiLU epilogue to the other half (rows that the
+        // matrix work does not read); a barrier ends each phase.
+        //   A(l): MFMA rows 0-63 of layer l   | epilogue rows 64-127 of l - 1
+        //   B(l): MFMA rows 64-127 of layer l | epilogue rows 0-63 of l
+        // (layer 0 is the input layer, K = 16)
+        v16f lo[2], hi[2];
+        phase<H, kFusedKin, false>(act, wf, p + a.boff[0], 0, lo, 1, hi, lane, w);
+        __syncthreads();
+        phase<H, kFusedKin, false>(act, wf, p + a.boff[0], 1, hi, 0, lo, lane, w);
+        half_epilogue<H>(act, lo, 0, lane, w);
+        __syncthreads();
+        for (int l = 1; l <= a.nhh; ++l) {   // each phase is straight-line code
+            const uint16_t* wcur = wf + H * kFusedKin + (long long)(l - 1) * H * H;
+            phase<H, H, true>(act, wcur, p + a.boff[l], 0, lo, 1, hi, lane, w);
+            __syncthreads();
+            phase<H, H, true>(act, wcur, p + a.boff[l], 1, hi, 0, lo, lane, w);
+            __syncthreads();
+        }
+        half_epilogue<H>(act, hi, 1, lane, w);
+        __syncthreads();
+        wl += (long long)a.nhh * H * H;
+    }
     const v8bf* wv = reinterpret_cast<const v8bf*>(wl);
     const float* bias = p + a.boff[a.nhh + 1];
 #pragma unroll
