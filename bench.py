@@ -331,7 +331,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the pusher shards by "
-              "particles and this build measures one GPU's share", file=sys.stderr)
+              "particles and the benchmark measures one GPU's share", file=sys.stderr)
         sys.exit(2)
     if a.workload == "surrogate":
         surrogate_bench(a)

@@ -284,7 +284,7 @@ contains
             case ('-ss', '--sort_steps'); sort_steps = to_int(val)
             case ('-db2', '-nb', '-pd', '-ptf', '-de', '-du', &
                   '-sf1', '-sn1', '-s2e', '-ii', '-sf2', '-sn2')
-                ! accepted and ignored: options of paths this build does not run
+                ! accepted and ignored: options of paths the GPU pusher does not run
             case default
                 write(*, "(A,A)") " unknown option ", trim(key)
                 stop 1
