@@ -377,3 +377,44 @@ def test_f32h_mover_distribution_close_to_f64():
     h64, _ = np.histogram(np.log(a["p"]), bins=40, range=(-1, 1))
     h16, _ = np.histogram(np.log(b["p"]), bins=40, range=(-1, 1))
     assert np.abs(h64 / h64.sum() - h16 / h16.sum()).sum() < 0.01
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32h"])
+def test_3d_fast_paths_with_time_interpolation_close_to_f64(precision):
+    """Two field frames blended in time: the FP32 and FP16-gradient gathers
+    read both frames (each with its own gradient scale) as the FP64 path does."""
+    c = cfg3d(12, pbcx=1, dt_max_rel=1e-2, time_interp=1)
+    n1, n2 = nodes_for(c, seed=1), nodes_for(c, seed=2)
+    res = {}
+    for prec in ("f64", precision):
+        pz, _ = make(c, n1, precision=prec, nptl_max=8000, nodes2=n2)
+        pz.inject_particles_spatial_uniform(6000, 1e-7, 1, 5.0, 0.0, 0.2, (0, 0, 0, 1, 1, 1))
+        pz.particle_mover(0.0, 0.2, 1, 10)
+        res[prec] = by_id(pz.get_particles())
+        pz.close()
+    a, b = res["f64"], res[precision]
+    common = np.intersect1d(a["tag_injected"], b["tag_injected"])
+    assert len(common) > 0.98 * len(a["p"])
+    ia = np.searchsorted(a["tag_injected"], common)
+    ib = np.searchsorted(b["tag_injected"], common)
+    dp = np.abs(a["p"][ia] - b["p"][ib]) / a["p"][ia]
+    assert np.median(dp) < 1e-4
+    h64, _ = np.histogram(np.log(a["p"]), bins=40, range=(-1, 1))
+    hb, _ = np.histogram(np.log(b["p"]), bins=40, range=(-1, 1))
+    assert np.abs(h64 / h64.sum() - hb / hb.sum()).sum() < 0.01
+
+
+def test_f32h_paused_launches_are_bit_identical():
+    c = cfg3d(12, pbcx=1, dt_max_rel=1e-2, time_interp=1)
+    n1, n2 = nodes_for(c, seed=1), nodes_for(c, seed=2)
+    out = {}
+    for ss in (0, 5):
+        pz, _ = make(dict(c, sort_steps=ss), n1, precision="f32h", nptl_max=8000, nodes2=n2)
+        pz.inject_particles_spatial_uniform(4000, 1e-7, 1, 5.0, 0.0, 0.2, (0, 0, 0, 1, 1, 1))
+        steps = pz.particle_mover(0.0, 0.2, 1, 10)["steps"]
+        out[ss] = (by_id(pz.get_particles()), steps)
+        pz.close()
+    (g0, s0), (g1, s1) = out[0], out[5]
+    assert s0 == s1
+    for k in g0:
+        np.testing.assert_array_equal(g0[k], g1[k], err_msg=k)
