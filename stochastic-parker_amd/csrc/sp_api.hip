@@ -523,9 +523,14 @@ int sp_fields_gradients(void* ctx, int32_t slot, float* farray32) {
         std::memcpy(&gmax_v, &bits, sizeof(float));
         float scale = 1.0f;
         if (gmax_v > 0.0f && std::isfinite(gmax_v)) {
-            int ex = 0;
-            std::frexp(16384.0f / gmax_v, &ex);   // 16384 / gmax in [2^(ex-1), 2^ex)
-            scale = std::ldexp(1.0f, ex - 1);
+            const float q = 16384.0f / gmax_v;
+            if (std::isfinite(q)) {
+                int ex = 0;
+                std::frexp(q, &ex);   // 16384 / gmax in [2^(ex-1), 2^ex)
+                scale = std::ldexp(1.0f, ex - 1);
+            } else {
+                scale = std::ldexp(1.0f, 126);   // gradients below ~1e-34 everywhere
+            }
         }
         c->gscale[slot] = scale;
         if (e == hipSuccess) e = sp::launch_pack_half(c->dp, c->n8[slot], scale, c->rec[slot], fa, c->st);
