@@ -25,7 +25,9 @@ HBM before timing starts.
 Printed JSON line (one):
   value          particle-steps/s over the K timed intervals (FP32 gather path)
   roofline       mover kernel: algorithmic HBM bytes / kernel time (HIP events
-                 on the library's stream) against 8 TB/s
+                 on the library's stream) against 8 TB/s; valu_issue: VALU
+                 instructions per step (committed counter run) x steps per
+                 kernel second against the chip's non-packed VALU issue rate
   cpu_baseline   the FP64 CPU oracle (OpenMP, per-thread MT19937 increments) on
                  a bounded sample of the same workload, on this host
   accuracy       f(p) and spatial-map L1 of the FP32 path against the FP64
@@ -218,6 +220,28 @@ def measured_traffic(wl, n, precision):
     return t.get("traffic_bytes_per_launch")
 
 
+# MI355X non-packed VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_LANE_INSTR_PEAK = 256 * 4 * 16 * 2.4e9
+
+
+def instruction_issue(wl, precision, steps_per_s):
+    """The mover's VALU issue rate against the chip's, from the committed
+    counter run's VALU instructions per particle-step (a per-step figure that
+    does not depend on the particle count); None for other workloads."""
+    path = os.path.join(ROOT, "profiles", "mover_counters_1e7.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(wl.kind)
+    except (OSError, ValueError):
+        return None
+    if not t or t.get("precision") != precision:
+        return None
+    lane_rate = t["valu_instr_per_step"] * steps_per_s
+    return {"valu_instr_per_step": t["valu_instr_per_step"], "achieved_lane_instr_per_s": lane_rate,
+            "peak_lane_instr_per_s": VALU_LANE_INSTR_PEAK, "frac": lane_rate / VALU_LANE_INSTR_PEAK,
+            "source": t["source"]}
+
+
 def surrogate_bench(a):
     """Trains the surrogate on pusher spectra and measures a 1e5-query sweep."""
     from sp_hip import surrogate as S
@@ -378,6 +402,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(wl, n, a.precision),
             "kernel": "mover_kernel", "bytes_per_step": bytes_per_step,
+            # the gathers are served by L2/MALL (traffic is far below the
+            # algorithmic bytes), so the issue rate is the other bound to read
+            "valu_issue": instruction_issue(wl, a.precision, r["steps"] / (r["kernel_ms"] * 1e-3)),
         },
     }
     if not a.no_accuracy:
