@@ -6,7 +6,7 @@ Workloads:
                 sized for 2000 diffusion-limited steps per particle.
   3d            3D turbulence snapshot on a 512^3 grid with a compression
                 slab, 1e8 particles, 1000 diffusion-limited steps per interval,
-                particles re-sorted every 200 steps; FP32 arithmetic on node
+                particles sorted once per interval; FP32 arithmetic on node
                 records with FP16-stored gradients (--precision f32 for FP32
                 records).
   batch         64 diffusion-parameter variants over the 512^2 2D snapshot in

@@ -201,15 +201,17 @@ def fields_3d(n: int, seed: int = FIELD_SEED, kmin: float = 2.0, kmax: float = 6
 def config_3d(n: int, **over) -> dict:
     """sp_config keys of the 3D snapshot on an n^3 grid over [0,1]^3: open
     along x (the compression slab's normal), periodic along y and z.  The
-    particles still moving are re-sorted every 200 steps of each (measured
-    best on MI355X for 512^3 and 1e8 particles; results do not depend on it)."""
+    particles are sorted once per interval, before the mover (with the
+    8^3-node record bricks this measured faster on MI355X for 512^3 and 1e8
+    particles than re-sorting within the interval every 100 or 200 steps;
+    results do not depend on it)."""
     c = dict(
         ndim=3, nx=n, ny=n, nz=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n,
         xmin=0.0, ymin=0.0, zmin=0.0, xmax=1.0, ymax=1.0, zmax=1.0, lx=1.0, ly=1.0, lz=1.0,
         pbcx=1, pbcy=0, pbcz=0, momentum_dependency=1, mag_dependency=0, charge=-1,
         p0=1.0, pmin=1e-2, pmax=1e4, b0=1.0, gamma_turb=5.0 / 3.0, kpara0=1e-3, kret=0.01,
         dt_min_rel=1e-7, dt_max_rel=1e-3, drift1=4e7, drift2=2e8, check_drift_2d=0,
-        acc_region_flag=0, time_interp=0, sort_steps=200,
+        acc_region_flag=0, time_interp=0, sort_steps=0,
     )
     c.update(over)
     return c
