@@ -176,3 +176,60 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision, inject):
     flat = np.concatenate([esc[0]["x"].ravel(), esc[0]["y"].ravel()])
     assert kset == 1 and size == flat.size
     np.testing.assert_array_equal(np.frombuffer(raw, dtype="<f8", count=size, offset=off + 12), flat)
+
+
+@pytest.mark.gpu
+def test_driver_3d_fp16_records_matches_host_api(tmp_path):
+    """The driver over a 3D snapshot (-nd 3) on the FP16-gradient records:
+    quick.dat and the global f(p) equal the Python host path's."""
+    _need(DRIVER)
+    n, npart, dtf = 16, 3000, 0.05
+    os.makedirs(tmp_path / "mhd")
+    os.makedirs(tmp_path / "config")
+    os.makedirs(tmp_path / "data")
+    dx = 1.0 / n
+    reals = [dx, dx, dx, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, dtf]
+    ints = [n, n, n, n, n, n, 1, 1, 1, 8, 0, 0, 0, 0]
+    with open(tmp_path / "mhd" / "mhd_config.dat", "wb") as f:
+        f.write(struct.pack("<13d14i", *reals, *ints))
+    nodes = workloads.fields_3d(n)
+    for k in (0, 1):
+        nodes.tofile(tmp_path / "mhd" / f"mhd_data_{k:04d}")
+    conf = (CONF.replace("pbcx = 0", "pbcx = 1").replace("pbcy = 1", "pbcy = 0")
+            .replace("rz1 = 1", "rz1 = 4"))
+    (tmp_path / "config" / "conf.dat").write_text(conf)
+    args = [DRIVER, "-dm", "mhd/", "-np", str(npart), "-nm", "20000", "-ts", "0", "-te", "1",
+            "-st", "1", "-df", "1", "-sf", "0", "-nl", ".false.", "-ug", "1", "-nd", "3",
+            "-ld", ".true.", "-dd", "data/", "-dt", "1e-7", "-gp", "f32h"]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    quick = (tmp_path / "data" / "quick.dat").read_text().splitlines()
+    assert len(quick) == 3  # header + frames 0, 1
+
+    cfg = workloads.config_3d(n, dt_max_rel=1e-2, gamma_turb=1.6666667, precision="f32h",
+                              nptl_max=20000)
+    dist = dict(npp_global=64, nmu_global=1, local_dist=1,
+                local=[dict(enabled=1, npbins=8, nmu=1, rx=4, ry=4, rz=4, pmin=1e-2, pmax=1e4),
+                       None, None, None])
+    rows = []
+    with ParkerPusher(**cfg) as pz:
+        pz.upload_fields(nodes)
+        pz.calc_fields_gradients()
+        pz.init_particle_distributions(**dist)
+        pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, 0.0, dtf, (0, 0, 0, 1, 1, 1))
+        rows.append((0, pz.quick_check()))
+        pz.particle_mover(0.0, dtf, 1, 10)
+        rows.append((1, pz.quick_check()))
+        fg, _ = pz.calc_particle_distributions()
+    keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
+            "pdt_avg")
+    for line, (tf, q) in zip(quick[1:], rows):
+        assert int(line[:6]) == tf
+        got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
+        np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
+    assert rows[-1][1]["leak"] > 0   # open along x: some particles leave
+    raw = (tmp_path / "data" / "fdists_0001.dat").read_bytes()
+    assert struct.unpack_from("<i", raw, 0)[0] == 64
+    fglob = np.frombuffer(raw, dtype="<f8", count=64, offset=4 + 8 * 65)
+    np.testing.assert_array_equal(fglob, fg)
