@@ -233,3 +233,48 @@ def test_driver_3d_fp16_records_matches_host_api(tmp_path):
     assert struct.unpack_from("<i", raw, 0)[0] == 64
     fglob = np.frombuffer(raw, dtype="<f8", count=64, offset=4 + 8 * 65)
     np.testing.assert_array_equal(fglob, fg)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_driver_time_interpolation_matches_host_api(tmp_path, precision):
+    """-ti 1 over three different frames: the driver reads frame tf into the
+    second slot before each interval and copies it down afterwards, as the
+    reference's time loop does; quick.dat equals the host path's."""
+    _need(DRIVER)
+    n, npart, dtf = 64, 3000, 0.05
+    write_case(tmp_path, n, dtf)
+    frames = [workloads.fields_2d(n, n, seed=11 + k) for k in range(3)]
+    for k, f in enumerate(frames):
+        f.tofile(tmp_path / "mhd" / f"mhd_data_{k:04d}")
+    args = [DRIVER, "-dm", "mhd/", "-np", str(npart), "-nm", "20000", "-ts", "0", "-te", "2",
+            "-st", "0", "-df", "1", "-sf", "0", "-ti", "1", "-nl", ".false.", "-ug", "1",
+            "-nd", "2", "-ld", ".true.", "-dd", "data/", "-dt", "1e-7", "-gp", precision]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    quick = (tmp_path / "data" / "quick.dat").read_text().splitlines()
+    assert len(quick) == 4
+
+    cfg = workloads.config_2d(n, pbcy=1, dt_max_rel=1e-2, gamma_turb=1.6666667,
+                              precision=precision, nptl_max=20000, time_interp=1)
+    rows = []
+    with ParkerPusher(**cfg) as pz:
+        pz.upload_fields(frames[0], 0)
+        pz.calc_fields_gradients(0)
+        for tf in (1, 2):
+            pz.upload_fields(frames[tf], 1)
+            pz.calc_fields_gradients(1)
+            if tf == 1:
+                pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, 0.0, dtf,
+                                                    (0.0, 0.0, 0.0, 1.0, 1.0, 1.0))
+                rows.append((0, pz.quick_check()))
+            pz.particle_mover((tf - 1) * dtf, dtf, 1, 10)
+            rows.append((tf, pz.quick_check()))
+            pz.copy_fields()
+    keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
+            "pdt_avg")
+    for line, (tf, q) in zip(quick[1:], rows):
+        assert int(line[:6]) == tf
+        got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
+        np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
