@@ -698,6 +698,8 @@ struct Mlp {
     float* ens_spread = nullptr;
     uint16_t* wf = nullptr;             // fragment-ordered BF16 weights (fused inference)
     uint64_t pver = 1, fver = 0;        // versions of the BF16 weights and of wf
+    float* ey = nullptr;                // outputs of device-array ensemble chunks
+    long long ey_rows = 0;
     std::vector<hipEvent_t> tev;        // per-chunk timing events of the fused launches
     long long step = 0;                 // AdamW step count
 };
@@ -718,6 +720,7 @@ void destroy(Mlp* n) {
     dfree(n->y); dfree(n->t); dfree(n->gh[0]); dfree(n->gh[1]); dfree(n->part);
     dfree(n->loss_hist); dfree(n->xin); dfree(n->xd); dfree(n->td); dfree(n->bidx);
     dfree(n->wpart); dfree(n->ens_spec); dfree(n->ens_spread); dfree(n->ts); dfree(n->wf);
+    dfree(n->ey);
     for (auto& e : n->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : n->tev)
@@ -919,9 +922,11 @@ void launch_fused(const sp::mlp::FusedArgs& a, int members, hipStream_t s) {
     hipLaunchKernelGGL((sp::mlp::mlp_infer_kernel<H, NOUT, sp::mlp::kFusedRows / 32>), grid, dim3(2 * H), 0, s, a);
 }
 
-// ms[i]->y (rows x nout) = network i (x), all members in one launch on s.
-// Needs fused_ok and equal shapes, and every member's wf refreshed.
-int fused_forward(Mlp* const* ms, int members, const float* x, int rows, hipStream_t s) {
+// y[i] (rows x nout; ms[i]->y when y is NULL) = network i (x), all members
+// in one launch on s.  Needs fused_ok and equal shapes, and every member's wf
+// refreshed.
+int fused_forward(Mlp* const* ms, int members, const float* x, int rows, hipStream_t s,
+                  float* const* y = nullptr) {
     const Mlp* h = ms[0];
     sp::mlp::FusedArgs a{};
     a.x = x;
@@ -931,7 +936,7 @@ int fused_forward(Mlp* const* ms, int members, const float* x, int rows, hipStre
     for (int i = 0; i < members; ++i) {
         a.wf[i] = ms[i]->wf;
         a.p[i] = ms[i]->p;
-        a.y[i] = ms[i]->y;
+        a.y[i] = y ? y[i] : ms[i]->y;
     }
     for (int l = 0; l < h->L; ++l) a.boff[l] = h->boff[l];
     if (rows == 0) return SP_OK;
@@ -995,6 +1000,24 @@ int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, d
         MLP_HIP(hipMalloc(&h->ens_spec, 4 * (size_t)h->bmax * nout));
         MLP_HIP(hipMalloc(&h->ens_spread, 4 * (size_t)h->bmax));
     }
+    // device arrays in and out: chunks of up to 2^20 rows into the members'
+    // own output buffers (fewer, fuller launches than batch_max-row chunks)
+    std::vector<float*> ys((size_t)members, nullptr);
+    for (int i = 0; i < members; ++i) ys[(size_t)i] = ms[(size_t)i]->y;
+    if (dev && fused && rows > 0) {
+        chunk = (int)std::min<int64_t>(rows, (int64_t)1 << 20);
+        for (int i = 0; i < members; ++i) {
+            Mlp* n = ms[(size_t)i];
+            if (n->ey_rows < chunk) {
+                dfree(n->ey);
+                n->ey = nullptr;
+                n->ey_rows = 0;
+                MLP_HIP(hipMalloc(&n->ey, 4 * (size_t)chunk * nout));
+                n->ey_rows = chunk;
+            }
+            ys[(size_t)i] = n->ey;
+        }
+    }
     const int64_t nchunks = (rows + chunk - 1) / chunk;
     if (fused) {
         for (auto* n : ms) {
@@ -1015,7 +1038,7 @@ int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, d
                 xc = h->xin;
             }
             MLP_HIP(hipEventRecord(h->tev[(size_t)(2 * c)], h->st));
-            rc = fused_forward(ms.data(), members, xc, r, h->st);
+            rc = fused_forward(ms.data(), members, xc, r, h->st, ys.data());
             if (rc != SP_OK) return rc;
             MLP_HIP(hipEventRecord(h->tev[(size_t)(2 * c + 1)], h->st));
         } else {
@@ -1027,7 +1050,7 @@ int ensemble(void* const* nets, int32_t members, const float* x, int64_t rows, d
             }
         }
         sp::mlp::EnsembleArgs e{};
-        for (int i = 0; i < members; ++i) e.y[i] = ms[(size_t)i]->y;
+        for (int i = 0; i < members; ++i) e.y[i] = ys[(size_t)i];
         e.members = members;
         e.nout = nout;
         e.floor = (float)floor;
