@@ -64,6 +64,9 @@ struct Ctx {
     int cur = 0;
     long long nptl = 0, nmax = 0, nsplit = 0, nesc = 0;
     int tag_max = 0;
+    // cumulative leak / leak_negp in 2^-32 weight units; the device counters
+    // are folded in and cleared after every mover call
+    unsigned __int128 leak_acc = 0, negp_acc = 0;
     sp::Counters* ctr = nullptr;
     unsigned* tile_cnt = nullptr;
     unsigned* tile_esc = nullptr;
@@ -104,6 +107,8 @@ struct Ctx {
 };
 
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+double fx_weight(unsigned __int128 v) { return (double)v * kInvFx; }
 
 void free_dev(void* p) {
     if (p) (void)hipFree(p);
@@ -578,6 +583,16 @@ int sp_particles_upload(void* ctx, const sp_particles* h, int64_t n) {
     const size_t sz[17] = {8, 8, 8, 8, 8, 8, 8, 8, 8, 1, 1, 4, 4, 4, 4, 4, 4};
     for (int k = 0; k < 17; ++k)
         if (!src[k] && n > 0) return fail(SP_ERR_ARG, "every particle array is required on upload");
+    // weights are summed exactly as integers of 2^-32 (split weights 2^-k, k <= 32)
+    for (int64_t i = 0; i < n; ++i) {
+        const double w = h->weight[i] * sp::kFxScale;
+        if (!(w >= 0.0 && w < 9.2e18) || w != std::floor(w))
+            return fail(SP_ERR_ARG, "particle " + std::to_string(i) +
+                                        ": weight must be a multiple of 2^-32 in [0, 2^31)");
+        if (h->split_times[i] < 0 || h->split_times[i] > sp::kMaxSplitTimes)
+            return fail(SP_ERR_ARG, "particle " + std::to_string(i) + ": split_times must be in [0, " +
+                                        std::to_string(sp::kMaxSplitTimes) + "]");
+    }
     for (int k = 0; k < 17 && n > 0; ++k)
         SP_HIP(hipMemcpyAsync(dst[k], src[k], sz[k] * (size_t)n, hipMemcpyHostToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
@@ -827,6 +842,9 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     SP_HIP(hipEventRecord(c->ev[2], c->st));
     sp::Counters hc{};
     SP_HIP(hipMemcpyAsync(&hc, c->ctr, sizeof(hc), hipMemcpyDeviceToHost, c->st));
+    // the leak counters of this call move to the host totals (a u64 of 2^-32
+    // units would wrap after a cumulative weight of 2^32)
+    SP_HIP(hipMemsetAsync(&c->ctr->leak_fx, 0, 2 * sizeof(unsigned long long), c->st));
     unsigned long long tot[2] = {0, 0};
     if (c->nptl > 0) SP_HIP(hipMemcpyAsync(tot, c->totals, sizeof(tot), hipMemcpyDeviceToHost, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
@@ -837,14 +855,16 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
         c->nptl = kept;
         c->nesc += esc;
     }
+    c->leak_acc += hc.leak_fx;
+    c->negp_acc += hc.negp_fx;
     float ms_all = 0.f;
     SP_HIP(hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]));
     if (stats) {
         stats->steps = (int64_t)steps;
         stats->nptl_current = c->nptl;
         stats->nptl_escaped = c->nesc;
-        stats->leak = (double)hc.leak_fx * kInvFx;
-        stats->leak_negp = (double)hc.negp_fx * kInvFx;
+        stats->leak = fx_weight(c->leak_acc);
+        stats->leak_negp = fx_weight(c->negp_acc);
         stats->kernel_ms = ms_push;
         stats->total_ms = ms_all;
     }
@@ -856,14 +876,23 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split) {
     Ctx* c = live_ctx(ctx, rc);
     if (!c) return rc;
     if (c->nptl == 0) return SP_OK;
+    if (!(split_ratio > 1.0)) return fail(SP_ERR_ARG, "split_ratio must be > 1");
     sp::SplitArgs sa{split_ratio, pmin_split, c->cfg.p0, c->cfg.pmax};
     const long long nt = sp::num_tiles(c->nptl);
-    SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 1, sa, c->tile_cnt, nullptr, nullptr,
+    // pass 1 counts the candidates and those already split kMaxSplitTimes times
+    SP_HIP(sp::launch_tile_count(c->P[c->cur], c->nptl, 1, sa, c->tile_cnt, c->tile_esc, nullptr,
                                  nullptr, c->st));
-    SP_HIP(sp::launch_tile_scan(c->tile_cnt, nullptr, nt, c->tile_off, c->totals, c->st));
-    SP_HIP(sp::launch_split_apply(c->P[c->cur], c->nptl, c->nmax, sa, c->tile_off, c->st));
+    SP_HIP(sp::launch_tile_scan(c->tile_cnt, c->tile_esc, nt, c->tile_off, c->totals, c->st));
     unsigned long long tot[2] = {0, 0};
     SP_HIP(hipMemcpyAsync(tot, c->totals, sizeof(tot), hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
+    if (tot[1] > 0)
+        return fail(SP_ERR_UNSUPPORTED,
+                    std::to_string(tot[1]) + " split candidates have split_times = " +
+                        std::to_string(sp::kMaxSplitTimes) + ": their copies would weigh 2^-" +
+                        std::to_string(sp::kMaxSplitTimes + 1) +
+                        ", below the 2^-32 resolution of the exact weight sums (nothing was split)");
+    SP_HIP(sp::launch_split_apply(c->P[c->cur], c->nptl, c->nmax, sa, c->tile_off, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
     const long long ncand = (long long)tot[0];
     const long long room = c->nmax - c->nptl;
@@ -1069,7 +1098,7 @@ int sp_distributions(void* ctx, double* fglobal, double* flocal1, double* flocal
         if (c->dloc[k]) SP_HIP(hipMemsetAsync(c->dloc[k], 0, sizeof(unsigned long long) * c->loc_size[k], c->st));
     }
     const int grid = (int)std::max<long long>(1, std::min<long long>((c->nptl + 255) / 256, (long long)c->ncu * 8));
-    SP_HIP(sp::launch_dist(c->P[c->cur], c->nptl, c->da, c->dglob, gl, grid, c->st));
+    SP_HIP(sp::launch_dist(c->P[c->cur], c->nptl, c->da, c->dglob, gl, c->loc_size, grid, c->st));
     std::vector<unsigned long long> hb((size_t)c->nglob);
     SP_HIP(hipMemcpyAsync(hb.data(), c->dglob, sizeof(unsigned long long) * c->nglob, hipMemcpyDeviceToHost, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
@@ -1243,7 +1272,7 @@ int sp_variant_distributions(void* ctx, int32_t nvar, double* fglobal, int64_t* 
     unsigned long long* hist = c->vhist;
     unsigned long long* cnt = c->vhist + nbins;
     SP_HIP(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * (nbins + (size_t)nvar), c->st));
-    SP_HIP(sp::launch_variant_dist(c->P[c->cur], c->nptl, c->da, nvar, hist, cnt, c->st));
+    SP_HIP(sp::launch_variant_dist(c->P[c->cur], c->nptl, c->da, nvar, hist, cnt, c->ncu, c->st));
     std::vector<unsigned long long> hb(nbins + (size_t)nvar);
     SP_HIP(hipMemcpyAsync(hb.data(), hist, sizeof(unsigned long long) * hb.size(), hipMemcpyDeviceToHost, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
@@ -1302,8 +1331,8 @@ int sp_quick_check(void* ctx, double out[9]) {
     out[0] = (double)c->nptl;
     out[1] = (double)c->nsplit;
     out[2] = (double)W * kInvFx;
-    out[3] = (double)hc.leak_fx * kInvFx;
-    out[4] = (double)hc.negp_fx * kInvFx;
+    out[3] = fx_weight(c->leak_acc + hc.leak_fx);
+    out[4] = fx_weight(c->negp_acc + hc.negp_fx);
     out[5] = mn;
     out[6] = mx;
     out[7] = c->nptl > 0 ? S / (double)c->nptl : 0.0;

@@ -93,8 +93,11 @@ __device__ __forceinline__ double boundary_condition(int flags, double& x, doubl
     return leaked;
 }
 
-// Weight -> fixed point (2^-32 units).  Split weights are 2^-k, k <= 32 in
-// any run the reference supports, so the conversion and every sum are exact.
+// Weight -> fixed point (2^-32 units).  Particles carry weight 1 or a split
+// weight 2^-k; the split refuses to go past k = kMaxSplitTimes and uploads
+// reject weights that are not multiples of 2^-32, so the conversion is exact
+// and every sum is exact and independent of the order of the additions (up to
+// a total weight of 2^32, more particles than one GPU holds).
 __device__ __forceinline__ unsigned long long weight_fx(double w) {
     return (unsigned long long)(w * kFxScale);
 }

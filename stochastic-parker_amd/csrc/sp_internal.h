@@ -113,6 +113,8 @@ struct Counters {
 };
 
 constexpr double kFxScale = 4294967296.0;  // 2^32 weight units per unit weight
+// Largest split_times whose weight 2^-split_times the fixed point holds.
+constexpr int kMaxSplitTimes = 32;
 
 // Particle arrays of one buffer set live in one allocation with a common
 // capacity `cap` (a multiple of 256): the 10 FP64 arrays first, then the two

@@ -87,9 +87,11 @@ hipError_t launch_compact_scatter(const Ptl& S, const Ptl& D, long long n,
 hipError_t launch_split_apply(const Ptl& P, long long n, long long nmax, const SplitArgs& sa,
                               const unsigned long long* tile_off, hipStream_t s);
 hipError_t launch_dist(const Ptl& P, long long n, const DistArgs& da, unsigned long long* gglob,
-                       unsigned long long* const gloc[4], int grid, hipStream_t s);
+                       unsigned long long* const gloc[4], const long long loc_size[4], int grid,
+                       hipStream_t s);
 hipError_t launch_variant_dist(const Ptl& P, long long n, const DistArgs& da, int nvar,
-                               unsigned long long* hist, unsigned long long* counts, hipStream_t s);
+                               unsigned long long* hist, unsigned long long* counts, int ncu,
+                               hipStream_t s);
 hipError_t launch_stats(const Ptl& P, long long n, double* part, int grid, hipStream_t s);
 
 }  // namespace sp

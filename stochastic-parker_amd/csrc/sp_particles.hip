@@ -395,20 +395,24 @@ __global__ void __launch_bounds__(kBlock) tile_count_kernel(Ptl P, long long n, 
     for (int it = 0; it < kTileItems; ++it) {
         const long long i = t0 + (long long)it * kBlock + threadIdx.x;
         if (i < n) {
-            c += predicate(P, i, mode, sa) ? 1u : 0u;
+            const bool hit = predicate(P, i, mode, sa);
+            c += hit ? 1u : 0u;
             if (mode == 0) {
                 const int cf = P.count_flag[i];
                 e += cf < 0 ? 1u : 0u;
                 if (cf < 0 && ea.on) escaped_hist(P, i, -cf, da, ea);
+            } else {
+                // split candidates whose copy would weigh 2^-(k+1) < 2^-32
+                e += hit && P.split_times[i] >= kMaxSplitTimes ? 1u : 0u;
             }
         }
     }
     unsigned tc = 0, te = 0;
     block_exclusive(c, lds, tc);
-    if (mode == 0) block_exclusive(e, lds, te);
+    block_exclusive(e, lds, te);
     if (threadIdx.x == 0) {
         tile_cnt[blockIdx.x] = tc;
-        if (mode == 0) tile_esc[blockIdx.x] = te;
+        if (tile_esc) tile_esc[blockIdx.x] = te;
     }
 }
 
@@ -514,12 +518,14 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(Ptl P, long long n,
         if (cand) {
             const long long o = n + (long long)(base + ex);
             if (o < nmax) {
+                // k < kMaxSplitTimes: the host refuses the call otherwise
                 const int k = (int)P.split_times[i];
                 const double wnew = ldexp(1.0, -(1 + k));  // 0.5**(1.0 + split_times)
                 P.w[i] = wnew;
                 P.split_times[i] = (int8_t)(k + 1);
                 copy_particle(P, i, P, o);
-                P.tag_splitted[o] = P.tag_splitted[i] + (int32_t)(1u << k);
+                // tag_splitted + 2**k in wrapping 32-bit arithmetic
+                P.tag_splitted[o] = (int32_t)((uint32_t)P.tag_splitted[i] + (1u << k));
             }
         }
         base += tot;
@@ -530,19 +536,56 @@ __global__ void __launch_bounds__(kBlock) split_apply_kernel(Ptl P, long long n,
 // Distributions (diagnostics.f90:738-906).  Weights are accumulated as
 // unsigned 2^-32 fixed point, so every sum is exact and independent of the
 // order of the atomics.
-__global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, DistArgs da,
+//
+// A block takes one contiguous slice of the particle array; after the
+// spatial sort its particles occupy a few map cells, so most of their
+// local-map updates fall on a few hundred bins.  The block sums those in an
+// LDS hash table (open addressing, keys = 4 * bin + set) and adds each
+// occupied slot to global memory once; an update that finds no free slot
+// within kProbe probes goes straight to global memory.  f(p) has its own
+// LDS bins.
+constexpr int kMapSlotsLog2 = 11;
+constexpr int kMapSlots = 1 << kMapSlotsLog2;
+constexpr int kProbe = 8;
+constexpr unsigned kEmptyKey = 0xffffffffu;
+
+__device__ __forceinline__ bool lds_hash_add(unsigned* tk, unsigned long long* tv, unsigned key,
+                                             unsigned long long w) {
+    const unsigned h = (key * 2654435761u) >> (32 - kMapSlotsLog2);
+#pragma unroll 1
+    for (int probe = 0; probe < kProbe; ++probe) {
+        const unsigned s = (h + (unsigned)probe) & (kMapSlots - 1);
+        unsigned cur = tk[s];
+        if (cur == kEmptyKey) cur = atomicCAS(&tk[s], kEmptyKey, key);
+        if (cur == kEmptyKey || cur == key) {
+            atomicAdd(&tv[s], w);
+            return true;
+        }
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, long long slice,
+                                                      DistArgs da, int use_hash,
                                                       unsigned long long* gglob,
                                                       unsigned long long* gloc0,
                                                       unsigned long long* gloc1,
                                                       unsigned long long* gloc2,
                                                       unsigned long long* gloc3) {
     extern __shared__ unsigned long long sh[];
+    __shared__ unsigned tkey[kMapSlots];
+    __shared__ unsigned long long tval[kMapSlots];
     const int nglob = da.npp * da.nmu;
     for (int b = threadIdx.x; b < nglob; b += blockDim.x) sh[b] = 0ull;
+    for (int s = threadIdx.x; s < kMapSlots; s += blockDim.x) {
+        tkey[s] = kEmptyKey;
+        tval[s] = 0ull;
+    }
     __syncthreads();
     unsigned long long* gl[4] = {gloc0, gloc1, gloc2, gloc3};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
+    const long long i0 = (long long)blockIdx.x * slice;
+    const long long i1 = i0 + slice < n ? i0 + slice : n;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const double p = P.p[i], mu = P.mu[i];
         const unsigned long long wfx = weight_fx(P.w[i]);
         const double lp = log10(p);
@@ -569,37 +612,64 @@ __global__ void __launch_bounds__(kBlock) dist_kernel(Ptl P, long long n, DistAr
                 const long long idx =
                     (imu - 1) + (long long)L.nmu * ((ip - 1) + (long long)L.npbins *
                                  ((ix - 1) + L.nr[0] * ((iy - 1) + L.nr[1] * (iz - 1))));
-                atomicAdd(&gl[k][idx], wfx);
+                if (!use_hash || !lds_hash_add(tkey, tval, (unsigned)(idx * 4 + k), wfx))
+                    atomicAdd(&gl[k][idx], wfx);
             }
         }
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nglob; b += blockDim.x)
         if (sh[b]) atomicAdd(&gglob[b], sh[b]);
+    for (int s = threadIdx.x; s < kMapSlots; s += blockDim.x) {
+        const unsigned key = tkey[s];
+        if (key != kEmptyKey) atomicAdd(&gl[key & 3u][key >> 2], tval[s]);
+    }
+}
+
+// f(p) per parameter variant: the global momentum bins of dist_kernel
+// (summed over mu) for the particles whose origin is each variant, plus the
+// particle count of each variant.  A block owns LDS bins for the chunk of
+// variants blockIdx.y * vchunk .. + vchunk - 1 and reads its contiguous
+// slice of particles, skipping other variants; it adds its bins and counts
+// to global memory once, non-zero entries only.  Fixed-point weights, so the
+// sums do not depend on the order of the atomics.
+constexpr int kVarBlock = 1024;
+constexpr int kVarBins = 16384;        // 128 KiB of u64 bins per block
+constexpr int kVarCounts = 1024;
+
+__global__ void __launch_bounds__(kVarBlock) variant_dist_kernel(Ptl P, long long n,
+                                                                 long long slice, DistArgs da,
+                                                                 int nvar, int vchunk,
+                                                                 unsigned long long* hist,
+                                                                 unsigned long long* counts) {
+    __shared__ unsigned long long bins[kVarBins];
+    __shared__ unsigned cnt[kVarCounts];
+    const int v0 = (int)blockIdx.y * vchunk;
+    const int nv = nvar - v0 < vchunk ? nvar - v0 : vchunk;
+    const int nb = nv * da.npp;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) bins[b] = 0ull;
+    for (int b = threadIdx.x; b < nv; b += blockDim.x) cnt[b] = 0u;
+    __syncthreads();
+    const long long i0 = (long long)blockIdx.x * slice;
+    const long long i1 = i0 + slice < n ? i0 + slice : n;
+    for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int v = P.origin[i] - v0;
+        if ((unsigned)v >= (unsigned)nv) continue;
+        atomicAdd(&cnt[v], 1u);
+        const double p = P.p[i];
+        if (!(p > da.pmin && p <= da.pmax)) continue;
+        const long long ip = floor_ll((log10(p) - da.pmin_log) / da.dp_log) + 1;
+        if (ip >= 1 && ip <= da.npp) atomicAdd(&bins[(ip - 1) + (long long)da.npp * v], weight_fx(P.w[i]));
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        if (bins[b]) atomicAdd(&hist[(long long)da.npp * v0 + b], bins[b]);
+    for (int b = threadIdx.x; b < nv; b += blockDim.x)
+        if (cnt[b]) atomicAdd(&counts[v0 + b], (unsigned long long)cnt[b]);
 }
 
 // quick_check / get_pmax_global partials, one row per block:
 // {sum w (fixed point, as double bits), sum dt, min dt, max dt, max p}
-// f(p) per parameter variant: the global momentum bins of dist_kernel
-// (summed over mu) for the particles whose origin is each variant, plus the
-// particle count of each variant.  Fixed-point weights, so the sums do not
-// depend on the order of the atomics.
-__global__ void __launch_bounds__(kBlock) variant_dist_kernel(Ptl P, long long n, DistArgs da,
-                                                              int nvar,
-                                                              unsigned long long* hist,
-                                                              unsigned long long* counts) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int v = P.origin[i];
-    if (v < 0 || v >= nvar) return;
-    atomicAdd(&counts[v], 1ull);
-    const double p = P.p[i];
-    if (!(p > da.pmin && p <= da.pmax)) return;
-    const long long ip = floor_ll((log10(p) - da.pmin_log) / da.dp_log) + 1;
-    if (ip >= 1 && ip <= da.npp)
-        atomicAdd(&hist[(ip - 1) + (long long)da.npp * v], weight_fx(P.w[i]));
-}
-
 __global__ void __launch_bounds__(kBlock) stats_kernel(Ptl P, long long n, double* part) {
     __shared__ double s_sum[kBlock / kWave], s_min[kBlock / kWave], s_max[kBlock / kWave],
         s_pmx[kBlock / kWave];
@@ -731,20 +801,34 @@ hipError_t launch_split_apply(const Ptl& P, long long n, long long nmax, const S
 }
 
 hipError_t launch_dist(const Ptl& P, long long n, const DistArgs& da, unsigned long long* gglob,
-                       unsigned long long* const gloc[4], int grid, hipStream_t s) {
+                       unsigned long long* const gloc[4], const long long loc_size[4], int grid,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    // a hash key holds 4 * bin + set in 32 bits (all ones is the empty slot)
+    int use_hash = 1;
+    for (int k = 0; k < 4; ++k)
+        if (da.local_dist && da.loc[k].enabled && loc_size[k] >= (1ll << 30)) use_hash = 0;
+    const long long slice = ((n + grid - 1) / grid + kBlock - 1) / kBlock * kBlock;
+    const int nblk = (int)((n + slice - 1) / slice);
     const size_t shm = sizeof(unsigned long long) * (size_t)da.npp * da.nmu;
-    hipLaunchKernelGGL(dist_kernel, dim3(grid), dim3(kBlock), shm, s, P, n, da, gglob, gloc[0],
-                       gloc[1], gloc[2], gloc[3]);
+    hipLaunchKernelGGL(dist_kernel, dim3(nblk), dim3(kBlock), shm, s, P, n, slice, da, use_hash,
+                       gglob, gloc[0], gloc[1], gloc[2], gloc[3]);
     return hipGetLastError();
 }
 
 hipError_t launch_variant_dist(const Ptl& P, long long n, const DistArgs& da, int nvar,
-                               unsigned long long* hist, unsigned long long* counts, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(variant_dist_kernel, dim3(grid), dim3(kBlock), 0, s, P, n, da, nvar, hist,
-                       counts);
+                               unsigned long long* hist, unsigned long long* counts, int ncu,
+                               hipStream_t s) {
+    if (n <= 0 || nvar <= 0) return hipSuccess;
+    if (da.npp > kVarBins) return hipErrorInvalidValue;
+    const int vchunk = std::min(std::min(nvar, kVarBins / da.npp), kVarCounts);
+    const int nchunk = (nvar + vchunk - 1) / vchunk;
+    // one block per CU per variant chunk, each over a contiguous slice
+    const long long want = std::max<long long>(1, ncu);
+    const long long slice = ((n + want - 1) / want + kVarBlock - 1) / kVarBlock * kVarBlock;
+    const unsigned gx = (unsigned)((n + slice - 1) / slice);
+    hipLaunchKernelGGL(variant_dist_kernel, dim3(gx, (unsigned)nchunk), dim3(kVarBlock), 0, s, P,
+                       n, slice, da, nvar, vchunk, hist, counts);
     return hipGetLastError();
 }
 

@@ -418,3 +418,175 @@ def test_f32h_paused_launches_are_bit_identical():
     assert s0 == s1
     for k in g0:
         np.testing.assert_array_equal(g0[k], g1[k], err_msg=k)
+
+
+# ---------------------------------------------------------------------------
+# Acceleration-region gate (particle_in_acceleration_region,
+# particle_module.f90:2885-2906; applied at :3592-3600 and :4888-4900): a box
+# covering part of a domain that is neither at the origin nor of unit length.
+ACC_BOX = (0.2, 0.65, 0.3, 0.9, 0.1, 0.7)
+
+
+def _acc_cfg(ndim, n, **over):
+    base = dict(xmin=-0.5, xmax=1.5, lx=2.0, ymin=0.25, ymax=1.75, ly=1.5,
+                acc_region_flag=1, acc_region=ACC_BOX, dt_max_rel=1e-2)
+    if ndim == 2:
+        c = cfg2d(n, dx=2.0 / n, dy=1.5 / n, pbcy=1, **base)
+    else:
+        c = cfg3d(n, dx=2.0 / n, dy=1.5 / n, dz=0.5 / n, zmin=-1.0, zmax=-0.5, lz=0.5, pbcx=1,
+                  **base)
+    c.update(over)
+    return c
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_mover_acc_region_f64_matches_oracle(ndim):
+    c = _acc_cfg(ndim, 32 if ndim == 2 else 12)
+    steps = _mover_case(c, nodes_for(c), 2000, 0.2)
+    assert steps > 2000 * 50
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_step_once_acc_region_gate(ndim, precision):
+    """Both step paths: the gate changes only the momentum, keeps the ungated
+    increment inside the box and zeroes it outside (the 0.25 p0 floor still
+    applies); membership is decided by numpy on the positions returned."""
+    c1 = _acc_cfg(ndim, 32 if ndim == 2 else 12)
+    c0 = dict(c1, acc_region_flag=0)
+    nodes = nodes_for(c1)
+    n = 4000
+    a = random_states(c1, n, seed=81)
+    rng = np.random.default_rng(82)
+    low = rng.random(n) < 0.2
+    a["p"] = np.where(low, rng.uniform(0.05, 0.24, n), a["p"])
+    r = rng.uniform(-SQ3, SQ3, size=(n, 4))
+    fixed = rng.integers(0, 2, n).astype(np.int8)
+    state = np.stack([a[k] for k in STATE_COLS], 1)
+    out = {}
+    for name, cc in (("off", c0), ("on", c1)):
+        pz, _ = make(cc, nodes, precision=precision)
+        out[name] = pz.step_once(state, r, fixed, 0.0, 0.05)
+        pz.close()
+    (s0, d0), (s1, d1) = out["off"], out["on"]
+    np.testing.assert_array_equal(np.delete(s1, 3, axis=1), np.delete(s0, 3, axis=1))
+    np.testing.assert_array_equal(d1[:, :3], d0[:, :3])
+    xn = (s0[:, 0] - c1["xmin"]) / c1["lx"]
+    yn = (s0[:, 1] - c1["ymin"]) / c1["ly"]
+    inside = (xn >= ACC_BOX[0]) & (xn <= ACC_BOX[1]) & (yn >= ACC_BOX[2]) & (yn <= ACC_BOX[3])
+    if ndim == 3:
+        zn = (s0[:, 2] - c1["zmin"]) / c1["lz"]
+        inside &= (zn >= ACC_BOX[4]) & (zn <= ACC_BOX[5])
+    assert 0.1 * n < inside.sum() < 0.6 * n
+    np.testing.assert_array_equal(s1[inside, 3], s0[inside, 3])
+    np.testing.assert_array_equal(d1[inside, 3], d0[inside, 3])
+    p_in = state[:, 3]
+    keep = ~inside & (p_in >= 0.25 * c1["p0"])
+    floor = ~inside & (p_in < 0.25 * c1["p0"])
+    assert keep.sum() > 0 and floor.sum() > 0
+    np.testing.assert_array_equal(s1[keep, 3], p_in[keep])
+    assert np.all(d1[keep, 3] == 0.0)
+    assert np.all(s1[floor, 3] == 0.25 * c1["p0"])
+    np.testing.assert_array_equal(d1[floor, 3], 0.25 * c1["p0"] - p_in[floor])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_mover_acc_region_f32_close_to_f64(ndim):
+    """Whole interval with the gate on: the FP32 path's momenta stay within
+    FP32 rounding of the FP64 path's, particle by particle."""
+    c = _acc_cfg(ndim, 32 if ndim == 2 else 12)
+    nodes = nodes_for(c)
+    res = {}
+    for prec in ("f64", "f32"):
+        pz, _ = make(c, nodes, precision=prec, nptl_max=8000)
+        box = (c["xmin"], c["ymin"], c["zmin"], c["xmax"], c["ymax"], c["zmax"])
+        pz.inject_particles_spatial_uniform(6000, 1e-7, 1, 5.0, 0.0, 0.2, box)
+        pz.particle_mover(0.0, 0.2, 1, 10)
+        res[prec] = by_id(pz.get_particles())
+        pz.close()
+    a, b = res["f64"], res["f32"]
+    common = np.intersect1d(a["tag_injected"], b["tag_injected"])
+    assert len(common) > 0.98 * len(a["p"])
+    ia = np.searchsorted(a["tag_injected"], common)
+    ib = np.searchsorted(b["tag_injected"], common)
+    dp = np.abs(a["p"][ia] - b["p"][ib]) / a["p"][ia]
+    assert np.median(dp) < 1e-4
+    # particles that never entered the box keep their injected momentum
+    # exactly; both paths agree on which particles those are, except where a
+    # trajectory grazes a face of the box
+    never_a, never_b = a["p"][ia] == 1.0, b["p"][ib] == 1.0
+    assert never_a.sum() > 0.05 * len(common)
+    assert np.sum(never_a != never_b) <= 0.002 * len(common)
+
+
+# ---------------------------------------------------------------------------
+# Interval-end undo and fixed-dt redo against the closed form (uniform B,
+# constant divV, dt_min = dt_max, dtf not a multiple of dt): p after the
+# interval is p0 (1 - divV dt / 3)^N (1 - divV (dtf - N dt) / 3) after N + 2
+# executed steps (particle_module.f90:1600-1803).
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fixed_dt_redo_matches_closed_form(precision):
+    from test_oracle_physics import redo_case, redo_closed_form, redo_particles
+    c, nodes, dtf, dt, ndt, a = redo_case()
+    n = 4000
+    pin = redo_particles(n)
+    pz, _ = make(c, nodes, precision=precision, nptl_max=n)
+    pz.set_particles(oracle.soa_from_particles(pin))
+    st = pz.particle_mover(0.0, dtf, 1, 10)
+    g = by_id(pz.get_particles())
+    pz.close()
+    assert st["steps"] == n * (ndt + 2) and st["nptl_current"] == n
+    np.testing.assert_allclose(g["t"], dtf, rtol=1e-13)
+    # the FP32 path's dt rule rounds dt to FP32; its time stays FP64
+    dt_used = dt if precision == "f64" else float(np.float32(dt))
+    assert np.all(g["dt"] == dt_used)
+    want = redo_closed_form(pin["p"], a, dt_used, dtf, ndt)
+    # FP32 step arithmetic: dp = FP32(dp_dt * dt) per step, ~1e-7 of dp
+    tol = 1e-12 if precision == "f64" else 2e-8
+    np.testing.assert_allclose(g["p"], want, rtol=tol)
+    assert np.all(np.abs(g["p"] / want - 1) < 1e-3 * abs(a * dt / 3.0))
+
+
+def test_split_stops_at_the_fixed_point_limit():
+    """Weights are summed exactly in units of 2^-32.  A split that would give a
+    copy split_times 33 (weight 2^-33) is refused before anything changes;
+    up to split_times 32 the sums of weights stay exact."""
+    from sp_hip import SpError
+    c = cfg2d(16)
+    n = 2000
+    a = random_states(c, n, seed=90)
+    k = np.where(np.arange(n) % 2 == 0, 30, 31).astype(np.int8)
+    a["split_times"] = k
+    a["weight"] = 0.5 ** k.astype(np.float64)
+    a["p"] = 6.0          # above pmin_split p0 1.05^k for every k <= 36
+    a["mu"] = 0.5
+    pz, _ = make(c, nodes_for(c), nptl_max=4 * n)
+    pz.set_particles(oracle.soa_from_particles(a))
+    pz.init_particle_distributions(npp_global=64, nmu_global=1, local_dist=0)
+    pz.split_particle(1.05, 1.0)
+    ref, ns = oracle.split(oracle.params(c), a, 4 * n, 1.05, 1.0)
+    assert ns == n and pz.counts()["nptl_split"] == n
+    g = by_id(pz.get_particles())
+    r = by_id(oracle.soa_from_particles(ref))
+    for key in r:
+        np.testing.assert_array_equal(g[key], r[key], err_msg=key)
+    assert g["split_times"].max() == 32 and g["weight"].min() == 2.0 ** -32
+    # the weight sum is conserved exactly: 1000 (2^-30 + 2^-31)
+    want = (n // 2) * (2.0 ** -30 + 2.0 ** -31)
+    assert pz.quick_check()["ntot"] == want
+    fg, _ = pz.calc_particle_distributions()
+    assert fg.sum() == want
+    # the next split would create weights 2^-33: refused, particles unchanged
+    with pytest.raises(SpError, match="split_times"):
+        pz.split_particle(1.05, 1.0)
+    g2 = by_id(pz.get_particles())
+    for key in g:
+        np.testing.assert_array_equal(g2[key], g[key], err_msg=key)
+    assert pz.counts()["nptl_split"] == n
+    # uploads must hold weights the exact sums can represent
+    bad = oracle.soa_from_particles(a)
+    bad["weight"] = bad["weight"].copy()
+    bad["weight"][7] = 0.3
+    with pytest.raises(SpError, match="multiple of 2\\^-32"):
+        pz.set_particles(bad)
+    pz.close()

@@ -109,7 +109,7 @@ def _shock_nodes(nx, ny, dx, xs, width, u1, r):
     return out.astype(np.float32)
 
 
-@pytest.mark.parametrize("r", [2.0, 4.0])
+@pytest.mark.parametrize("r", [2.0, 3.0, 4.0])
 def test_shock_power_law_index(r):
     """Particles injected at the shock at p0 are followed until they leave the
     box; the time integral of f(p) is the steady-state spectrum of continuous

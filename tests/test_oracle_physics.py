@@ -528,3 +528,200 @@ def test_nlgc_step_uses_the_particle_mu():
         _, d_b = push(prm, 3, f, k, rands=(0.0, 0.0, 0.0, 0.0))
         rows.append(np.abs(d_a[:3] - d_b[:3]).max())
     assert rows[0] == 0.0 and rows[1] > 0.0
+
+
+# ---------------------------------------------------------------------------
+# Acceleration-region gate (particle_in_acceleration_region,
+# particle_module.f90:2885-2906, applied at :3592-3600 and :4888-4900): the
+# momentum increment is kept only when the particle's new position, normalised
+# to (x - xmin) / lx, lies in the closed box acc_region; outside it the
+# increment is zero and the 0.25 p0 floor still applies.  The pins compare
+# the gated step with the ungated one on the same state and increments, and
+# decide membership with numpy on the positions the step produced.
+
+def _shifted_cfg(ndim, n, **over):
+    """A domain that is neither at the origin nor of unit length, so that a
+    normalisation without xmin or lx puts particles in the wrong box."""
+    base = dict(xmin=-0.5, xmax=1.5, lx=2.0, ymin=0.25, ymax=1.75, ly=1.5)
+    if ndim == 2:
+        c = cfg2d(n, dx=2.0 / n, dy=1.5 / n, **base)
+    else:
+        c = cfg3d(n, dx=2.0 / n, dy=1.5 / n, dz=0.5 / n, zmin=-1.0, zmax=-0.5, lz=0.5, **base)
+    c.update(over)
+    return c
+
+
+def _box_member(c, x, y, z, box):
+    ndim = c["ndim"]
+    xn = (x - c["xmin"]) / c["lx"]
+    yn = (y - c["ymin"]) / c["ly"]
+    m = (xn >= box[0]) & (xn <= box[1]) & (yn >= box[2]) & (yn <= box[3])
+    if ndim == 3:
+        zn = (z - c["zmin"]) / c["lz"]
+        m &= (zn >= box[4]) & (zn <= box[5])
+    return m
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_acceleration_region_gate_keeps_dp_inside_only(ndim):
+    from helpers import nodes_for, random_states
+    box = (0.2, 0.65, 0.3, 0.9, 0.1, 0.7)
+    c0 = _shifted_cfg(ndim, 32 if ndim == 2 else 12)
+    c1 = dict(c0, acc_region_flag=1, acc_region=box)
+    nodes = nodes_for(c0)
+    p0, p1 = oracle.params(c0), oracle.params(c1)
+    fa = oracle.gradients(p0, nodes)
+    n = 3000
+    a = random_states(c0, n, seed=71)
+    # a fifth of the particles start below the 0.25 p0 floor
+    rng = np.random.default_rng(72)
+    low = rng.random(n) < 0.2
+    a["p"] = np.where(low, rng.uniform(0.05, 0.24, n), a["p"])
+    r = rng.uniform(-np.sqrt(3), np.sqrt(3), size=(n, 4))
+    fixed = rng.integers(0, 2, n)
+    inside = np.zeros(n, bool)
+    for i in range(n):
+        g0, d0 = oracle.step_once(p0, fa, a[i], 0.0, 0.05, fixed[i], r[i])
+        g1, d1 = oracle.step_once(p1, fa, a[i], 0.0, 0.05, fixed[i], r[i])
+        # the gate changes nothing but the momentum
+        for k in ("x", "y", "z", "t", "dt"):
+            assert g1[k] == g0[k], k
+        np.testing.assert_array_equal(d1[:3], d0[:3])
+        inside[i] = _box_member(c0, g0["x"], g0["y"], g0["z"], box)
+        if inside[i]:
+            assert g1["p"] == g0["p"] and d1[3] == d0[3]
+        elif a["p"][i] >= 0.25 * c0["p0"]:
+            assert g1["p"] == a["p"][i] and d1[3] == 0.0
+        else:
+            # outside the region below the floor: the floor lifts p anyway
+            assert g1["p"] == 0.25 * c0["p0"]
+            assert d1[3] == 0.25 * c0["p0"] - a["p"][i]
+    # the box covers part of the domain: both outcomes occur
+    assert 0.1 * n < inside.sum() < 0.6 * n
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_acceleration_region_edges_are_inclusive(ndim):
+    """A particle exactly on a face of the region is accelerated; one ulp
+    outside is not (V = 0, uniform B, no noise: the step does not move it)."""
+    c = _shifted_cfg(ndim, 16, acc_region_flag=1, acc_region=(0.25, 0.75, 0.5, 1.0, 0.0, 0.5))
+    prm = oracle.params(c)
+    f = np.zeros(32)
+    f[4], f[7] = 1.0, 1.0          # B along x, |B| = 1
+    f[8] = 0.6                     # dvx/dx: dp_dt = -p divV / 3 != 0
+    k = kappa(prm, 1.0, f)
+    ctr = [c["xmin"] + 0.5 * c["lx"], c["ymin"] + 0.75 * c["ly"], c["zmin"] + 0.25 * c["lz"]]
+    # faces: (axis, normalised value) with lx, ly, lz exact powers of two or 1.5
+    faces = [(0, 0.25), (0, 0.75), (1, 0.5), (1, 1.0)]
+    if ndim == 3:
+        faces += [(2, 0.0), (2, 0.5)]
+    lo = (c["xmin"], c["ymin"], c["zmin"])
+    ln = (c["lx"], c["ly"], c["lz"])
+
+    def dp_at(pos):
+        row = np.zeros(1, dtype=oracle.PARTICLE_DTYPE)
+        row["x"], row["y"], row["z"] = pos
+        row["p"], row["dt"], row["weight"], row["count_flag"] = 1.0, 1e-3, 1.0, 1
+        d = np.zeros(4)
+        fn = oracle.load().orc_push_particle_2d if ndim == 2 else oracle.load().orc_push_particle_3d
+        fn(C.byref(prm), oracle.dptr(np.array([1e-9, 1.0])), row.ctypes.data_as(C.c_void_p),
+           oracle.dptr(np.ascontiguousarray(f)), C.byref(k), 1,
+           oracle.dptr(np.zeros(4)), oracle.dptr(d))
+        assert (row["x"][0], row["y"][0], row["z"][0]) == tuple(pos)
+        return d[3]
+
+    assert dp_at(ctr) == -(1.0 * 0.6 / 3.0) * 1e-3
+    for ax, v in faces:
+        pos = list(ctr)
+        pos[ax] = lo[ax] + v * ln[ax]
+        assert (pos[ax] - lo[ax]) / ln[ax] == v  # exactly on the face
+        assert dp_at(pos) != 0.0, (ax, v)
+        # step outward until the normalised coordinate leaves the face
+        upper = v == max(vv for a2, vv in faces if a2 == ax)
+        on = pos[ax]
+        for e in range(64):
+            step = np.spacing(abs(lo[ax]) + abs(on)) * 2.0 ** e
+            pos[ax] = on + step if upper else on - step
+            if (pos[ax] - lo[ax]) / ln[ax] != v:
+                break
+        xn = (pos[ax] - lo[ax]) / ln[ax]
+        assert (xn > v) if upper else (xn < v)
+        assert dp_at(pos) == 0.0, (ax, v)
+
+
+# ---------------------------------------------------------------------------
+# Interval-end undo and fixed-dt redo (particle_mover_one_cycle,
+# particle_module.f90:1707-1803) against the closed form: with a uniform B,
+# a constant divergence divV and dt_min = dt_max, every adaptive step
+# multiplies p by (1 - divV dt / 3) (dpp = 0: no momentum noise).  With dtf
+# not a multiple of dt, N = floor(dtf / dt) steps stay inside the interval,
+# step N + 1 overshoots and is undone, and the redo of length dtf - N dt
+# lands on t0 + dtf:  p = p0 (1 - divV dt / 3)^N (1 - divV (dtf - N dt) / 3),
+# after N + 2 executed steps.  A redo that forgot to undo Delta p, or that
+# used dt instead of the remainder, misses by a factor of 1 - divV dt / 3.
+
+def redo_case(n_grid=64, frac=0.37, ndt=300):
+    """(config, node fields, dtf, dt, N, divV) of the closed-form redo case:
+    vx = a (x - 1/2) on a power-of-two grid, so that node values and their
+    central differences are exact in FP32 and divV = a everywhere."""
+    a = -0.75
+    dtf = 0.05
+    dt = dtf / (ndt + frac)
+    c = cfg2d(n_grid, pbcx=0, pbcy=0, dt_min_rel=dt / dtf, dt_max_rel=dt / dtf, kret=0.1,
+              kpara0=1e-6)
+    nzg, nyg, nxg = 1, n_grid + 4, n_grid + 4
+    x = workloads_node_x(n_grid)
+    nodes = np.zeros((nzg, nyg, nxg, 8), dtype=np.float32)
+    nodes[..., 0] = (a * (x - 0.5))[None, None, :]
+    nodes[..., 3] = 1.0
+    nodes[..., 4] = 1.0
+    nodes[..., 7] = 1.0
+    assert np.all(nodes[..., 0].astype(np.float64) == (a * (x - 0.5))[None, None, :])
+    return c, nodes, dtf, dt, ndt, a
+
+
+def workloads_node_x(n):
+    from sp_hip import workloads
+    return workloads.node_coords(n, 1.0 / n, 0.0)
+
+
+def redo_particles(n, p0=1.0):
+    rng = np.random.default_rng(5)
+    a = np.zeros(n, dtype=oracle.PARTICLE_DTYPE)
+    # near the centre: |vx| <= 0.75 * 0.1, drift over dtf ~ 4e-3 << distance to the edges
+    a["x"] = rng.uniform(0.4, 0.6, n)
+    a["y"] = rng.uniform(0.4, 0.6, n)
+    a["p"] = p0 * np.exp(rng.uniform(-1.0, 1.0, n))
+    a["weight"] = 1.0
+    a["count_flag"] = 1
+    a["tag_injected"] = np.arange(n)
+    a["tag_splitted"] = 1
+    return a
+
+
+def redo_closed_form(p_start, a, dt, dtf, ndt):
+    g = 1.0 - a * dt / 3.0
+    return p_start * g ** ndt * (1.0 - a * (dtf - ndt * dt) / 3.0)
+
+
+def test_fixed_dt_redo_matches_closed_form():
+    c, nodes, dtf, dt, ndt, a = redo_case()
+    prm = oracle.params(c)
+    fa = oracle.gradients(prm, nodes)
+    # the gradient arrays hold divV = a exactly (dvx/dx; dvy/dy = 0)
+    assert np.all(fa[0, 2:-2, 2:-2, 8] == np.float32(a))
+    assert np.all(fa[0, 2:-2, 2:-2, 12] == 0.0)
+    n = 2000
+    pin = redo_particles(n)
+    out, acc = oracle.mover(prm, fa, pin, 0.0, dtf, nthreads=8)
+    assert len(out) == n and acc.leak == 0.0 and acc.leak_negp == 0.0
+    o = np.argsort(out["tag_injected"])
+    out = out[o]
+    np.testing.assert_allclose(out["t"], dtf, rtol=1e-13)
+    want = redo_closed_form(pin["p"], a, dt, dtf, ndt)
+    np.testing.assert_allclose(out["p"], want, rtol=1e-12)
+    # a redo that keeps the undone step's momentum change is off by (1 - a dt / 3)
+    assert np.all(np.abs(out["p"] / want - 1) < 1e-3 * abs(a * dt / 3.0))
+    # N in-interval steps, the undone overshoot and the redo
+    assert acc.steps == n * (ndt + 2)
+    assert np.all(out["dt"] == dt)
