@@ -24,10 +24,16 @@ HBM before timing starts.
 
 Printed JSON line (one):
   value          particle-steps/s over the K timed intervals (FP32 gather path)
-  roofline       mover kernel: algorithmic HBM bytes / kernel time (HIP events
-                 on the library's stream) against 8 TB/s; valu_issue: VALU
-                 instructions per step (committed counter run) x steps per
-                 kernel second against the chip's non-packed VALU issue rate
+  roofline       mover kernel against its binding roof, the VALU issue rate:
+                 VALU instructions per particle-step (committed rocprofv3
+                 counter run of the same configuration) x particle-steps per
+                 second of mover kernel time (HIP events on the library's
+                 stream) against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz;
+                 beside it the HBM view (algorithmic gather bytes and the
+                 counter-measured HBM bytes per step against 8 TB/s)
+  c3             the 3D 512^3 snapshot (1e8 particles) on the FP16-gradient
+                 and FP32 node records, each with its rate, roofline and
+                 f(p) / spatial-map L1 against FP64 (--no-3d skips it)
   cpu_baseline   the FP64 CPU oracle (OpenMP, per-thread MT19937 increments) on
                  a bounded sample of the same workload, on this host
   accuracy       f(p) and spatial-map L1 of the FP32 path against the FP64
@@ -50,9 +56,17 @@ from sp_hip import ParkerPusher, workloads  # noqa: E402
 
 METRIC = "particle-steps/s (2D/3D snapshot, 1e8 ptcl) at f(p) L1 err ≤1% vs FP64 SDE"
 HBM_PEAK_GBS = 8000.0
-# node components the step consumes x corners of the interpolation cell x 4 B
-BYTES_PER_STEP = {2: 4 * 15 * 4, 3: 8 * 21 * 4}
 COPY_BYTES = 1 << 30
+
+
+def bytes_per_step(ndim, precision):
+    """Algorithmic gather bytes of one particle-step: the stored node-record
+    bytes the step consumes x the corners of its interpolation cell.  2D: 15
+    FP32 components (240 B); 3D: 21 FP32 components (672 B), or with
+    FP16-stored gradients 6 FP32 + 15 FP16 components (432 B)."""
+    if ndim == 2:
+        return 4 * 15 * 4
+    return 8 * (6 * 4 + 15 * 2) if precision == "f32h" else 8 * 21 * 4
 DEFAULTS = {
     "2d": dict(grid=2048, nsteps=2000, nptl=1e8, cpu_nptl=1e6, precision="f32"),
     "3d": dict(grid=512, nsteps=1000, nptl=1e8, cpu_nptl=2e5, precision="f32h"),
@@ -213,42 +227,86 @@ def cpu_baseline(wl, n, nthreads):
     return steps / dt, steps, dt
 
 
-def measured_traffic(wl, n, precision):
-    """HBM bytes per mover launch from the committed rocprofv3 counter runs,
-    when they were taken on this workload; None otherwise."""
-    path = os.path.join(ROOT, "profiles", f"mover_{wl.kind}_{wl.grid}_{n:.0e}_traffic.json"
-                        .replace("+0", ""))
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    want = dict(grid=wl.grid, nptl=n, precision=precision, nsteps=wl.nsteps)
-    if any(t.get("workload", {}).get(k) != v for k, v in want.items()):
-        return None
-    return t.get("traffic_bytes_per_launch")
-
-
 # MI355X non-packed VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_LANE_INSTR_PEAK = 256 * 4 * 16 * 2.4e9
+COUNTERS = os.path.join(ROOT, "profiles", "mover_counters.json")
 
 
-def instruction_issue(wl, precision, steps_per_s):
-    """The mover's VALU issue rate against the chip's, from the committed
-    counter run's VALU instructions per particle-step (a per-step figure that
-    does not depend on the particle count); None for other workloads."""
-    path = os.path.join(ROOT, "profiles", "mover_counters_1e7.json")
+def counter_run(wl, n, precision):
+    """The committed rocprofv3 counter figures of the mover on exactly this
+    configuration (workload, grid, particles, steps per interval, precision),
+    or None."""
     try:
-        with open(path) as f:
-            t = json.load(f).get(wl.kind)
+        with open(COUNTERS) as f:
+            t = json.load(f).get(f"{wl.kind}_{precision}")
     except (OSError, ValueError):
         return None
-    if not t or t.get("precision") != precision:
+    if not t:
         return None
-    lane_rate = t["valu_instr_per_step"] * steps_per_s
-    return {"valu_instr_per_step": t["valu_instr_per_step"], "achieved_lane_instr_per_s": lane_rate,
-            "peak_lane_instr_per_s": VALU_LANE_INSTR_PEAK, "frac": lane_rate / VALU_LANE_INSTR_PEAK,
-            "source": t["source"]}
+    want = dict(grid=wl.grid, nptl=n, nsteps=wl.nsteps, precision=precision)
+    if any(t.get(k) != v for k, v in want.items()):
+        return None
+    return t
+
+
+def roofline(wl, n, precision, steps, kernel_ms, launches):
+    """The mover against its binding roof (VALU issue; the gathers are served
+    by L2 / MALL, so the algorithmic bytes overstate what reaches HBM), with
+    the HBM figures beside it.  Falls back to the HBM roof when no counter
+    run of this configuration is committed."""
+    rate = steps / (kernel_ms * 1e-3)
+    alg = bytes_per_step(wl.ndim, precision)
+    t = counter_run(wl, n, precision)
+    hbm = {"algorithmic_bytes_per_step": alg, "algorithmic_GBs": alg * rate / 1e9,
+           "algorithmic_frac": alg * rate / 1e9 / HBM_PEAK_GBS, "peak_GBs": HBM_PEAK_GBS,
+           "measured_bytes_per_step": t.get("hbm_bytes_per_step") if t else None,
+           "measured_GBs": t["hbm_bytes_per_step"] * rate / 1e9 if t and t.get("hbm_bytes_per_step") else None}
+    kern = "mover_kernel (persistent; one launch per MHD interval)"
+    if t and t.get("valu_instr_per_step"):
+        lane_rate = t["valu_instr_per_step"] * rate
+        return {"bound": "valu_issue", "achieved": lane_rate / 1e12,
+                "peak": VALU_LANE_INSTR_PEAK / 1e12, "unit": "T lane-instr/s",
+                "frac": lane_rate / VALU_LANE_INSTR_PEAK,
+                "traffic": t.get("traffic_bytes_per_launch"), "kernel": kern,
+                "valu_instr_per_step": t["valu_instr_per_step"], "launches": launches,
+                "launch_ms": kernel_ms / max(launches, 1), "source": t.get("source"), "hbm": hbm}
+    return {"bound": "hbm", "achieved": hbm["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hbm["algorithmic_frac"], "traffic": None, "kernel": kern, "launches": launches,
+            "launch_ms": kernel_ms / max(launches, 1), "hbm": hbm,
+            "note": "no committed counter run of this configuration"}
+
+
+def c3_extra(a):
+    """The 3D snapshot on both record formats: rate, roofline and accuracy."""
+    d = DEFAULTS["3d"]
+    wl = Workload("3d", d["grid"], d["nsteps"], d["nptl"])
+    n = wl.nptl
+    out = {"workload": f"{wl.desc}, {n:.3g} particles, {wl.nsteps} diffusion-limited steps "
+                       "per MHD interval", "grid": [wl.grid] * 3, "nptl": n}
+    for prec in ("f32h", "f32"):
+        with ParkerPusher(**wl.config(prec, n)) as pz:
+            wl.setup(pz, n)
+            r = run_intervals(wl, pz, 1, 1)
+        e = {"value": r["steps"] / r["seconds"], "unit": "particle-steps/s",
+             "particle_steps_timed": r["steps"], "seconds": r["seconds"],
+             "mover_kernel_ms": r["kernel_ms"],
+             "sort_steps": wl.config(prec, n).get("sort_steps", 0),
+             "roofline": roofline(wl, n, prec, r["steps"], r["kernel_ms"], 1)}
+        out[prec] = e
+        log(f"3d {prec}: {e['value']:.3e} particle-steps/s")
+    if not a.no_accuracy:
+        na = int(a.accuracy_nptl)
+        res = {}
+        for prec in ("f64", "f32h", "f32"):
+            with ParkerPusher(**wl.config(prec, na)) as pz:
+                wl.setup(pz, na)
+                res[prec] = run_intervals(wl, pz, 1, 1)
+        out["f64_sample_steps_per_s"] = res["f64"]["steps"] / res["f64"]["seconds"]
+        for prec in ("f32h", "f32"):
+            out[prec]["accuracy"] = {
+                "sample_nptl": na, "fp_l1_vs_f64": l1(res[prec]["fp"], res["f64"]["fp"]),
+                "map_l1_vs_f64": l1(res[prec]["fmap"], res["f64"]["fmap"])}
+    return out
 
 
 def surrogate_bench(a):
@@ -361,6 +419,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-3d", action="store_true",
+                    help="skip the 3D snapshot entries of the 2D run's line")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the pusher shards by "
@@ -384,8 +444,6 @@ def main():
         log(f"{n} particles, dtf={wl.dtf:.5g}")
         r = run_intervals(wl, pz, a.warmup, a.steps)
     value = r["steps"] / r["seconds"]
-    bytes_per_step = BYTES_PER_STEP[wl.ndim]
-    achieved = bytes_per_step * r["steps"] / (r["kernel_ms"] * 1e-3) / 1e9
     result = {
         "metric": METRIC,
         "value": value,
@@ -406,15 +464,7 @@ def main():
             "position_time_dtype": "f64", "particle_steps_timed": r["steps"],
             "mover_kernel_ms": r["kernel_ms"],
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(wl, n, a.precision),
-            "kernel": "mover_kernel", "bytes_per_step": bytes_per_step,
-            # the gathers are served by L2/MALL (traffic is far below the
-            # algorithmic bytes), so the issue rate is the other bound to read
-            "valu_issue": instruction_issue(wl, a.precision, r["steps"] / (r["kernel_ms"] * 1e-3)),
-        },
+        "roofline": roofline(wl, n, a.precision, r["steps"], r["kernel_ms"], a.steps),
     }
     if not a.no_accuracy:
         na = int(a.accuracy_nptl)
@@ -438,6 +488,8 @@ def main():
                       f"({s} steps, {secs:.1f} s), "
                       "FP64 oracle, AoS particles, 32-component node records, MT19937 per thread",
         }
+    if a.workload == "2d" and not a.no_3d:
+        result["c3"] = c3_extra(a)
     print(json.dumps(result), flush=True)
 
 

@@ -16,7 +16,7 @@ module gpat_hip
     private
     save
     public mhd_configuration, mhd_config, tstamps_mhd, get_variable, load_mhd_config, &
-        init_tstamps_mhd, calc_tstamps_mhd, free_tstamps_mhd, &
+        init_tstamps_mhd, calc_tstamps_mhd, load_tstamps_mhd, free_tstamps_mhd, &
         read_particle_boundary_conditions, read_particle_params, read_diagnostics_params, &
         set_drift_parameters, set_flag_check_drift_2d, set_precision, set_sort_steps, &
         set_dpp_params, set_nlgc, &
@@ -63,6 +63,31 @@ module gpat_hip
     integer(c_int64_t) :: loc_size(4) = 0
     real(dp), allocatable, target :: fescaped(:), fescaped_local(:, :)
     integer(c_int64_t) :: esc_size(4) = 0
+    !< Fortran-order dimensions (nmu, npbins, nrx, nry, nrz) of local set k
+    integer(c_int64_t) :: loc_dims(5, 4) = 0
+    real(dp), allocatable :: mubins_edges_global(:)
+
+    !< HDF5 writer of the distribution files (fortran/sp_h5.c, libsp_h5.so)
+    interface
+        integer(c_int64_t) function sp_h5_open(fname, create) bind(C, name="sp_h5_open")
+            import :: c_int64_t, c_int32_t, c_char
+            character(kind=c_char), intent(in) :: fname(*)
+            integer(c_int32_t), value :: create
+        end function sp_h5_open
+        integer(c_int32_t) function sp_h5_write(fid, name, rank, fdims, data) &
+                bind(C, name="sp_h5_write")
+            import :: c_int64_t, c_int32_t, c_char, c_double
+            integer(c_int64_t), value :: fid
+            character(kind=c_char), intent(in) :: name(*)
+            integer(c_int32_t), value :: rank
+            integer(c_int64_t), intent(in) :: fdims(*)
+            real(c_double), intent(in) :: data(*)
+        end function sp_h5_write
+        integer(c_int32_t) function sp_h5_close(fid) bind(C, name="sp_h5_close")
+            import :: c_int64_t, c_int32_t
+            integer(c_int64_t), value :: fid
+        end function sp_h5_close
+    end interface
 
 contains
 
@@ -145,6 +170,48 @@ contains
         if (allocated(tstamps_mhd)) deallocate(tstamps_mhd)
         allocate(tstamps_mhd(t_end - t_start + 1))
     end subroutine init_tstamps_mhd
+
+    !< load_tstamps_mhd (mhd_config.f90:221-255): dir_mhd_data/time_stamps.dat
+    !< holds 8-byte slots, the first frame of the MHD output (read as a default
+    !< integer from the first slot), the last frame, then one FP64 time per
+    !< frame.  Frames t_start .. min(tmax_mhd, t_end) are read; frames past
+    !< tmax_mhd repeat the last interval.
+    subroutine load_tstamps_mhd(t_start, t_end, tmax_mhd, dir_mhd_data)
+        integer, intent(in) :: t_start, t_end, tmax_mhd
+        character(*), intent(in) :: dir_mhd_data
+        character(len=256) :: filename
+        integer :: fh, ts_mhd, i, nread, ios
+        integer(8) :: fsize
+        real(dp) :: dt_last
+        filename = trim(dir_mhd_data)//"time_stamps.dat"
+        open(newunit=fh, file=filename, access='stream', status='old', form='unformatted', &
+             action='read', iostat=ios)
+        if (ios /= 0) then
+            write(*, "(A,A)") "cannot open ", trim(filename)
+            stop 1
+        endif
+        read(fh, pos=1) ts_mhd
+        nread = min(tmax_mhd, t_end) - t_start + 1
+        inquire(unit=fh, size=fsize)
+        if (t_start < ts_mhd .or. nread < 1 .or. &
+                (int(t_start - ts_mhd + 2, 8) + nread) * 8 > fsize) then
+            write(*, "(A,A,A,I0,A,I0)") trim(filename), " does not hold the frames ", &
+                "from ", t_start, " to ", min(tmax_mhd, t_end)
+            stop 1
+        endif
+        read(fh, pos=(t_start - ts_mhd + 2) * 8 + 1) tstamps_mhd(1:nread)
+        close(fh)
+        if (tmax_mhd < t_end) then
+            if (nread < 2) then
+                write(*, "(A)") "time_stamps.dat: extrapolating past tmax_mhd needs two frames"
+                stop 1
+            endif
+            dt_last = tstamps_mhd(nread) - tstamps_mhd(nread - 1)
+            do i = tmax_mhd + 1, t_end
+                tstamps_mhd(i - t_start + 1) = tstamps_mhd(i - t_start) + dt_last
+            enddo
+        endif
+    end subroutine load_tstamps_mhd
 
     !< calc_tstamps_mhd (mhd_config.f90:263-271)
     subroutine calc_tstamps_mhd(t_start, t_end)
@@ -490,14 +557,17 @@ contains
         dcfg%escaped_dist = merge(1, 0, dump_escaped_dist)
         call check_status(sp_distributions_init(ctx, dcfg), "sp_distributions_init")
         allocate(fglobal(dcfg%npp_global), pbins_edges_global(dcfg%npp_global + 1))
+        allocate(mubins_edges_global(dcfg%nmu_global + 1))
         pmin_log = log10(pmin_)
         dp_log = (log10(pmax_) - pmin_log) / dcfg%npp_global
         do i = 1, dcfg%npp_global + 1
             pbins_edges_global(i) = 10**(pmin_log + (i-1) * dp_log)
         enddo
+        call mu_edges(dcfg%nmu_global, mubins_edges_global)
         do k = 1, 4
             call check_status(sp_local_dist_dims(ctx, int(k, c_int32_t), dims), &
                 "sp_local_dist_dims")
+            loc_dims(:, k) = dims
             loc_size(k) = product(dims)
             if (dims(1) == 0) loc_size(k) = 0
             ! init_local_escaped_distributions (diagnostics.f90:359-395): the
@@ -515,54 +585,141 @@ contains
         endif
     end subroutine init_particle_distributions
 
+    !< mubins edges -1 + (i - 1) dmu with dmu = 2.0 / nmu in default real
+    !< (diagnostics.f90:206-209)
+    subroutine mu_edges(nmu, edges)
+        integer, intent(in) :: nmu
+        real(dp), intent(out) :: edges(:)
+        real(dp) :: dmu
+        integer :: i
+        dmu = 2.0 / nmu
+        do i = 1, nmu + 1
+            edges(i) = -1.0 + (i - 1) * dmu
+        enddo
+    end subroutine mu_edges
+
+    !< Opens (create) or reopens the HDF5 file fname; stops on failure.
+    function h5_file(fname, create) result(fid)
+        character(*), intent(in) :: fname
+        logical, intent(in) :: create
+        integer(c_int64_t) :: fid
+        fid = sp_h5_open(trim(fname)//c_null_char, merge(1_c_int32_t, 0_c_int32_t, create))
+        if (fid < 0) then
+            write(*, "(A,A)") "HDF5: cannot create ", trim(fname)
+            stop 1
+        endif
+    end function h5_file
+
+    !< One FP64 dataset with Fortran-order dimensions dims.
+    subroutine h5_dataset(fid, name, dims, data)
+        integer(c_int64_t), intent(in) :: fid
+        character(*), intent(in) :: name
+        integer(c_int64_t), intent(in) :: dims(:)
+        real(dp), intent(in) :: data(*)
+        if (sp_h5_write(fid, trim(name)//c_null_char, int(size(dims), c_int32_t), dims, &
+                data) /= 0) then
+            write(*, "(A,A)") "HDF5: cannot write dataset ", trim(name)
+            stop 1
+        endif
+    end subroutine h5_dataset
+
+    subroutine h5_done(fid)
+        integer(c_int64_t), intent(in) :: fid
+        if (sp_h5_close(fid) /= 0) then
+            write(*, "(A)") "HDF5: closing a file failed"
+            stop 1
+        endif
+    end subroutine h5_done
+
+    !< save_mu_pbins (diagnostics.f90:1244-1278): the mu and momentum bin edges
+    subroutine save_mu_pbins(fid, nmu, npp, dname_mubins, dname_pbins, pmin, pmax)
+        integer(c_int64_t), intent(in) :: fid
+        integer, intent(in) :: nmu, npp
+        character(*), intent(in) :: dname_mubins, dname_pbins
+        real(dp), intent(in) :: pmin, pmax
+        real(dp) :: mue(nmu + 1), pbe(npp + 1), pmin_log, dp_log
+        integer :: i
+        call mu_edges(nmu, mue)
+        pmin_log = log10(pmin)
+        dp_log = (log10(pmax) - pmin_log) / npp
+        do i = 1, npp + 1
+            pbe(i) = 10**(pmin_log + (i-1) * dp_log)
+        enddo
+        call h5_dataset(fid, dname_mubins, [int(nmu + 1, c_int64_t)], mue)
+        call h5_dataset(fid, dname_pbins, [int(npp + 1, c_int64_t)], pbe)
+    end subroutine save_mu_pbins
+
     !< distributions_diagnostics (diagnostics.f90:1652-1682): f(p) and the local
-    !< distributions, written as fdists_NNNN.dat (stream: npp, pbins edges,
-    !< fglobal, then for each enabled local set its size and values)
+    !< distributions into fdists_NNNN.h5 (save_global_distributions,
+    !< save_local_distributions, :1285-1323, :1420-1488), the distributions of
+    !< the particles that escaped since the previous frame into
+    !< escaped_dists_NNNN.h5 (save_escaped_distributions,
+    !< save_local_escaped_distributions, :1331-1372, :1533-1640): the
+    !< reference's dataset names, Fortran-order shapes and bin edges
     subroutine distributions_diagnostics(iframe, file_path, local_dist, dump_escaped_dist)
         integer, intent(in) :: iframe
         character(*), intent(in) :: file_path
         logical, intent(in) :: local_dist, dump_escaped_dist
         type(c_ptr) :: lp(4)
         character(len=256) :: fname
-        integer :: fh, k
+        character(len=1) :: s
+        integer(c_int64_t) :: fid, d(5), off
+        integer :: k
         do k = 1, 4
             lp(k) = c_null_ptr
             if (local_dist .and. loc_size(k) > 0) lp(k) = c_loc(flocal(1, k))
         enddo
         call check_status(sp_distributions(ctx, c_loc_dp(fglobal), lp(1), lp(2), lp(3), lp(4)), &
             "sp_distributions")
-        write(fname, "(A,A,I4.4,A)") trim(file_path), "fdists_", iframe, ".dat"
-        open(newunit=fh, file=fname, access='stream', status='replace', form='unformatted', &
-             action='write')
-        write(fh) int(dcfg%npp_global, c_int32_t), pbins_edges_global, fglobal
+        write(fname, "(A,A,I4.4,A)") trim(file_path), "fdists_", iframe, ".h5"
+        fid = h5_file(fname, .true.)
+        call save_mu_pbins(fid, dcfg%nmu_global, dcfg%npp_global, "mubins_edges_global", &
+            "pbins_edges_global", pmin_, pmax_)
+        call h5_dataset(fid, "fglobal", [int(dcfg%nmu_global, c_int64_t), &
+            int(dcfg%npp_global, c_int64_t)], fglobal)
         do k = 1, 4
             if (local_dist .and. loc_size(k) > 0) then
-                write(fh) int(k, c_int32_t), loc_size(k), flocal(1:loc_size(k), k)
+                write(s, "(I1)") k
+                call save_mu_pbins(fid, int(loc_dims(1, k)), int(loc_dims(2, k)), &
+                    "mubins"//s//"_edges", "pbins"//s//"_edges", dcfg%local(k)%pmin, &
+                    dcfg%local(k)%pmax)
+                call h5_dataset(fid, "flocal"//s, loc_dims(:, k), flocal(1:loc_size(k), k))
             endif
         enddo
-        close(fh)
+        call h5_done(fid)
         if (.not. dump_escaped_dist) return
-        ! calc_escaped_distributions + save_escaped_distributions
-        ! (diagnostics.f90:913-1231,1331-1372) and the local face arrays, written
-        ! as escaped_dists_NNNN.dat (stream: npp, pbins edges, number of faces,
-        ! fescaped, then for each enabled local set its size and values)
+        ! calc_escaped_distributions (diagnostics.f90:913-1231) happened in the
+        ! mover's removal passes; the face arrays of set k come back to back
         do k = 1, 4
             lp(k) = c_null_ptr
             if (esc_size(k) > 0) lp(k) = c_loc(fescaped_local(1, k))
         enddo
         call check_status(sp_escaped_distributions(ctx, c_loc_dp(fescaped), lp(1), lp(2), &
             lp(3), lp(4)), "sp_escaped_distributions")
-        write(fname, "(A,A,I4.4,A)") trim(file_path), "escaped_dists_", iframe, ".dat"
-        open(newunit=fh, file=fname, access='stream', status='replace', form='unformatted', &
-             action='write')
-        write(fh) int(dcfg%npp_global, c_int32_t), pbins_edges_global, &
-            int(2 * ndim_field, c_int32_t), fescaped
+        write(fname, "(A,A,I4.4,A)") trim(file_path), "escaped_dists_", iframe, ".h5"
+        fid = h5_file(fname, .true.)
+        call save_mu_pbins(fid, dcfg%nmu_global, dcfg%npp_global, "mubins_edges", &
+            "pbins_edges", pmin_, pmax_)
+        call h5_dataset(fid, "fescaped", [int(dcfg%nmu_global, c_int64_t), &
+            int(dcfg%npp_global, c_int64_t), int(2 * ndim_field, c_int64_t)], fescaped)
         do k = 1, 4
-            if (esc_size(k) > 0) then
-                write(fh) int(k, c_int32_t), esc_size(k), fescaped_local(1:esc_size(k), k)
+            if (esc_size(k) == 0) cycle
+            write(s, "(I1)") k
+            d = loc_dims(:, k)
+            call save_mu_pbins(fid, int(d(1)), int(d(2)), "mubins"//s//"_edges", &
+                "pbins"//s//"_edges", dcfg%local(k)%pmin, dcfg%local(k)%pmax)
+            call h5_dataset(fid, "fescaped"//s//"_x", [d(1), d(2), d(4), d(5), 2_c_int64_t], &
+                fescaped_local(1:, k))
+            off = d(1) * d(2) * d(4) * d(5) * 2
+            call h5_dataset(fid, "fescaped"//s//"_y", [d(1), d(2), d(3), d(5), 2_c_int64_t], &
+                fescaped_local(off + 1:, k))
+            if (ndim_field > 2) then
+                off = off + d(1) * d(2) * d(3) * d(5) * 2
+                call h5_dataset(fid, "fescaped"//s//"_z", [d(1), d(2), d(3), d(4), 2_c_int64_t], &
+                    fescaped_local(off + 1:, k))
             endif
         enddo
-        close(fh)
+        call h5_done(fid)
     end subroutine distributions_diagnostics
 
     function c_loc_dp(a) result(p)

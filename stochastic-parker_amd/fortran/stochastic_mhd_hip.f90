@@ -56,7 +56,11 @@ program stochastic_hip
 
     call load_mhd_config(trim(dir_mhd_data)//trim(mhd_config_filename))
     call init_tstamps_mhd(t_start, t_end)
-    call calc_tstamps_mhd(t_start, t_end)
+    if (varying_dt_mhd) then
+        call load_tstamps_mhd(t_start, t_end, tmax_mhd, dir_mhd_data)
+    else
+        call calc_tstamps_mhd(t_start, t_end)
+    endif
     call init_field_data(time_interp_flag, ndim_field)
     call read_particle_boundary_conditions('config/'//trim(conf_file))
     call read_particle_params('config/'//trim(conf_file))
@@ -161,7 +165,6 @@ contains
         if (track_particle_flag) call stop_scope("-tf .true. (particle tracking)")
         if (inject_at_shock) call stop_scope("-is .true. (shock injection)")
         if (inject_large_db2) call stop_scope("-ib .true. (needs the turbulence-amplitude map)")
-        if (varying_dt_mhd) call stop_scope("-vdt .true. (time_stamps.dat)")
         if (ndim_field /= 2 .and. ndim_field /= 3) call stop_scope("-nd other than 2 or 3")
         if (size_mpi_sub /= 1) write(*, "(A)") " -sm ignored: one GPU per run"
     end subroutine check_scope

@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from h5read import read as read_h5
 from sp_hip import ParkerPusher, workloads
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -149,33 +150,29 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision, inject):
         got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
         np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
     assert rows[-1][1]["nptl_split"] > 0
-    # fdists_0002.dat: npp, edges, fglobal, then (set, size, values) of set 1
-    raw = (tmp_path / "data" / "fdists_0002.dat").read_bytes()
-    npp = struct.unpack_from("<i", raw, 0)[0]
-    assert npp == 64
-    off = 4 + 8 * (npp + 1)
-    fglob = np.frombuffer(raw, dtype="<f8", count=npp, offset=off)
-    np.testing.assert_array_equal(fglob, fg)
-    off += 8 * npp
-    kset, size = struct.unpack_from("<iq", raw, off)
-    assert kset == 1 and size == loc[0].size
-    floc = np.frombuffer(raw, dtype="<f8", count=size, offset=off + 12)
-    np.testing.assert_array_equal(floc, loc[0].ravel())
-    # escaped_dists_0002.dat: npp, edges, number of faces, fescaped, then
-    # (set, size, values) of set 1's face arrays
-    raw = (tmp_path / "data" / "escaped_dists_0002.dat").read_bytes()
-    assert struct.unpack_from("<i", raw, 0)[0] == 64
-    off = 4 + 8 * 65
-    assert struct.unpack_from("<i", raw, off)[0] == 4
-    fesc = np.frombuffer(raw, dtype="<f8", count=fe.size, offset=off + 4)
-    np.testing.assert_array_equal(fesc, fe.ravel())
+    # fdists_0002.h5 / escaped_dists_0002.h5: the reference's datasets
+    # (diagnostics.f90:1244-1650), Fortran-order shapes seen reversed in C
+    h = read_h5(str(tmp_path / "data" / "fdists_0002.h5"))
+    assert sorted(h) == ["fglobal", "flocal1", "mubins1_edges", "mubins_edges_global",
+                         "pbins1_edges", "pbins_edges_global"]
+    assert h["fglobal"].shape == (64, 1)
+    np.testing.assert_array_equal(h["fglobal"][:, 0], fg)
+    np.testing.assert_allclose(h["pbins_edges_global"], np.logspace(-2, 4, 65), rtol=1e-13)
+    np.testing.assert_array_equal(h["mubins_edges_global"], [-1.0, 1.0])
+    np.testing.assert_allclose(h["pbins1_edges"], np.logspace(-2, 4, 9), rtol=1e-13)
+    # flocal1 (nmu, npbins, nrx, nry, nrz) = (1, 8, 16, 16, 1)
+    assert h["flocal1"].shape == (1, 16, 16, 8, 1) == loc[0].shape
+    np.testing.assert_array_equal(h["flocal1"], loc[0])
+    e = read_h5(str(tmp_path / "data" / "escaped_dists_0002.h5"))
+    assert sorted(e) == ["fescaped", "fescaped1_x", "fescaped1_y", "mubins1_edges",
+                         "mubins_edges", "pbins1_edges", "pbins_edges"]
+    assert e["fescaped"].shape == (4, 64, 1)
+    np.testing.assert_array_equal(e["fescaped"], fe)
     if inject is None:
         assert fe.sum() > 0   # the jz / divV regions lie away from the open y faces
-    off += 4 + 8 * fe.size
-    kset, size = struct.unpack_from("<iq", raw, off)
-    flat = np.concatenate([esc[0]["x"].ravel(), esc[0]["y"].ravel()])
-    assert kset == 1 and size == flat.size
-    np.testing.assert_array_equal(np.frombuffer(raw, dtype="<f8", count=size, offset=off + 12), flat)
+    for face in ("x", "y"):
+        assert e[f"fescaped1_{face}"].shape == esc[0][face].shape
+        np.testing.assert_array_equal(e[f"fescaped1_{face}"], esc[0][face])
 
 
 @pytest.mark.gpu
@@ -229,10 +226,9 @@ def test_driver_3d_fp16_records_matches_host_api(tmp_path):
         got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
         np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
     assert rows[-1][1]["leak"] > 0   # open along x: some particles leave
-    raw = (tmp_path / "data" / "fdists_0001.dat").read_bytes()
-    assert struct.unpack_from("<i", raw, 0)[0] == 64
-    fglob = np.frombuffer(raw, dtype="<f8", count=64, offset=4 + 8 * 65)
-    np.testing.assert_array_equal(fglob, fg)
+    h = read_h5(str(tmp_path / "data" / "fdists_0001.h5"))
+    np.testing.assert_array_equal(h["fglobal"][:, 0], fg)
+    assert h["flocal1"].shape == (4, 4, 4, 8, 1)
 
 
 @pytest.mark.gpu
@@ -278,3 +274,88 @@ def test_driver_time_interpolation_matches_host_api(tmp_path, precision):
         assert int(line[:6]) == tf
         got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
         np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
+
+
+def write_tstamps(path, first, last, stamps):
+    """time_stamps.dat: 8-byte slots, first and last frame of the MHD output,
+    then one FP64 time per frame (mhd_config.f90:212-234)."""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<qq", first, last))
+        f.write(np.asarray(stamps, dtype="<f8").tobytes())
+
+
+def run_tstamps(d, t_start, t_end, tmax_mhd):
+    exe = os.path.join(BIN, "tstamps_check")
+    _need(exe)
+    r = subprocess.run([exe, str(d) + "/", str(t_start), str(t_end), str(tmax_mhd)],
+                       capture_output=True, text=True)
+    return r.returncode, r.stdout + r.stderr
+
+
+def test_time_stamps_reader(tmp_path):
+    """load_tstamps_mhd (mhd_config.f90:221-255) as the -vdt path of the driver
+    calls it: frames t_start .. min(tmax_mhd, t_end) from the file, later
+    frames spaced by the last read interval."""
+    stamps = [0.0, 0.1, 0.25, 0.3, 0.5, 0.55, 0.7]      # frames 3 .. 9
+    write_tstamps(tmp_path / "time_stamps.dat", 3, 9, stamps)
+    rc, out = run_tstamps(tmp_path, 3, 9, 100000)
+    assert rc == 0, out
+    np.testing.assert_array_equal([float(v) for v in out.split()], stamps)
+    rc, out = run_tstamps(tmp_path, 5, 7, 100000)
+    assert rc == 0, out
+    np.testing.assert_array_equal([float(v) for v in out.split()], stamps[2:5])
+    # tmax_mhd = 8 < t_end = 12: frames 9 .. 12 repeat the interval 8 - 7 (0.05)
+    rc, out = run_tstamps(tmp_path, 4, 12, 8)
+    assert rc == 0, out
+    got = np.array([float(v) for v in out.split()])
+    want = stamps[1:6] + [0.55 + 0.05 * k for k in range(1, 5)]
+    np.testing.assert_allclose(got, want, rtol=1e-14)
+    assert np.all(np.diff(got) > 0)
+    # frames the file does not hold are an error, not garbage
+    for args in ((2, 5, 100000), (3, 12, 100000)):
+        rc, out = run_tstamps(tmp_path, *args)
+        assert rc != 0 and "does not hold" in out
+    rc, out = run_tstamps(tmp_path / "nowhere", 3, 5, 100000)
+    assert rc != 0 and "cannot open" in out
+
+
+@pytest.mark.gpu
+def test_driver_varying_mhd_intervals_match_host_api(tmp_path):
+    """-vdt .true.: the driver takes uneven MHD intervals from time_stamps.dat
+    (here 0.03 then 0.05); quick.dat equals the host path run over the same
+    intervals."""
+    _need(DRIVER)
+    n, npart = 64, 3000
+    nodes = write_case(tmp_path, n, 0.04)
+    stamps = [0.0, 0.03, 0.08]
+    write_tstamps(tmp_path / "mhd" / "time_stamps.dat", 0, 2, stamps)
+    args = [DRIVER, "-dm", "mhd/", "-np", str(npart), "-nm", "20000", "-ts", "0", "-te", "2",
+            "-st", "1", "-df", "1", "-sf", "0", "-nl", ".false.", "-ug", "1", "-nd", "2",
+            "-ld", ".true.", "-dd", "data/", "-dt", "1e-7", "-gp", "f64", "-vdt", ".true."]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    quick = (tmp_path / "data" / "quick.dat").read_text().splitlines()
+    assert len(quick) == 4
+    cfg = workloads.config_2d(n, pbcy=1, dt_max_rel=1e-2, gamma_turb=1.6666667,
+                              precision="f64", nptl_max=20000)
+    rows = []
+    with ParkerPusher(**cfg) as pz:
+        pz.upload_fields(nodes)
+        pz.calc_fields_gradients()
+        pz.inject_particles_spatial_uniform(npart, 1e-7, 1, 5.0, stamps[0], stamps[1] - stamps[0],
+                                            (0.0, 0.0, 0.0, 1.0, 1.0, 1.0))
+        rows.append((0, pz.quick_check()))
+        for tf in (1, 2):
+            pz.particle_mover(stamps[tf - 1], stamps[tf] - stamps[tf - 1], 1, 10)
+            rows.append((tf, pz.quick_check()))
+    keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
+            "pdt_avg")
+    for line, (tf, q) in zip(quick[1:], rows):
+        assert int(line[:6]) == tf
+        got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
+        np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
+    # dt_max = dt_max_rel * dtf follows each interval (the diffusive limit,
+    # (dx/2)^2 / (2 kpara) ~ 0.03, lies above it)
+    assert rows[1][1]["pdt_max"] == pytest.approx(1e-2 * 0.03, rel=1e-12)
+    assert rows[2][1]["pdt_max"] == pytest.approx(1e-2 * 0.05, rel=1e-12)
