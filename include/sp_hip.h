@@ -235,7 +235,8 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
 
 /* ---- launch autotuning (no reference counterpart) -------------------------
  * Times the mover over [t0, t0 + dtf] from the current particles for every
- * pair of resident blocks per CU (1 .. occupancy, at most 4) and step budget
+ * triple of particles per lane (1, or 1 and 2 for the FP32 precisions),
+ * resident 256-lane blocks per CU (1 .. occupancy, at most 4) and step budget
  * between re-sorts (0, 100, 200, 400 when sort_every > 0), keeps the fastest
  * for this context's later mover calls, and restores the particles and
  * counters, so the call changes no result.  Pick dtf so that a trial takes a
@@ -244,7 +245,7 @@ typedef struct sp_tune_result {
     int32_t blocks_per_cu;        /* chosen resident mover blocks per CU */
     int32_t sort_steps;           /* chosen step budget between re-sorts */
     int32_t ntrials;
-    int32_t reserved0;
+    int32_t particles_per_lane;   /* chosen particles each mover lane carries */
     double steps_per_s;           /* mover throughput of the chosen setting */
     double default_steps_per_s;   /* mover throughput of the untuned setting */
 } sp_tune_result;

@@ -45,6 +45,8 @@ constexpr double kInvFx = 1.0 / sp::kFxScale;
 // fit its L2 with two blocks, 3D records (96 B, 8 corners) with one.
 constexpr int kMoverBlocksPerCU2D = 2;
 constexpr int kMoverBlocksPerCU3D = 1;
+// Particles per lane of the FP32 movers unless tuned or overridden.
+constexpr int kMoverPPLDefault = 1;
 
 struct Ctx {
     sp_config cfg{};
@@ -88,7 +90,9 @@ struct Ctx {
     int nvar = 0, nvar_cap = 0;
     unsigned long long* vhist = nullptr;   // per-variant f(p) bins, then per-variant counts
     size_t vhist_len = 0;
-    int mover_blocks_per_cu[3] = {0, 0, 0};   // per precision (F64, F32, F32H)
+    // occupancy of the mover per precision (F64, F32, F32H) and particles per lane (1, 2)
+    int mover_bpc[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    int tuned_ppl = 0;                        // autotuned particles per lane (0: not tuned)
     float gscale[2] = {1.0f, 1.0f};           // FP16 records: gradient scale per frame
     unsigned* gmax = nullptr;
     // distributions
@@ -221,24 +225,41 @@ Ctx* live_ctx(void* p, int& rc) {
     return as_ctx(p);
 }
 
+// Particles each mover lane carries: 1 for the FP64 mover and for two-frame
+// time interpolation; the FP32 movers take 1 or 2.  Precedence, last wins:
+// the default, the autotuner's choice, SP_MOVER_PPL, an explicit trial value.
+int mover_ppl(const Ctx* c, int ppl_trial = 0) {
+    if (c->cfg.precision == SP_PREC_F64 || c->cfg.time_interp) return 1;
+    int ppl = kMoverPPLDefault;
+    if (c->tuned_ppl > 0) ppl = c->tuned_ppl;
+    if (const char* e = std::getenv("SP_MOVER_PPL")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2) ppl = v;
+    }
+    if (ppl_trial == 1 || ppl_trial == 2) ppl = ppl_trial;
+    return ppl;
+}
+
 // Occupancy-sized persistent grid for the mover (bpc > 0: that many blocks
 // per CU, within the occupancy limit).
-int mover_grid(Ctx* c, long long n, int bpc = 0) {
+int mover_grid(Ctx* c, long long n, int bpc = 0, int ppl = 1) {
     const int prec = c->cfg.precision;
+    const int occ = c->mover_bpc[prec][ppl - 1];
     // Resident 256-lane blocks per CU.  Particles in flight are, on each XCD,
     // a window of consecutive (spatially sorted) indices; a smaller window
     // keeps its field records in that XCD's L2.  Measured on MI355X: two
     // blocks per CU on the 2048^2 snapshot, one on the 512^3 snapshot, for
     // both precisions; SP_MOVER_BLOCKS_PER_CU overrides.
     const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D : kMoverBlocksPerCU2D;
-    int per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), per_cu_default);
-    if (c->tuned_bpc > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), c->tuned_bpc);
+    int per_cu = std::min(std::max(1, occ), per_cu_default);
+    if (c->tuned_bpc > 0) per_cu = std::min(std::max(1, occ), c->tuned_bpc);
     if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {
         const int v = std::atoi(e);
-        if (v > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), v);
+        if (v > 0) per_cu = std::min(std::max(1, occ), v);
     }
-    if (bpc > 0) per_cu = std::min(std::max(1, c->mover_blocks_per_cu[prec]), bpc);
-    const long long need = (n + 255) / 256;
+    if (bpc > 0) per_cu = std::min(std::max(1, occ), bpc);
+    const long long lanes = 256ll * ppl;
+    const long long need = (n + lanes - 1) / lanes;
     const long long cap = (long long)c->ncu * per_cu;
     return (int)std::max<long long>(1, std::min(need, cap));
 }
@@ -472,10 +493,12 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     {
         const bool ti = cfg->time_interp != 0;
         const int fx = step_features(*cfg);
-        c->mover_blocks_per_cu[0] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, fx));
-        c->mover_blocks_per_cu[1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, fx));
-        c->mover_blocks_per_cu[2] =
-            cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, fx & sp::FX_ACC)) : 1;
+        c->mover_bpc[0][0] = c->mover_bpc[0][1] = std::max(1, sp::mover_blocks_per_cu_f64(cfg->ndim, ti, fx));
+        for (int ppl = 1; ppl <= 2; ++ppl) {
+            c->mover_bpc[1][ppl - 1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, fx, ppl));
+            c->mover_bpc[2][ppl - 1] =
+                cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, fx & sp::FX_ACC, ppl)) : 1;
+        }
     }
     SP_HIP_C(hipStreamSynchronize(c->st));
 #undef SP_HIP_C
@@ -758,7 +781,8 @@ int sp_inject_particles_threshold(void* ctx, int32_t quantity, int64_t nptl, dou
 // current particles, with re-sorts between launches when a step budget is set.
 // bpc > 0 / sort_steps >= 0 override the launch settings (autotuner trials).
 int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_interval, int bpc,
-              int sort_steps, unsigned long long& steps, float& ms_push) {
+              int sort_steps, int ppl_trial, unsigned long long& steps, float& ms_push) {
+    const int ppl = mover_ppl(c, ppl_trial);
     int rc = SP_OK;
     const bool ti = c->cfg.time_interp != 0;
     const int fx = step_features(c->cfg);
@@ -787,9 +811,9 @@ int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_inte
         // clear the work, step and pause counters (leak counters are cumulative)
         SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
         SP_HIP(hipEventRecord(c->ev[1], c->st));
-        const int grid = mover_grid(c, nrun, bpc);
-        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, fx, grid, c->st));
-        else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, fx, c->cfg.ndim, grid, c->st));
+        const int grid = mover_grid(c, nrun, bpc, ppl);
+        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, fx, ppl, grid, c->st));
+        else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, fx, c->cfg.ndim, ppl, grid, c->st));
         else SP_HIP(sp::launch_mover_f64(m, ti, fx, c->cfg.ndim, grid, c->st));
         SP_HIP(hipEventRecord(c->ev[3], c->st));
         unsigned long long sp_[2] = {0, 0};
@@ -827,7 +851,7 @@ int sp_particle_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     ++c->mover_calls;
     unsigned long long steps = 0;
     float ms_push = 0.f;
-    rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, 0, -1, steps, ms_push);
+    rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, 0, -1, 0, steps, ms_push);
     if (rc != SP_OK) return rc;
     // remove_particles: stable compaction into the other buffer set
     long long kept = 0, esc = 0;
@@ -1138,36 +1162,44 @@ int sp_autotune_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     SP_HIP(hipMemcpyAsync(c->pbuf_save, c->pbuf[cur0], bytes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
     const int prec = c->cfg.precision;
-    const int max_bpc = std::max(1, std::min(4, c->mover_blocks_per_cu[prec]));
     std::vector<int> budgets = {0};
     if (c->cfg.sort_every > 0) budgets = {0, 100, 200, 400};
-    const int saved_bpc = c->tuned_bpc, saved_ss = c->tuned_sort_steps;
+    // (particles per lane, blocks per CU, step budget) triples
+    struct Trial { int ppl, bpc, ss; };
+    std::vector<Trial> trials;
+    for (int ppl = 1; ppl <= mover_ppl(c, 2); ++ppl) {
+        const int max_bpc = std::max(1, std::min(4, c->mover_bpc[prec][ppl - 1]));
+        for (int bpc = 1; bpc <= max_bpc; ++bpc)
+            for (int ss : budgets) trials.push_back(Trial{ppl, bpc, ss});
+    }
+    const int saved_bpc = c->tuned_bpc, saved_ss = c->tuned_sort_steps, saved_ppl = c->tuned_ppl;
     c->tuned_bpc = 0;
     c->tuned_sort_steps = -1;
+    c->tuned_ppl = 0;
     double best = -1.0, dflt = -1.0;
-    int best_bpc = 1, best_ss = 0, ntrials = 0;
-    for (int trial = -1; trial < max_bpc * (int)budgets.size(); ++trial) {
-        // trial -1: the untuned defaults; then every (bpc, budget) pair
-        const int bpc = trial < 0 ? 0 : 1 + trial / (int)budgets.size();
-        const int ss = trial < 0 ? -1 : budgets[(size_t)(trial % (int)budgets.size())];
-        if (trial >= 0) {
+    int best_bpc = 1, best_ss = 0, best_ppl = 1, ntrials = 0;
+    for (int t = -1; t < (int)trials.size(); ++t) {
+        // trial -1: the untuned defaults; then every triple
+        const Trial tr = t < 0 ? Trial{0, 0, -1} : trials[(size_t)t];
+        if (t >= 0) {
             SP_HIP(hipMemcpyAsync(c->pbuf[cur0], c->pbuf_save, bytes, hipMemcpyDeviceToDevice, c->st));
             c->cur = cur0;
         }
         unsigned long long steps = 0;
         float ms = 0.f;
-        rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, bpc, ss, steps, ms);
+        rc = mover_run(c, t0, dtf, num_fine_steps, nsteps_interval, tr.bpc, tr.ss, tr.ppl, steps, ms);
         if (rc != SP_OK) break;
         const double rate = ms > 0.f ? (double)steps / (1e-3 * (double)ms) : 0.0;
-        if (trial < 0) {
+        if (t < 0) {
             dflt = rate;
             continue;
         }
         ++ntrials;
         if (rate > best) {
             best = rate;
-            best_bpc = bpc;
-            best_ss = ss;
+            best_bpc = tr.bpc;
+            best_ss = tr.ss;
+            best_ppl = tr.ppl;
         }
     }
     // restore the particles and counters whatever the trials did
@@ -1178,15 +1210,17 @@ int sp_autotune_mover(void* ctx, double t0, double dtf, int32_t num_fine_steps,
     if (rc != SP_OK) {
         c->tuned_bpc = saved_bpc;
         c->tuned_sort_steps = saved_ss;
+        c->tuned_ppl = saved_ppl;
         return rc;
     }
     c->tuned_bpc = best_bpc;
     c->tuned_sort_steps = c->cfg.sort_every > 0 ? best_ss : -1;
+    c->tuned_ppl = best_ppl;
     if (out) {
         out->blocks_per_cu = best_bpc;
         out->sort_steps = best_ss;
         out->ntrials = ntrials;
-        out->reserved0 = 0;
+        out->particles_per_lane = best_ppl;
         out->steps_per_s = best;
         out->default_steps_per_s = dflt;
     }

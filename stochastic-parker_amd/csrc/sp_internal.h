@@ -210,16 +210,18 @@ hipError_t launch_gradient_max(const DevParams& prm, const float* nodes8, unsign
                                hipStream_t s);
 hipError_t launch_pack_half(const DevParams& prm, const float* nodes8, float scale, float* rec,
                             float* farray32, hipStream_t s);
-int mover_blocks_per_cu_f32h(bool ti, int fx);
-hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int grid, hipStream_t s);
+// ppl: particles per lane of the FP32 movers (1 or 2); the FP64 mover takes one
+int mover_blocks_per_cu_f32h(bool ti, int fx, int ppl);
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ppl, int grid,
+                             hipStream_t s);
 hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* state,
                                  const double* rands, const int8_t* fixed, double* deltas,
                                  hipStream_t s);
 int mover_blocks_per_cu_f64(int ndim, bool ti, int fx);
-int mover_blocks_per_cu_f32(int ndim, bool ti, int fx);
+int mover_blocks_per_cu_f32(int ndim, bool ti, int fx, int ppl);
 hipError_t launch_mover_f64(const MoverParams& m, bool ti, int fx, int ndim, int grid,
                             hipStream_t s);
-hipError_t launch_mover_f32(const MoverParams& m, bool ti, int fx, int ndim, int grid,
+hipError_t launch_mover_f32(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
                             hipStream_t s);
 hipError_t launch_step_once_f64(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                 const double* rands, const int8_t* fixed, double* deltas,

@@ -5,51 +5,64 @@
 
 namespace sp {
 
-struct PolicyF32 {
-    static constexpr bool kOnly3D = false;
-    static constexpr bool kDpp = true;
+// FP32 gather and coefficients; one or two particles per lane, each with its
+// own copy of the per-particle diffusion parameters (KF).
+template <bool HALF>
+struct PolicyF32Base {
+    static constexpr int kMaxPPL = 2;
+    using Var = f32::KF;
+    __device__ __forceinline__ static Var var_of(const MoverParams& m) { return f32::kf_of(m); }
+    __device__ __forceinline__ static void set_var(MoverParams&, Var& v, const KappaVar& kv) {
+        v = f32::kf_of(kv);
+    }
     template <int NDIM, bool TI, int FX>
-    __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
-                                                uint32_t tagi, uint32_t tags, uint32_t rstep,
-                                                double d[4]) {
+    __device__ __forceinline__ static void step(const MoverParams& m, const Var& kf, State& s,
+                                                bool fixed, uint32_t tagi, uint32_t tags,
+                                                uint32_t rstep, double d[4]) {
         float r[4];
         f32::step_rands(m, tagi, tags, rstep, r);
-        f32::gather_kappa_push<NDIM, TI, FX>(m, s, fixed, r, d);
+        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, r, d);
+    }
+    // PPL particles: the Philox blocks, the gathers and the steps of all
+    // slots in one block, for the scheduler to interleave
+    template <int NDIM, bool TI, int FX, int PPL>
+    __device__ __forceinline__ static void step_n(const MoverParams& m, const Var* kf, State* s,
+                                                  const bool* fixed, const Slot* sl,
+                                                  double (*d)[4]) {
+        float r[PPL][4];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) f32::step_rands(m, sl[k].tagi, sl[k].tags, sl[k].rstep, r[k]);
+#pragma unroll
+        for (int k = 0; k < PPL; ++k)
+            f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf[k], s[k], fixed[k], r[k], d[k]);
     }
     template <int NDIM, bool TI, int FX>
-    __device__ __forceinline__ static void step_given(const MoverParams& m, State& s, bool fixed,
-                                                      const double* r, double d[4]) {
+    __device__ __forceinline__ static void step_given(const MoverParams& m, const Var& kf,
+                                                      State& s, bool fixed, const double* r,
+                                                      double d[4]) {
         const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
-        f32::gather_kappa_push<NDIM, TI, FX>(m, s, fixed, rf, d);
+        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, rf, d);
     }
+};
+
+struct PolicyF32 : PolicyF32Base<false> {
+    static constexpr bool kOnly3D = false;
+    static constexpr bool kDpp = true;
 };
 
 // FP32 arithmetic on 3D records with FP16 gradients.
-struct PolicyF32H {
+struct PolicyF32H : PolicyF32Base<true> {
     static constexpr bool kOnly3D = true;
     static constexpr bool kDpp = false;   // no momentum-diffusion slots in these records
-    template <int NDIM, bool TI, int FX>
-    __device__ __forceinline__ static void step(const MoverParams& m, State& s, bool fixed,
-                                                uint32_t tagi, uint32_t tags, uint32_t rstep,
-                                                double d[4]) {
-        float r[4];
-        f32::step_rands(m, tagi, tags, rstep, r);
-        f32::gather_kappa_push<NDIM, TI, FX, true>(m, s, fixed, r, d);
-    }
-    template <int NDIM, bool TI, int FX>
-    __device__ __forceinline__ static void step_given(const MoverParams& m, State& s, bool fixed,
-                                                      const double* r, double d[4]) {
-        const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
-        f32::gather_kappa_push<NDIM, TI, FX, true>(m, s, fixed, rf, d);
-    }
 };
 
-int mover_blocks_per_cu_f32h(bool ti, int fx) {
-    return mover_occupancy<PolicyF32H>(3, ti, fx);
+int mover_blocks_per_cu_f32h(bool ti, int fx, int ppl) {
+    return mover_occupancy<PolicyF32H>(3, ti, fx, ppl);
 }
 
-hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int grid, hipStream_t s) {
-    return launch_mover<PolicyF32H>(m, ti, fx, 3, grid, s);
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ppl, int grid,
+                             hipStream_t s) {
+    return launch_mover<PolicyF32H>(m, ti, fx, 3, ppl, grid, s);
 }
 
 hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* state,
@@ -58,13 +71,13 @@ hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* 
     return launch_step_once<PolicyF32H>(m, ti, fx, 3, state, rands, fixed, deltas, s);
 }
 
-int mover_blocks_per_cu_f32(int ndim, bool ti, int fx) {
-    return mover_occupancy<PolicyF32>(ndim, ti, fx);
+int mover_blocks_per_cu_f32(int ndim, bool ti, int fx, int ppl) {
+    return mover_occupancy<PolicyF32>(ndim, ti, fx, ppl);
 }
 
-hipError_t launch_mover_f32(const MoverParams& m, bool ti, int fx, int ndim, int grid,
+hipError_t launch_mover_f32(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
                             hipStream_t s) {
-    return launch_mover<PolicyF32>(m, ti, fx, ndim, grid, s);
+    return launch_mover<PolicyF32>(m, ti, fx, ndim, ppl, grid, s);
 }
 
 hipError_t launch_step_once_f32(const MoverParams& m, bool ti, int fx, int ndim, double* state,

@@ -31,6 +31,8 @@
 
 #include "sp_device.h"
 
+#include <type_traits>
+
 namespace sp {
 
 enum : int { PEND_NONE = 0, PEND_PRE = 1, PEND_REDO = 2 };
@@ -68,17 +70,44 @@ __device__ __forceinline__ void pin_constants(MoverParams& m) {
     pin_vgpr(m.lx); pin_vgpr(m.ly); pin_vgpr(m.lz);
 }
 
+// Per-slot state of the particle a lane is pushing.
+struct Slot {
+    long long idx;                // particle index (-1: none)
+    State s;
+    double w, dtarget, dt_old;
+    double dl[4];                 // the last step's increments (for the undo)
+    int cf, nsp, pending;
+    uint32_t tagi, tags, rstep;
+    unsigned ksteps;              // steps of this particle in this launch
+    bool resumed;
+};
+
 // At most two resident waves per SIMD: the mover runs one or two 256-lane
 // blocks per CU (fewer particles in flight keep their field records in L2),
 // so the compiler may spend registers on keeping the gather's loads in flight
 // rather than on a higher occupancy that is never used (3D: +13 %).
-template <class Pol, int NDIM, bool TINTERP, int FX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+//
+// PPL particles per lane: each lane carries PPL independent particles
+// ("slots") through the loop above.  Their steps are computed in one
+// straight-line block (every slot steps, a slot that has no step to take
+// discards its result), so the compiler can interleave two particles' loads,
+// Philox rounds and arithmetic and fill the latency one particle alone leaves
+// idle.  Every slot draws its increments from its own particle's counter, so
+// the particles per lane never change a result.  Two 2D particles per lane
+// fit the two-wave register budget (233 VGPRs) once the constants are left to
+// the compiler; two 3D particles need the one-wave budget.
+template <int NDIM, int PPL>
+constexpr bool kTwoWavePair = NDIM == 2 && PPL == 2;
+
+template <class Pol, int NDIM, bool TINTERP, int FX, int PPL>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(kTwoWavePair<NDIM, PPL> ? 2 : 1, 2)))
 mover_kernel(MoverParams m0) {
+    static_assert(PPL >= 1 && PPL <= Pol::kMaxPPL, "particles per lane of this policy");
     MoverParams m = m0;
     // within the two-wave register budget the constants fit in VGPRs for 2D
     // and 3D alike (2D: +5 %, 3D: +1 %)
-    pin_constants(m);
+    if constexpr (!kTwoWavePair<NDIM, PPL>) pin_constants(m);
     const int lane = threadIdx.x & (kWave - 1);
     double* const px = m.P.d(PD_X);
     double* const py = m.P.d(PD_Y);
@@ -91,24 +120,34 @@ mover_kernel(MoverParams m0) {
     uint32_t* const prstep = reinterpret_cast<uint32_t*>(m.P.i(PI_RSTEP));
     double* const prdt = m.P.d(PD_RDT);
 
-    long long idx = -1;
     bool exhausted = false;
     // start on this die's part of the range, then help the other parts
     int part = (int)(xcc_id() % (unsigned)kParts), parts_done = 0;
-    State s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double w = 0.0, dtarget = 0.0, dt_old = 0.0;
-    double dl[4] = {0.0, 0.0, 0.0, 0.0};
-    int cf = 0, nsp = 0, pending = PEND_NONE;
-    uint32_t tagi = 0, tags = 0, rstep = 0;
-    unsigned ksteps = 0;          // steps of the current particle in this launch
-    bool resumed = false;
+    Slot sl[PPL];
+    typename Pol::Var var[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        sl[k].idx = -1;
+        sl[k].s = State{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        sl[k].w = sl[k].dtarget = sl[k].dt_old = 0.0;
+        sl[k].dl[0] = sl[k].dl[1] = sl[k].dl[2] = sl[k].dl[3] = 0.0;
+        sl[k].cf = sl[k].nsp = 0;
+        sl[k].pending = PEND_NONE;
+        sl[k].tagi = sl[k].tags = sl[k].rstep = 0u;
+        sl[k].ksteps = 0u;
+        sl[k].resumed = false;
+        var[k] = Pol::var_of(m);
+    }
     unsigned long long my_steps = 0, my_paused = 0;
 
     while (true) {
-        // ---- refill idle lanes (wave-aggregated work fetch) ----
-        const bool need = idx < 0 && !exhausted;
-        const unsigned long long need_mask = __ballot(need);
-        if (need_mask) {
+        // ---- refill idle slots (wave-aggregated work fetch, one slot index at a time) ----
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            Slot& q = sl[k];
+            const bool need = q.idx < 0 && !exhausted;
+            const unsigned long long need_mask = __ballot(need);
+            if (!need_mask) continue;
             const int leader = __ffsll((unsigned long long)need_mask) - 1;
             const unsigned long long cnt = (unsigned long long)__popcll(need_mask);
             unsigned long long base = 0;
@@ -125,157 +164,189 @@ mover_kernel(MoverParams m0) {
                 const long long j =
                     pb + (long long)(base + (unsigned long long)__popcll(need_mask & ((1ull << lane) - 1ull)));
                 if (j < pe) {
-                    idx = j;
-                    s.x = px[idx];
-                    s.y = py[idx];
-                    s.z = pz[idx];
-                    s.p = pp[idx];
-                    s.t = pt[idx];
-                    s.dt = pdt[idx];
-                    if constexpr ((FX & FX_NLGC) != 0) s.mu = m.P.d(PD_MU)[idx];
-                    w = m.P.d(PD_W)[idx];
-                    cf = pflag[idx];
-                    nsp = ppushed[idx];
+                    q.idx = j;
+                    q.s.x = px[j];
+                    q.s.y = py[j];
+                    q.s.z = pz[j];
+                    q.s.p = pp[j];
+                    q.s.t = pt[j];
+                    q.s.dt = pdt[j];
+                    if constexpr ((FX & FX_NLGC) != 0) q.s.mu = m.P.d(PD_MU)[j];
+                    q.w = m.P.d(PD_W)[j];
+                    q.cf = pflag[j];
+                    q.nsp = ppushed[j];
                     if (m0.nvar > 0) {
                         // this particle's variant: its diffusion parameters
-                        int v = m.P.i(PI_ORIGIN)[idx];
+                        int v = m.P.i(PI_ORIGIN)[j];
                         v = v < 0 ? 0 : (v >= m0.nvar ? m0.nvar - 1 : v);
-                        const KappaVar kv = m0.var[v];
-                        m.kpara0 = kv.kpara0; m.kret = kv.kret;
-                        m.pindex = kv.pindex; m.gamma_m2 = kv.gamma_m2;
-                        m.p0_2mpi = kv.p0_2mpi;
-                        m.perp_mag = kv.perp_mag; m.perp_mom = kv.perp_mom;
-                        m.f_perp_mag = kv.f_perp_mag; m.f_perp_mom = kv.f_perp_mom;
-                        m.f_kpara0 = kv.f_kpara0; m.f_kret = kv.f_kret;
-                        m.f_pindex = kv.f_pindex; m.f_gm2 = kv.f_gm2;
+                        Pol::set_var(m, var[k], m0.var[v]);
                     }
-                    tagi = (uint32_t)m.P.i(PI_TAGI)[idx];
-                    tags = (uint32_t)m.P.i(PI_TAGS)[idx];
-                    rstep = prstep[idx];
-                    dl[0] = dl[1] = dl[2] = dl[3] = 0.0;
-                    ksteps = 0;
-                    const double rd = prdt[idx];
-                    resumed = rd > 0.0;
-                    if (resumed) {
+                    q.tagi = (uint32_t)m.P.i(PI_TAGI)[j];
+                    q.tags = (uint32_t)m.P.i(PI_TAGS)[j];
+                    q.rstep = prstep[j];
+                    q.dl[0] = q.dl[1] = q.dl[2] = q.dl[3] = 0.0;
+                    q.ksteps = 0;
+                    const double rd = prdt[j];
+                    q.resumed = rd > 0.0;
+                    if (q.resumed) {
                         // paused at the top of the inner loop by an earlier launch
-                        dtarget = rd;
-                        pending = PEND_NONE;
+                        q.dtarget = rd;
+                        q.pending = PEND_NONE;
                     } else {
                         // target time from the particle's own time stamp (:1570-1578)
-                        const double stp = ceil((s.t - m.t0) / m.dt_fine);
-                        if (stp <= 0.0) dtarget = m.dt_fine; else dtarget = stp * m.dt_fine;
-                        if (dtarget > m.dtf) dtarget = m.dtf;
-                        pending = PEND_PRE;
+                        const double stp = ceil((q.s.t - m.t0) / m.dt_fine);
+                        if (stp <= 0.0) q.dtarget = m.dt_fine; else q.dtarget = stp * m.dt_fine;
+                        if (q.dtarget > m.dtf) q.dtarget = m.dtf;
+                        q.pending = PEND_PRE;
                     }
                 }
             }
         }
-        if (__ballot(idx >= 0 || !exhausted) == 0ull) break;
+        bool busy = false;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) busy = busy || sl[k].idx >= 0;
+        if (__ballot(busy || !exhausted) == 0ull) break;
 
-        // ---- choose this lane's action ----
-        int act = ACT_NONE;
-        bool need_bc = false;
-        if (idx >= 0) {
-            if (pending == PEND_PRE) {
+        bool do_push[PPL], fixed[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+            Slot& q = sl[k];
+            do_push[k] = false;
+            fixed[k] = false;
+            if (q.idx < 0) continue;
+            // ---- choose this slot's action ----
+            int act = ACT_NONE;
+            bool need_bc = false;
+            if (q.pending == PEND_PRE) {
                 need_bc = true;  // safety check before the loop (:1581-1592)
                 act = ACT_PRE;
-            } else if (pending == PEND_REDO) {
-                s.dt = dt_old;   // the redo step is done; restore dt, check, advance (:1804-1826)
+            } else if (q.pending == PEND_REDO) {
+                q.s.dt = q.dt_old;  // the redo step is done; restore dt, check, advance (:1804-1826)
                 need_bc = true;
                 act = ACT_ADVANCE;
-            } else if ((s.t - m.t0) < dtarget && cf == SP_COUNT_FLAG_INBOX) {
-                if (m.step_budget != 0u && ksteps >= m.step_budget) {
+            } else if ((q.s.t - m.t0) < q.dtarget && q.cf == SP_COUNT_FLAG_INBOX) {
+                if (m.step_budget != 0u && q.ksteps >= m.step_budget) {
                     act = ACT_PAUSE;
                 } else {
                     need_bc = true;  // top of the inner loop (:1600-1612)
                     act = ACT_PUSH;
                 }
-            } else if ((s.t - m.t0) > dtarget && cf == SP_COUNT_FLAG_INBOX) {
+            } else if ((q.s.t - m.t0) > q.dtarget && q.cf == SP_COUNT_FLAG_INBOX) {
                 // land exactly on the target time (:1707-1720)
-                s.x = s.x - dl[0];
-                s.y = s.y - dl[1];
-                s.z = s.z - dl[2];
-                s.p = s.p - dl[3];
-                s.t = s.t - s.dt;
-                dt_old = s.dt;
-                s.dt = m.t0 + dtarget - s.t;
-                if (s.dt > 0) {
-                    nsp = nsp - 1;
+                q.s.x = q.s.x - q.dl[0];
+                q.s.y = q.s.y - q.dl[1];
+                q.s.z = q.s.z - q.dl[2];
+                q.s.p = q.s.p - q.dl[3];
+                q.s.t = q.s.t - q.s.dt;
+                q.dt_old = q.s.dt;
+                q.s.dt = m.t0 + q.dtarget - q.s.t;
+                if (q.s.dt > 0) {
+                    q.nsp = q.nsp - 1;
                     act = ACT_PUSH_FIXED;
                 } else {
-                    s.dt = dt_old;
+                    q.s.dt = q.dt_old;
                     need_bc = true;
                     act = ACT_ADVANCE;
                 }
             } else {
                 act = ACT_ADVANCE;
             }
-            pending = PEND_NONE;
+            q.pending = PEND_NONE;
+
+            // ---- the slack-bound check: momentum sign first, then the faces ----
+            if (need_bc) {
+                if (q.s.p < 0.0 && q.cf == SP_COUNT_FLAG_INBOX) {
+                    q.cf = SP_COUNT_FLAG_OTHERS;
+                    atomicAdd(&m.ctr->negp_fx, weight_fx(q.w));
+                } else {
+                    const double lk = boundary_condition<NDIM>(m.flags, q.s.x, q.s.y, q.s.z, q.cf,
+                                                               q.w, m.slo[0], m.slo[1], m.slo[2],
+                                                               m.shi[0], m.shi[1], m.shi[2]);
+                    if (lk != 0.0) atomicAdd(&m.ctr->leak_fx, weight_fx(lk));
+                }
+            }
+
+            // ---- transitions ----
+            bool fin = false;
+            if (act == ACT_PUSH) {
+                if (q.cf == SP_COUNT_FLAG_INBOX) do_push[k] = true;
+                else fin = true;  // the inner loop exits and the outer loop ends on count_flag
+            } else if (act == ACT_PUSH_FIXED) {
+                do_push[k] = true;
+                fixed[k] = true;
+            } else if (act == ACT_ADVANCE) {
+                q.dtarget = q.dtarget + m.dt_fine;
+                fin = !(q.dtarget < m.dt_limit) || q.cf != SP_COUNT_FLAG_INBOX;
+            } else if (act == ACT_PRE) {
+                fin = q.cf != SP_COUNT_FLAG_INBOX || !(q.dtarget < m.dt_limit);
+            }
+            const bool pause = act == ACT_PAUSE;
+            if (fin || pause) {
+                if (fin && q.cf == SP_COUNT_FLAG_INBOX) {
+                    const double lk = boundary_condition<NDIM>(m.flags, q.s.x, q.s.y, q.s.z, q.cf,
+                                                               q.w, m.elo[0], m.elo[1], m.elo[2],
+                                                               m.ehi[0], m.ehi[1], m.ehi[2]);
+                    if (lk != 0.0) atomicAdd(&m.ctr->leak_fx, weight_fx(lk));
+                }
+                const long long j = q.idx;
+                if (pause) {
+                    prdt[j] = q.dtarget;
+                    ++my_paused;
+                } else if (q.resumed) {
+                    prdt[j] = 0.0;
+                }
+                px[j] = q.s.x;
+                py[j] = q.s.y;
+                pz[j] = q.s.z;
+                pp[j] = q.s.p;
+                pt[j] = q.s.t;
+                pdt[j] = q.s.dt;
+                pflag[j] = (int8_t)q.cf;
+                ppushed[j] = q.nsp;
+                prstep[j] = q.rstep;
+                q.idx = -1;
+            }
         }
 
-        // ---- the slack-bound check: momentum sign first, then the faces ----
-        if (need_bc) {
-            if (s.p < 0.0 && cf == SP_COUNT_FLAG_INBOX) {
-                cf = SP_COUNT_FLAG_OTHERS;
-                atomicAdd(&m.ctr->negp_fx, weight_fx(w));
+        // ---- one particle-step for every slot that takes one ----
+        bool any_push = false;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) any_push = any_push || do_push[k];
+        if (any_push) {
+            if constexpr (PPL == 1) {
+                Pol::template step<NDIM, TINTERP, FX>(m, var[0], sl[0].s, fixed[0], sl[0].tagi,
+                                                      sl[0].tags, sl[0].rstep, sl[0].dl);
             } else {
-                const double lk = boundary_condition<NDIM>(m.flags, s.x, s.y, s.z, cf, w, m.slo[0],
-                                                           m.slo[1], m.slo[2], m.shi[0], m.shi[1],
-                                                           m.shi[2]);
-                if (lk != 0.0) atomicAdd(&m.ctr->leak_fx, weight_fx(lk));
+                // every slot steps (one straight-line block); a slot without a
+                // step to take keeps its state
+                State ns[PPL];
+                double nd[PPL][4];
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) ns[k] = sl[k].s;
+                Pol::template step_n<NDIM, TINTERP, FX, PPL>(m, var, ns, fixed, sl, nd);
+#pragma unroll
+                for (int k = 0; k < PPL; ++k) {
+                    if (do_push[k]) {
+                        sl[k].s = ns[k];
+                        sl[k].dl[0] = nd[k][0];
+                        sl[k].dl[1] = nd[k][1];
+                        sl[k].dl[2] = nd[k][2];
+                        sl[k].dl[3] = nd[k][3];
+                    }
+                }
             }
-        }
-
-        // ---- transitions ----
-        bool fin = false, do_push = false;
-        if (act == ACT_PUSH) {
-            if (cf == SP_COUNT_FLAG_INBOX) do_push = true;
-            else fin = true;  // the inner loop exits and the outer loop ends on count_flag
-        } else if (act == ACT_PUSH_FIXED) {
-            do_push = true;
-        } else if (act == ACT_ADVANCE) {
-            dtarget = dtarget + m.dt_fine;
-            fin = !(dtarget < m.dt_limit) || cf != SP_COUNT_FLAG_INBOX;
-        } else if (act == ACT_PRE) {
-            fin = cf != SP_COUNT_FLAG_INBOX || !(dtarget < m.dt_limit);
-        }
-        const bool pause = act == ACT_PAUSE;
-        if (fin || pause) {
-            if (fin && cf == SP_COUNT_FLAG_INBOX) {
-                const double lk = boundary_condition<NDIM>(m.flags, s.x, s.y, s.z, cf, w, m.elo[0],
-                                                           m.elo[1], m.elo[2], m.ehi[0], m.ehi[1],
-                                                           m.ehi[2]);
-                if (lk != 0.0) atomicAdd(&m.ctr->leak_fx, weight_fx(lk));
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (!do_push[k]) continue;
+                Slot& q = sl[k];
+                q.rstep += 1u;
+                int v = q.nsp + 1;  // mod(nsteps_pushed + 1, nsteps_interval)
+                if ((unsigned)v >= (unsigned)m.nsteps_interval) v %= m.nsteps_interval;
+                q.nsp = v;
+                ++my_steps;
+                ++q.ksteps;
+                if (fixed[k]) q.pending = PEND_REDO;
             }
-            if (pause) {
-                prdt[idx] = dtarget;
-                ++my_paused;
-            } else if (resumed) {
-                prdt[idx] = 0.0;
-            }
-            px[idx] = s.x;
-            py[idx] = s.y;
-            pz[idx] = s.z;
-            pp[idx] = s.p;
-            pt[idx] = s.t;
-            pdt[idx] = s.dt;
-            pflag[idx] = (int8_t)cf;
-            ppushed[idx] = nsp;
-            prstep[idx] = rstep;
-            idx = -1;
-        }
-
-        // ---- one particle-step for every lane that takes one ----
-        if (do_push) {
-            const bool fixed = act == ACT_PUSH_FIXED;
-            Pol::template step<NDIM, TINTERP, FX>(m, s, fixed, tagi, tags, rstep, dl);
-            rstep += 1u;
-            int v = nsp + 1;  // mod(nsteps_pushed + 1, nsteps_interval)
-            if ((unsigned)v >= (unsigned)m.nsteps_interval) v %= m.nsteps_interval;
-            nsp = v;
-            ++my_steps;
-            ++ksteps;
-            if (fixed) pending = PEND_REDO;
         }
     }
 
@@ -300,7 +371,7 @@ __global__ void __launch_bounds__(256) step_once_kernel(MoverParams m, double* s
             state[6 * i + 3], state[6 * i + 4], state[6 * i + 5], 0.0};
     if constexpr ((FX & FX_NLGC) != 0) s.mu = m.P.d(PD_MU)[i];
     double d[4] = {0.0, 0.0, 0.0, 0.0};
-    Pol::template step_given<NDIM, TINTERP, FX>(m, s, fixed[i] != 0, rands + 4 * i, d);
+    Pol::template step_given<NDIM, TINTERP, FX>(m, Pol::var_of(m), s, fixed[i] != 0, rands + 4 * i, d);
     state[6 * i + 0] = s.x;
     state[6 * i + 1] = s.y;
     state[6 * i + 2] = s.z;
@@ -347,24 +418,40 @@ auto dispatch_variant(int ndim, bool ti, int fx, F&& f) {
     return dispatch_fx<DPP, 3>(ti, fx, f);
 }
 
+// Calls f(v, std::integral_constant<int, PPL>{}) for the runtime particles
+// per lane: 1, or 2 where the policy supports it and the step gathers one
+// frame (two frames per particle and two particles spill registers).
+template <class Pol, class V, class F>
+auto dispatch_ppl(int ppl, V v, F&& f) {
+    if constexpr (Pol::kMaxPPL >= 2 && !V::ti) {
+        if (ppl == 2) return f(v, std::integral_constant<int, 2>{});
+    }
+    return f(v, std::integral_constant<int, 1>{});
+}
+
 template <class Pol>
-int mover_occupancy(int ndim, bool ti, int fx) {
-    return dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [](auto v) {
-        using V = decltype(v);
-        int nb = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, mover_kernel<Pol, V::ndim, V::ti, V::fx>, 256, 0);
-        return e == hipSuccess ? nb : 1;
+int mover_occupancy(int ndim, bool ti, int fx, int ppl) {
+    return dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
+        return dispatch_ppl<Pol>(ppl, v, [](auto vv, auto pc) {
+            using V = decltype(vv);
+            int nb = 0;
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, mover_kernel<Pol, V::ndim, V::ti, V::fx, decltype(pc)::value>, 256, 0);
+            return e == hipSuccess ? nb : 1;
+        });
     });
 }
 
 template <class Pol>
-hipError_t launch_mover(const MoverParams& m, bool ti, int fx, int ndim, int grid, hipStream_t st) {
+hipError_t launch_mover(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
+                        hipStream_t st) {
     dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
-        using V = decltype(v);
-        hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::fx>), dim3(grid), dim3(256), 0,
-                           st, m);
-        return 0;
+        return dispatch_ppl<Pol>(ppl, v, [&](auto vv, auto pc) {
+            using V = decltype(vv);
+            hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::fx, decltype(pc)::value>),
+                               dim3(grid), dim3(256), 0, st, m);
+            return 0;
+        });
     });
     return hipGetLastError();
 }

@@ -29,6 +29,20 @@ struct Interp {
     float w[8];
 };
 
+// The diffusion parameters a batched run varies per particle (KappaVar's
+// FP32 forms): one copy per particle a lane carries.
+struct KF {
+    float kpara0, kret, pindex, gm2, perp_mag, perp_mom;
+};
+
+__device__ __forceinline__ KF kf_of(const MoverParams& m) {
+    return KF{m.f_kpara0, m.f_kret, m.f_pindex, m.f_gm2, m.f_perp_mag, m.f_perp_mom};
+}
+
+__device__ __forceinline__ KF kf_of(const KappaVar& v) {
+    return KF{v.f_kpara0, v.f_kret, v.f_pindex, v.f_gm2, v.f_perp_mag, v.f_perp_mom};
+}
+
 template <int NDIM>
 __device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, double y, double z) {
     Interp I;
@@ -173,7 +187,7 @@ __device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& 
 // Momentum diffusion in FP32 (f64::dpp_terms, particle_module.f90:2918-2979):
 // va^2 = B^2 / rho; p^(pindex-1) p0^(2-pindex) = p0^2 (p/p0)^pindex / p.
 template <int NDIM>
-__device__ __forceinline__ void dpp_terms(const MoverParams& m, float pf, const float* f,
+__device__ __forceinline__ void dpp_terms(const MoverParams& m, const KF& kf, float pf, const float* f,
                                           float kpara, float knorm_para, float b2, float bx,
                                           float by, float bz, float ib, float divv, float& dp_dt,
                                           float& dpp) {
@@ -211,9 +225,9 @@ __device__ __forceinline__ void dpp_terms(const MoverParams& m, float pf, const 
                                        2.0f * (sxy * sxy + sxz * sxz + syz * syz));
         }
         if (gshear > 0.0f) {
-            const float q = fpow(pf * (float)m.inv_p0, m.f_pindex);
+            const float q = fpow(pf * (float)m.inv_p0, kf.pindex);
             const float dd = gshear * m.f_tau0 * knorm_para * m.f_p02 * q;
-            dp_dt += (2.0f + m.f_pindex) * dd * frcp(pf);
+            dp_dt += (2.0f + kf.pindex) * dd * frcp(pf);
             dpp += dd;
         }
     }
@@ -221,10 +235,14 @@ __device__ __forceinline__ void dpp_terms(const MoverParams& m, float pf, const 
 
 // One Parker step in FP32 coefficients.  Semantics match f64::push: same dt
 // rule (including the dt_min fallback when a rate is exactly zero), same
-// increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.
+// increments, same momentum floor; d = (dx, dy, dz, dp) as doubles.  The
+// configuration's flags and the per-particle choices (the dt rule's branches,
+// the acceleration gate, the floor) are selects rather than branches, so
+// the step of several particles per lane is one straight-line block.
 template <int NDIM, int FX>
-__device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const float* f,
-                                           bool fixed_dt, const float r[4], double d[4]) {
+__device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, State& s,
+                                           const float* f, bool fixed_dt, const float r[4],
+                                           double d[4]) {
     constexpr bool ACC = (FX & FX_ACC) != 0;
     constexpr bool DPP = (FX & FX_DPP) != 0;
     constexpr bool NLGC = (FX & FX_NLGC) != 0;
@@ -240,26 +258,28 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     const float pf = (float)s.p;
     const bool mag = (m.flags & MF_MAG_DEP) != 0;
 
+    const bool momdep = (m.flags & MF_MOMENTUM_DEP) != 0;
+    const float pnorm = (float)(s.p * m.inv_p0);
+
     // --- kappa ---------------------------------------------------------
-    float knorm = 1.0f;
-    if (mag) knorm = fpow(b, m.f_gm2);
+    float knorm = mag ? fpow(b, kf.gm2) : 1.0f;
     const float knorm_para = knorm;
-    if (m.flags & MF_MOMENTUM_DEP) knorm *= fpow((float)(s.p * m.inv_p0), m.f_pindex);
-    const float kpara = m.f_kpara0 * knorm;
+    knorm = momdep ? knorm * fpow(pnorm, kf.pindex) : knorm;
+    const float kpara = kf.kpara0 * knorm;
     float kperp, kpp;
     if constexpr (NLGC) {
         // kpara0 * kperp_kpara * B^((gamma-2)/3) * (p/p0)^((5-gamma)/3) * mu^2
-        float kn_perp = mag ? fpow(b, m.f_perp_mag) : 1.0f;
-        if (m.flags & MF_MOMENTUM_DEP) kn_perp *= fpow((float)(s.p * m.inv_p0), m.f_perp_mom);
+        float kn_perp = mag ? fpow(b, kf.perp_mag) : 1.0f;
+        kn_perp = momdep ? kn_perp * fpow(pnorm, kf.perp_mom) : kn_perp;
         const float muf = (float)s.mu;
-        kperp = m.f_kpara0 * m.f_kperp_kpara * kn_perp * (muf * muf);
+        kperp = kf.kpara0 * m.f_kperp_kpara * kn_perp * (muf * muf);
         kpp = kpara - kperp;
     } else {
-        kperp = kpara * m.f_kret;
+        kperp = kpara * kf.kret;
         // kpara - kperp as a product of non-negative factors: a contracted
         // kpara - kpara * kret can come out as a negative rounding error when
         // kret = 1, and its square root as NaN
-        kpp = kpara * (1.0f - m.f_kret);
+        kpp = kpara * (1.0f - kf.kret);
     }
     const float skpara = fsqrt(2.0f * kpara);
     const float skperp = fsqrt(2.0f * kperp);
@@ -279,8 +299,8 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         const float dby_dx = f[R2_DBYDX], dby_dy = f[R2_DBYDY];
         const float dbz_dx = f[R2_DBZDX], dbz_dy = f[R2_DBZDY];
         const float db_dx = f[R2_DBDX], db_dy = f[R2_DBDY];
-        float dkdx = 0.f, dkdy = 0.f;
-        if (mag) { dkdx = db_dx * ibk * m.f_gm2; dkdy = db_dy * ibk * m.f_gm2; }
+        const float dkdx = mag ? db_dx * ibk * kf.gm2 : 0.f;
+        const float dkdy = mag ? db_dy * ibk * kf.gm2 : 0.f;
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
         if constexpr (NLGC) {
             // d ln kperp = d ln kpara / 3 along B's magnitude
@@ -299,9 +319,8 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         const float ibp2 = ib * ib, ibp3 = ib * ibp2;
         const float vdx = vdp * (dbz_dy * ibp2 - 2.0f * bz * db_dy * ibp3);
         const float vdy = vdp * (-(dbz_dx * ibp2) + 2.0f * bz * db_dx * ibp3);
-        float vdz = 0.f;
-        if (m.flags & MF_DRIFT_2D)
-            vdz = vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3);
+        const float vdz = (m.flags & MF_DRIFT_2D)
+            ? vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3) : 0.f;
         dx_dt = f[R2_VX] + vdx + dkxx_dx + dkxy_dy;
         dy_dt = f[R2_VY] + vdy + dkxy_dx + dkyy_dy;
         dz_dt = vdz;
@@ -314,8 +333,9 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
         const float kb2 = kpp * ib2, kb3 = 2.0f * kpp * ib3;
         if constexpr (NLGC) {
             // NLGC keeps 1/B in the 3D logarithmic gradients (:2708-2715)
-            float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
-            if (mag) { dkdx = db_dx * ibk * m.f_gm2; dkdy = db_dy * ibk * m.f_gm2; dkdz = db_dz * ibk * m.f_gm2; }
+            const float dkdx = mag ? db_dx * ibk * kf.gm2 : 0.f;
+            const float dkdy = mag ? db_dy * ibk * kf.gm2 : 0.f;
+            const float dkdz = mag ? db_dz * ibk * kf.gm2 : 0.f;
             const float ex = dkdx * (1.0f / 3.0f), ey = dkdy * (1.0f / 3.0f), ez = dkdz * (1.0f / 3.0f);
             const float ax = (kpara * dkdx - kperp * ex) * ib2;
             const float ay = (kpara * dkdy - kperp * ey) * ib2;
@@ -330,8 +350,9 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
             dkyz_dy = ay * by * bz + kb2 * (dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
             dkyz_dz = az * by * bz + kb2 * (dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
         } else {
-            float dkdx = 0.f, dkdy = 0.f, dkdz = 0.f;
-            if (mag) { dkdx = db_dx * m.f_gm2; dkdy = db_dy * m.f_gm2; dkdz = db_dz * m.f_gm2; }
+            const float dkdx = mag ? db_dx * kf.gm2 : 0.f;
+            const float dkdy = mag ? db_dy * kf.gm2 : 0.f;
+            const float dkdz = mag ? db_dz * kf.gm2 : 0.f;
             dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
             dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
             dkzz_dz = kperp * dkdz + kb2 * dkdz * bz * bz + kb3 * bz * (dbz_dz * b - bz * db_dz);
@@ -357,31 +378,25 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
     }
     float dp_dt = -pf * divv * (1.0f / 3.0f);
     float dpp = 0.0f;
-    if constexpr (DPP) dpp_terms<NDIM>(m, pf, f, kpara, knorm_para, b2, bx, by, bz, ib, divv, dp_dt, dpp);
+    if constexpr (DPP) dpp_terms<NDIM>(m, kf, pf, f, kpara, knorm_para, b2, bx, by, bz, ib, divv, dp_dt, dpp);
 
-    float dtf32;
-    if (!fixed_dt) {
-        float mm;
-        if (dx_dt != 0.0f && dy_dt != 0.0f && dp_dt != 0.0f) {
-            const float sk = skperp > 0.0f ? skperp : skpara;
-            const float isk = frcp(skpara);
-            float a = m.f_half_dx * isk;
-            mm = a * a;
-            a = m.f_half_dy * isk; mm = fminf(mm, a * a);
-            if (NDIM == 3) { a = m.f_half_dz * isk; mm = fminf(mm, a * a); }
-            a = sk * frcp(dx_dt); mm = fminf(mm, a * a);
-            a = sk * frcp(dy_dt); mm = fminf(mm, a * a);
-            if (NDIM == 3) { a = sk * frcp(dz_dt); mm = fminf(mm, a * a); }
-            mm = fminf(mm, 0.1f * pf * frcp(fabsf(dp_dt)));
-        } else {
-            mm = m.f_dt_min;
-        }
-        mm = fminf(fmaxf(mm, m.f_dt_min), m.f_dt_max);
-        dtf32 = mm;
-        s.dt = (double)mm;
-    } else {
-        dtf32 = (float)s.dt;
-    }
+    // the adaptive dt (the dt_min fallback when a rate is exactly zero), or
+    // the fixed dt of a redo step
+    const float sk = skperp > 0.0f ? skperp : skpara;
+    const float isk = frcp(skpara);
+    float a = m.f_half_dx * isk;
+    float mm = a * a;
+    a = m.f_half_dy * isk; mm = fminf(mm, a * a);
+    if (NDIM == 3) { a = m.f_half_dz * isk; mm = fminf(mm, a * a); }
+    a = sk * frcp(dx_dt); mm = fminf(mm, a * a);
+    a = sk * frcp(dy_dt); mm = fminf(mm, a * a);
+    if (NDIM == 3) { a = sk * frcp(dz_dt); mm = fminf(mm, a * a); }
+    mm = fminf(mm, 0.1f * pf * frcp(fabsf(dp_dt)));
+    const bool rates = dx_dt != 0.0f && dy_dt != 0.0f && dp_dt != 0.0f;
+    mm = rates ? mm : m.f_dt_min;
+    mm = fminf(fmaxf(mm, m.f_dt_min), m.f_dt_max);
+    s.dt = fixed_dt ? s.dt : (double)mm;
+    const float dtf32 = fixed_dt ? (float)s.dt : mm;
     const float sdt = fsqrt(dtf32);
     float deltax, deltay, deltaz;
     if (NDIM == 2) {
@@ -409,15 +424,16 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, State& s, const
             const double zn = (s.z - m.zmin) / m.lz;
             in = in && (zn >= m.acc_region[4]) && (zn <= m.acc_region[5]);
         }
-        if (in) s.p += dp; else dp = 0.0;
+        s.p = in ? s.p + dp : s.p;
+        dp = in ? dp : 0.0;
     } else {
         s.p += dp;
     }
-    if (s.p < m.p_floor) {
-        s.p -= dp;
-        dp = m.p_floor - s.p;
-        s.p = m.p_floor;
-    }
+    // momentum floor: back out the increment, land on 0.25 p0
+    const bool low = s.p < m.p_floor;
+    const double pback = s.p - dp;
+    dp = low ? m.p_floor - pback : dp;
+    s.p = low ? m.p_floor : s.p;
     d[0] = (double)deltax;
     d[1] = (double)deltay;
     d[2] = NDIM == 2 ? 0.0 : (double)deltaz;
@@ -437,8 +453,8 @@ __device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i,
 }
 
 template <int NDIM, bool TINTERP, int FX, bool HALF = false>
-__device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s, bool fixed_dt,
-                                                  const float r[4], double d[4]) {
+__device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF& kf, State& s,
+                                                  bool fixed_dt, const float r[4], double d[4]) {
     constexpr bool DPP = (FX & FX_DPP) != 0;
     static_assert(!(HALF && DPP), "FP16-gradient records carry no momentum-diffusion slots");
     float f[RecShape<NDIM, DPP>::NC];
@@ -446,7 +462,7 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, State& s
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (HALF && NDIM == 3) gather_half<TINTERP>(m, I, rt, f);
     else gather<NDIM, TINTERP, DPP>(m, I, rt, f);
-    kappa_push<NDIM, FX>(m, s, f, fixed_dt, r, d);
+    kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
 }
 
 }  // namespace f32

@@ -74,7 +74,7 @@ module sp_hip_binding
     end type sp_variant
 
     type, bind(C) :: sp_tune_result
-        integer(c_int32_t) :: blocks_per_cu, sort_steps, ntrials, reserved0
+        integer(c_int32_t) :: blocks_per_cu, sort_steps, ntrials, particles_per_lane
         real(c_double) :: steps_per_s, default_steps_per_s
     end type sp_tune_result
 

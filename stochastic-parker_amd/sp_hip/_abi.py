@@ -92,7 +92,7 @@ class SpVariant(C.Structure):
 
 class SpTuneResult(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("sort_steps", C.c_int32), ("ntrials", C.c_int32),
-                ("reserved0", C.c_int32), ("steps_per_s", C.c_double),
+                ("particles_per_lane", C.c_int32), ("steps_per_s", C.c_double),
                 ("default_steps_per_s", C.c_double)]
 
 

@@ -191,7 +191,7 @@ class ParkerPusher:
         r = SpTuneResult()
         self._call("sp_autotune_mover", float(t0), float(dtf), int(num_fine_steps),
                    int(nsteps_interval), C.byref(r))
-        return {k: getattr(r, k) for k, _ in SpTuneResult._fields_ if k != "reserved0"}
+        return {k: getattr(r, k) for k, _ in SpTuneResult._fields_}
 
     def split_particle(self, split_ratio, pmin_split):
         self._call("sp_split_particle", float(split_ratio), float(pmin_split))

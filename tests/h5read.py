@@ -38,6 +38,10 @@ def lib():
             f.restype, f.argtypes = res, args
         if h.H5open() < 0:
             raise RuntimeError("H5open failed")
+        # errors are reported by return codes; no stack printing (the name
+        # listing below probes one index past the last link)
+        h.H5Eset_auto2.argtypes = [C.c_int64, C.c_void_p, C.c_void_p]
+        h.H5Eset_auto2(0, None, None)
         _lib = h
     return _lib
 

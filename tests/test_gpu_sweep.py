@@ -91,9 +91,15 @@ def test_sweep_scaled_matches_oracle_per_variant(sweep):
         ok = (p > cfg["pmin"]) & (p <= cfg["pmax"]) & (ib >= 0) & (ib < 256)
         want = np.bincount(ib[ok], weights=got["weight"][sel][ok], minlength=256)
         np.testing.assert_array_equal(f[k], want)
-    # the variants differ: the momentum spreads are not all alike
-    spread = [np.std(np.log(got["p"][got["origin"] == k])) for k in range(64)]
-    assert max(spread) > 3 * min(spread)
+    # the variants differ: kpara0 spans three decades, and so does the spread
+    # of the y displacements over the interval (y is open: no wrap)
+    s0 = by_id(start)
+    keep = np.isin(s0["tag_injected"], got["tag_injected"])
+    dy = got["y"] - s0["y"][keep]
+    spread = np.array([np.std(dy[got["origin"] == k]) for k in range(64)])
+    assert spread.max() > 10 * spread.min()
+    order = np.argsort(v["kpara0"])
+    assert spread[order[-8:]].mean() > 5 * spread[order[:8]].mean()
 
 
 def test_sweep_full_size_conserves_particles(sweep):

@@ -121,20 +121,26 @@ def test_variant_argument_errors():
     pz.close()
 
 
-def test_autotune_keeps_results_and_state():
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_autotune_keeps_results_and_state(precision):
     """The tuner's trials leave the particles as they were, and the tuned
     launch settings give the same per-particle results as the defaults."""
     c = cfg2d(32, pbcy=1, dt_max_rel=1e-2)
     nodes = nodes_for(c)
     runs = {}
     for tune in (False, True):
-        pz, start, _ = _batched(c, nodes, "f64", intervals=0)
+        pz, start, _ = _batched(c, nodes, precision, intervals=0)
         if tune:
             before = by_id(pz.get_particles())
             q0 = pz.quick_check()
             r = pz.autotune_mover(0.0, 0.05)
             assert r["ntrials"] >= 4 and 1 <= r["blocks_per_cu"] <= 4
             assert r["sort_steps"] in (0, 100, 200, 400)
+            if precision == "f64":
+                assert r["particles_per_lane"] == 1   # the FP64 mover takes one
+            else:
+                # one and two particles per lane were both tried
+                assert r["particles_per_lane"] in (1, 2) and r["ntrials"] >= 8
             assert r["steps_per_s"] > 0 and r["steps_per_s"] >= 0.5 * r["default_steps_per_s"]
             after = by_id(pz.get_particles())
             for k in before:
