@@ -232,6 +232,15 @@ int sp_split_particle(void* ctx, double split_ratio, double pmin_split);
  * (n <= nptl_current). */
 int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state,
                  const double* rands, const int8_t* fixed_dt, double* deltas);
+/* No counterpart in the reference.  sp_step_once without device allocation,
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same kernel and
+ * results, with the device staging of the rows in a fixed per-GPU array of
+ * the library (7.06 MiB, set up by sp_ctx_create).  At most 65536 rows per
+ * call (SP_ERR_CAPACITY beyond).  Calls on one GPU share that array: calls
+ * that overlap on one GPU (from several host threads or contexts) race. */
+int sp_step_once_noalloc(void* ctx, int64_t n, double t0, double dtf, double* state,
+                         const double* rands, const int8_t* fixed_dt, double* deltas);
 
 /* ---- launch autotuning (no reference counterpart) -------------------------
  * Times the mover over [t0, t0 + dtf] from the current particles for every

@@ -12,6 +12,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -111,6 +112,39 @@ struct Ctx {
 };
 
 Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+// Staging memory of sp_step_once_noalloc, one copy per GPU for the life of
+// the process: up to kNoAllocRows rows of state (6 doubles), increments (4
+// doubles), deltas (4 doubles) and fixed-dt flags (1 byte), 7.06 MiB.  In
+// host code `scratch` names a host-side stand-in; kernels and copies use the
+// device address that hipGetSymbolAddress returns for the current GPU, kept
+// per device in g_scratch.
+constexpr long long kNoAllocRows = 65536;
+constexpr size_t kScratchBytes = (size_t)kNoAllocRows * (14 * sizeof(double) + 1);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Looks up this GPU's copy of `scratch` once (the device must be current).
+int init_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(SP_ERR_UNSUPPORTED, "device index beyond the staging table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return SP_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SP_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+    return SP_OK;
+}
+
+void* device_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
 
 double fx_weight(unsigned __int128 v) { return (double)v * kInvFx; }
 
@@ -472,6 +506,8 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
             return bail(fail(SP_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_))); \
     } while (0)
     SP_HIP_C(hipGetDevice(&c->device));
+    rc = init_scratch(c->device);
+    if (rc != SP_OK) return bail(rc);
     hipDeviceProp_t prop;
     SP_HIP_C(hipGetDeviceProperties(&prop, c->device));
     c->ncu = prop.multiProcessorCount;
@@ -978,6 +1014,49 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     free_dev(dd);
     free_dev(df);
     if (e != hipSuccess) return fail(SP_ERR_HIP, std::string("step_once failed: ") + hipGetErrorString(e));
+    return SP_OK;
+}
+
+int sp_step_once_noalloc(void* ctx, int64_t n, double t0, double dtf, double* state,
+                         const double* rands, const int8_t* fixed_dt, double* deltas) {
+    int rc;
+    Ctx* c = live_ctx(ctx, rc);
+    if (!c) return rc;
+    if (n < 0) return fail(SP_ERR_ARG, "n must be >= 0");
+    if (n > kNoAllocRows)
+        return fail(SP_ERR_CAPACITY, "sp_step_once_noalloc takes at most " + std::to_string(kNoAllocRows) +
+                                         " rows per call");
+    if (n == 0) return SP_OK;
+    if (!state || !rands || !fixed_dt || !deltas) return fail(SP_ERR_ARG, "NULL buffer");
+    if (!c->have_fields[0]) return fail(SP_ERR_STATE, "fields (slot 0) are not prepared");
+    if (c->cfg.time_interp && !c->have_fields[1]) return fail(SP_ERR_STATE, "time_interp needs slot 1 prepared");
+    if (!(dtf > 0)) return fail(SP_ERR_ARG, "dtf must be > 0");
+    if (c->cfg.nlgc && n > c->nptl)
+        return fail(SP_ERR_ARG, "with nlgc, row i takes mu of particle i: need n <= nptl_current");
+    char* base = static_cast<char*>(device_scratch(c->device));
+    if (!base) return fail(SP_ERR_STATE, "the staging memory of this GPU was not set up");
+    // regions sized for kNoAllocRows rows; every byte a call reads is written
+    // by that call first (copies in, then the kernel's deltas)
+    double* ds = reinterpret_cast<double*>(base);
+    double* dr = ds + 6 * kNoAllocRows;
+    double* dd = dr + 4 * kNoAllocRows;
+    int8_t* df = reinterpret_cast<int8_t*>(dd + 4 * kNoAllocRows);
+    SP_HIP(hipMemcpyAsync(ds, state, sizeof(double) * 6 * n, hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipMemcpyAsync(dr, rands, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->st));
+    SP_HIP(hipMemcpyAsync(df, fixed_dt, (size_t)n, hipMemcpyHostToDevice, c->st));
+    sp::MoverParams m = mover_params(c, t0, dtf, 1, 1);
+    m.n = n;
+    const bool ti = c->cfg.time_interp != 0;
+    const int fx = step_features(c->cfg);
+    if (c->cfg.precision == SP_PREC_F32H)
+        SP_HIP(sp::launch_step_once_f32h(m, ti, fx, ds, dr, df, dd, c->st));
+    else if (c->cfg.precision == SP_PREC_F32)
+        SP_HIP(sp::launch_step_once_f32(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st));
+    else
+        SP_HIP(sp::launch_step_once_f64(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st));
+    SP_HIP(hipMemcpyAsync(state, ds, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipMemcpyAsync(deltas, dd, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, c->st));
+    SP_HIP(hipStreamSynchronize(c->st));
     return SP_OK;
 }
 

@@ -136,6 +136,8 @@ EXPORTS = {
     "sp_split_particle": [_P, C.c_double, C.c_double],
     "sp_sort_particles": [_P],
     "sp_step_once": [_P, C.c_int64, C.c_double, C.c_double, _D, _D, C.POINTER(C.c_int8), _D],
+    "sp_step_once_noalloc": [_P, C.c_int64, C.c_double, C.c_double, _D, _D,
+                             C.POINTER(C.c_int8), _D],
     "sp_distributions_init": [_P, C.POINTER(SpDistConfig)],
     "sp_distributions": [_P, _D, _D, _D, _D, _D],
     "sp_local_dist_dims": [_P, C.c_int32, C.POINTER(C.c_int64)],

@@ -200,14 +200,16 @@ class ParkerPusher:
         self._call("sp_sort_particles")
 
     def step_once(self, state: np.ndarray, rands: np.ndarray, fixed: np.ndarray, t0: float,
-                  dtf: float):
+                  dtf: float, noalloc: bool = False):
+        """One step per row (sp_step_once); noalloc uses sp_step_once_noalloc,
+        which stages the rows in the library's fixed per-GPU array."""
         st = np.ascontiguousarray(state, dtype=np.float64).copy()
         r = np.ascontiguousarray(rands, dtype=np.float64)
         fx = np.ascontiguousarray(fixed, dtype=np.int8)
         n = st.shape[0]
         d = np.zeros((n, 4), dtype=np.float64)
-        self._call("sp_step_once", n, float(t0), float(dtf), dptr(st), dptr(r),
-                   fx.ctypes.data_as(C.POINTER(C.c_int8)), dptr(d))
+        self._call("sp_step_once_noalloc" if noalloc else "sp_step_once", n, float(t0), float(dtf),
+                   dptr(st), dptr(r), fx.ctypes.data_as(C.POINTER(C.c_int8)), dptr(d))
         return st, d
 
     # -- diagnostics ------------------------------------------------------
