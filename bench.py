@@ -7,8 +7,8 @@ Workloads:
   3d            3D turbulence snapshot on a 512^3 grid with a compression
                 slab, 1e8 particles, 1000 diffusion-limited steps per interval,
                 particles sorted once per interval; FP32 arithmetic on node
-                records with FP16-stored gradients (--precision f32 for FP32
-                records).
+                records with FP16-stored velocity and gradients (--precision
+                f32 for FP32 records).
   batch         64 diffusion-parameter variants over the 512^2 2D snapshot in
                 one mover launch: kpara0, kret, momentum exponent and injection
                 region per variant, 1e5 .. 1e7 particles each (1.39e8 total).
@@ -31,9 +31,11 @@ Printed JSON line (one):
                  stream) against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz;
                  beside it the HBM view (algorithmic gather bytes and the
                  counter-measured HBM bytes per step against 8 TB/s)
-  c3             the 3D 512^3 snapshot (1e8 particles) on the FP16-gradient
-                 and FP32 node records, each with its rate, roofline and
-                 f(p) / spatial-map L1 against FP64 (--no-3d skips it)
+  c2_f32h        the same 2D workload on records with FP16-stored gradients:
+                 rate, roofline, f(p) / spatial-map L1 against FP64
+  c3             the 3D 512^3 snapshot (1e8 particles) on the FP16 and FP32
+                 node records, each with its rate, roofline and f(p) /
+                 spatial-map L1 against FP64 (--no-extras skips c2_f32h and c3)
   cpu_baseline   the FP64 CPU oracle (OpenMP, per-thread MT19937 increments) on
                  a bounded sample of the same workload, on this host
   accuracy       f(p) and spatial-map L1 of the FP32 path against the FP64
@@ -60,13 +62,21 @@ COPY_BYTES = 1 << 30
 
 
 def bytes_per_step(ndim, precision):
-    """Algorithmic gather bytes of one particle-step: the stored node-record
-    bytes the step consumes x the corners of its interpolation cell.  2D: 15
-    FP32 components (240 B); 3D: 21 FP32 components (672 B), or with
-    FP16-stored gradients 6 FP32 + 15 FP16 components (432 B)."""
-    if ndim == 2:
-        return 4 * 15 * 4
-    return 8 * (6 * 4 + 15 * 2) if precision == "f32h" else 8 * 21 * 4
+    """Algorithmic gather bytes of one particle-step: the stored bytes of the
+    node values the step consumes x the corners of its interpolation cell.
+    FP32 records hold V, B, divV and the B and |B| gradients: 2D 14 FP32
+    values per corner (224 B), 3D 19 (608 B).  FP16 records: 2D V and B as 5
+    FP32 and 9 FP16 gradients (152 B); 3D B as 3 FP32 and V, divV and the 12
+    gradients as 16 FP16 (352 B).  The FP64 path reads 15 / 21 FP32 node
+    values (240 / 672 B)."""
+    corners = 4 if ndim == 2 else 8
+    if precision == "f64":
+        return corners * (15 if ndim == 2 else 21) * 4
+    if precision == "f32h":
+        return corners * ((5 * 4 + 9 * 2) if ndim == 2 else (3 * 4 + 16 * 2))
+    return corners * (14 if ndim == 2 else 19) * 4
+
+
 DEFAULTS = {
     "2d": dict(grid=2048, nsteps=2000, nptl=1e8, cpu_nptl=1e6, precision="f32"),
     "3d": dict(grid=512, nsteps=1000, nptl=1e8, cpu_nptl=2e5, precision="f32h"),
@@ -173,25 +183,28 @@ def l1(a, b):
     return float(np.abs(a / a.sum() - b / b.sum()).sum())
 
 
-def accuracy(wl, n, intervals, fast="f32"):
-    """f(p) and map L1 of the fast path vs FP64 on the same particles and increments."""
+def accuracy(wl, n, intervals, fasts=("f32",)):
+    """f(p) and map L1 of each fast path vs FP64 on the same particles and
+    increments: {"f64_steps_per_s": .., path: {fp_l1, map_l1, steps_per_s}}."""
     out = {}
     res = {}
-    for prec in ("f64", fast):
+    for prec in ("f64",) + tuple(fasts):
         with ParkerPusher(**wl.config(prec, n)) as pz:
             wl.setup(pz, n)
-            r = run_intervals(wl, pz, intervals - 1, 1)
-            res[prec] = r
-            out[f"{prec}_steps_per_s"] = r["steps"] / r["seconds"]
-    f32, f64 = res[fast]["fp"], res["f64"]["fp"]
-    if wl.variants is not None:
-        per = [l1(a, b) for a, b in zip(f32, f64) if b.sum() > 0]
-        out["fp_l1"] = max(per)
-        out["fp_l1_mean"] = float(np.mean(per))
-        out["map_l1"] = None
-    else:
-        out["fp_l1"] = l1(f32, f64)
-        out["map_l1"] = l1(res[fast]["fmap"], res["f64"]["fmap"])
+            res[prec] = run_intervals(wl, pz, intervals - 1, 1)
+    out["f64_steps_per_s"] = res["f64"]["steps"] / res["f64"]["seconds"]
+    for fast in fasts:
+        e = {"steps_per_s": res[fast]["steps"] / res[fast]["seconds"]}
+        f32, f64 = res[fast]["fp"], res["f64"]["fp"]
+        if wl.variants is not None:
+            per = [l1(a, b) for a, b in zip(f32, f64) if b.sum() > 0]
+            e["fp_l1"] = max(per)
+            e["fp_l1_mean"] = float(np.mean(per))
+            e["map_l1"] = None
+        else:
+            e["fp_l1"] = l1(f32, f64)
+            e["map_l1"] = l1(res[fast]["fmap"], res["f64"]["fmap"])
+        out[fast] = e
     return out
 
 
@@ -250,10 +263,12 @@ def counter_run(wl, n, precision):
 
 
 def roofline(wl, n, precision, steps, kernel_ms, launches):
-    """The mover against its binding roof (VALU issue; the gathers are served
-    by L2 / MALL, so the algorithmic bytes overstate what reaches HBM), with
-    the HBM figures beside it.  Falls back to the HBM roof when no counter
-    run of this configuration is committed."""
+    """The mover against the VALU issue roof, its arithmetic ceiling (the
+    gathers are served by L2 / MALL, so the algorithmic bytes overstate what
+    reaches HBM), with the texture-path busy fractions and the HBM figures
+    beside it.  Instruction counts and busy fractions come from the committed
+    counter run of this exact configuration; without one the line falls back
+    to the HBM roof."""
     rate = steps / (kernel_ms * 1e-3)
     alg = bytes_per_step(wl.ndim, precision)
     t = counter_run(wl, n, precision)
@@ -264,16 +279,39 @@ def roofline(wl, n, precision, steps, kernel_ms, launches):
     kern = "mover_kernel (persistent; one launch per MHD interval)"
     if t and t.get("valu_instr_per_step"):
         lane_rate = t["valu_instr_per_step"] * rate
-        return {"bound": "valu_issue", "achieved": lane_rate / 1e12,
-                "peak": VALU_LANE_INSTR_PEAK / 1e12, "unit": "T lane-instr/s",
-                "frac": lane_rate / VALU_LANE_INSTR_PEAK,
-                "traffic": t.get("traffic_bytes_per_launch"), "kernel": kern,
-                "valu_instr_per_step": t["valu_instr_per_step"], "launches": launches,
-                "launch_ms": kernel_ms / max(launches, 1), "source": t.get("source"), "hbm": hbm}
+        out = {"bound": "valu_issue", "achieved": lane_rate / 1e12,
+               "peak": VALU_LANE_INSTR_PEAK / 1e12, "unit": "T lane-instr/s",
+               "frac": lane_rate / VALU_LANE_INSTR_PEAK,
+               "traffic": t.get("traffic_bytes_per_launch"), "kernel": kern,
+               "valu_instr_per_step": t["valu_instr_per_step"], "launches": launches,
+               "launch_ms": kernel_ms / max(launches, 1), "source": t.get("source"), "hbm": hbm}
+        if t.get("td_busy") is not None:
+            # the unit the counter run found saturated: the texture data path
+            # returning the gathered node records to the vector registers
+            out["vmem_path"] = {"ta_busy": t.get("ta_busy"), "td_busy": t["td_busy"],
+                                "vmem_instr_per_step": t.get("vmem_instr_per_step")}
+        return out
     return {"bound": "hbm", "achieved": hbm["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": hbm["algorithmic_frac"], "traffic": None, "kernel": kern, "launches": launches,
             "launch_ms": kernel_ms / max(launches, 1), "hbm": hbm,
             "note": "no committed counter run of this configuration"}
+
+
+def c2h_extra(a, wl, n, acc):
+    """The same 2D workload on FP16-gradient records: rate, roofline and the
+    accuracy of the shared particle sample."""
+    with ParkerPusher(**wl.config("f32h", n)) as pz:
+        wl.setup(pz, n)
+        r = run_intervals(wl, pz, 1, 1)
+    e = {"value": r["steps"] / r["seconds"], "unit": "particle-steps/s",
+         "particle_steps_timed": r["steps"], "seconds": r["seconds"],
+         "mover_kernel_ms": r["kernel_ms"], "precision": "f32h",
+         "roofline": roofline(wl, n, "f32h", r["steps"], r["kernel_ms"], 1)}
+    if acc is not None:
+        e["accuracy"] = {"sample_nptl": int(a.accuracy_nptl), "fp_l1_vs_f64": acc["f32h"]["fp_l1"],
+                         "map_l1_vs_f64": acc["f32h"]["map_l1"]}
+    log(f"2d f32h: {e['value']:.3e} particle-steps/s")
+    return e
 
 
 def c3_extra(a):
@@ -419,8 +457,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-3d", action="store_true",
-                    help="skip the 3D snapshot entries of the 2D run's line")
+    ap.add_argument("--no-extras", "--no-3d", dest="no_extras", action="store_true",
+                    help="skip the extra entries of the 2D run's line (FP16-record 2D path, "
+                         "3D snapshot)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the pusher shards by "
@@ -466,17 +505,20 @@ def main():
         },
         "roofline": roofline(wl, n, a.precision, r["steps"], r["kernel_ms"], a.steps),
     }
+    extra2 = a.workload == "2d" and not a.no_extras and a.precision == "f32"
+    acc = None
     if not a.no_accuracy:
         na = int(a.accuracy_nptl)
         fast = a.precision if a.precision != "f64" else "f32"
-        log(f"accuracy sample: {na} particles, {fast} vs f64")
-        acc = accuracy(wl, na, a.warmup + 1, fast)
-        result["accuracy"] = dict(sample_nptl=na, path=fast, fp_l1_vs_f64=acc["fp_l1"],
-                                  map_l1_vs_f64=acc["map_l1"],
+        fasts = (fast, "f32h") if extra2 else (fast,)
+        log(f"accuracy sample: {na} particles, {', '.join(fasts)} vs f64")
+        acc = accuracy(wl, na, a.warmup + 1, fasts)
+        result["accuracy"] = dict(sample_nptl=na, path=fast, fp_l1_vs_f64=acc[fast]["fp_l1"],
+                                  map_l1_vs_f64=acc[fast]["map_l1"],
                                   f64_steps_per_s=acc["f64_steps_per_s"],
-                                  fast_steps_per_s=acc[f"{fast}_steps_per_s"])
-        if "fp_l1_mean" in acc:
-            result["accuracy"]["fp_l1_mean_over_variants"] = acc["fp_l1_mean"]
+                                  fast_steps_per_s=acc[fast]["steps_per_s"])
+        if "fp_l1_mean" in acc[fast]:
+            result["accuracy"]["fp_l1_mean_over_variants"] = acc[fast]["fp_l1_mean"]
     if not a.no_cpu:
         nt = a.cpu_threads or min(16, os.cpu_count() or 1)
         nc = int(a.cpu_nptl or d["cpu_nptl"])
@@ -488,7 +530,9 @@ def main():
                       f"({s} steps, {secs:.1f} s), "
                       "FP64 oracle, AoS particles, 32-component node records, MT19937 per thread",
         }
-    if a.workload == "2d" and not a.no_3d:
+    if extra2:
+        result["c2_f32h"] = c2h_extra(a, wl, n, acc)
+    if a.workload == "2d" and not a.no_extras:
         result["c3"] = c3_extra(a)
     print(json.dumps(result), flush=True)
 

@@ -43,9 +43,10 @@ typedef enum sp_status {
 typedef enum sp_precision {
     SP_PREC_F64 = 0, /* reference arithmetic: FP32 fields, FP64 gather/coefficients/step */
     SP_PREC_F32 = 1, /* FP32 gather and coefficients; FP64 position, momentum and time */
-    SP_PREC_F32H = 2 /* 3D only: SP_PREC_F32 arithmetic on node records whose 15 gradient
-                        components are stored as FP16, scaled by a power of two per frame
-                        (64-byte records instead of 96) */
+    SP_PREC_F32H = 2 /* SP_PREC_F32 arithmetic on records whose gradients (divV, the
+                        B-component gradients, grad |B|) are stored as FP16, and in 3D
+                        also V, each scaled by a power of two per frame; B (and V in 2D)
+                        stay FP32.  Not with momentum diffusion. */
 } sp_precision;
 
 /* count_flag values, particle_module.f90:55-62 */

@@ -95,6 +95,7 @@ struct Ctx {
     int mover_bpc[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     int tuned_ppl = 0;                        // autotuned particles per lane (0: not tuned)
     float gscale[2] = {1.0f, 1.0f};           // FP16 records: gradient scale per frame
+    float vscale[2] = {1.0f, 1.0f};           // FP16 3D records: velocity scale per frame
     unsigned* gmax = nullptr;
     // distributions
     bool dist_ready = false;
@@ -194,8 +195,6 @@ int validate(const sp_config* c) {
     if (c->pbcx < 0 || c->pbcx > 1 || c->pbcy < 0 || c->pbcy > 1 || c->pbcz < 0 || c->pbcz > 1) return fail(SP_ERR_ARG, "pbc must be 0 or 1");
     if (c->precision != SP_PREC_F64 && c->precision != SP_PREC_F32 && c->precision != SP_PREC_F32H)
         return fail(SP_ERR_ARG, "precision must be SP_PREC_F64, SP_PREC_F32 or SP_PREC_F32H");
-    if (c->precision == SP_PREC_F32H && c->ndim != 3)
-        return fail(SP_ERR_UNSUPPORTED, "SP_PREC_F32H (FP16 gradient records) is for 3D grids");
     if (c->nptl_max < 1 || c->nptl_max > (1ll << 40)) return fail(SP_ERR_ARG, "nptl_max out of range");
     if (c->sort_steps < 0 || c->reserved0 != 0) return fail(SP_ERR_ARG, "need sort_steps >= 0 and reserved0 == 0");
     if (c->sort_every < 0) return fail(SP_ERR_ARG, "sort_every must be >= 0");
@@ -412,6 +411,8 @@ sp::MoverParams mover_params(Ctx* c, double t0, double dtf, int num_fine_steps, 
     m.dt_max = f.dt_max_rel * dtf;
     m.f_gs1 = 1.0f / c->gscale[0];
     m.f_gs2 = 1.0f / (f.time_interp ? c->gscale[1] : c->gscale[0]);
+    m.f_vs1 = 1.0f / c->vscale[0];
+    m.f_vs2 = 1.0f / (f.time_interp ? c->vscale[1] : c->vscale[0]);
     m.f_kpara0 = kv.f_kpara0;
     m.f_kret = kv.f_kret;
     m.f_pindex = kv.f_pindex;
@@ -493,7 +494,9 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     c->nmax = cfg->nptl_max;
     c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
     if (c->dp.ext_rec) c->recw = cfg->ndim == 2 ? sp::kRec2DX : sp::kRec3DX;
-    else c->recw = cfg->ndim == 2 ? sp::kRec2D : (cfg->precision == SP_PREC_F32H ? sp::kRec3DH : sp::kRec3D);
+    else if (cfg->precision == SP_PREC_F64) c->recw = cfg->ndim == 2 ? sp::kRec2D : sp::kRec3D;
+    else if (cfg->precision == SP_PREC_F32) c->recw = cfg->ndim == 2 ? sp::Pair2Shape::W32 : sp::kRec3F;
+    else c->recw = cfg->ndim == 2 ? sp::Pair2Shape::W16 : sp::kRec3H;
     c->nrec = cfg->ndim == 3 ? (long long)c->dp.bnx * c->dp.bny * c->dp.bnz * 512 : c->nnodes;
     auto bail = [&](int code) {
         destroy(c);
@@ -533,7 +536,7 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
         for (int ppl = 1; ppl <= 2; ++ppl) {
             c->mover_bpc[1][ppl - 1] = std::max(1, sp::mover_blocks_per_cu_f32(cfg->ndim, ti, fx, ppl));
             c->mover_bpc[2][ppl - 1] =
-                cfg->ndim == 3 ? std::max(1, sp::mover_blocks_per_cu_f32h(ti, fx & sp::FX_ACC, ppl)) : 1;
+                cfg->dpp_wave || cfg->dpp_shear ? 1 : std::max(1, sp::mover_blocks_per_cu_f32h(cfg->ndim, ti, fx, ppl));
         }
     }
     SP_HIP_C(hipStreamSynchronize(c->st));
@@ -575,29 +578,35 @@ int sp_fields_gradients(void* ctx, int32_t slot, float* farray32) {
     if (farray32) SP_HIP(hipMalloc(&fa, sizeof(float) * 32 * c->nnodes));
     hipError_t e = hipSuccess;
     if (c->cfg.precision == SP_PREC_F32H) {
-        // FP16 gradients: the largest |gradient| of the frame sets a power-of-two
-        // scale that maps it to 2^14 (exact to undo; no FP16 overflow)
-        if (!c->gmax) e = hipMalloc(&c->gmax, sizeof(unsigned));
-        unsigned bits = 0;
-        if (e == hipSuccess) e = hipMemsetAsync(c->gmax, 0, sizeof(unsigned), c->st);
-        if (e == hipSuccess) e = sp::launch_gradient_max(c->dp, c->n8[slot], c->gmax, c->st);
-        if (e == hipSuccess) e = hipMemcpyAsync(&bits, c->gmax, sizeof(unsigned), hipMemcpyDeviceToHost, c->st);
+        // FP16 values: the largest |gradient| and the largest |V| of the frame
+        // set power-of-two scales that map them into (2^13, 2^14] (exact to
+        // undo; no FP16 overflow)
+        if (!c->gmax) e = hipMalloc(&c->gmax, 2 * sizeof(unsigned));
+        unsigned bits[2] = {0u, 0u};
+        if (e == hipSuccess) e = hipMemsetAsync(c->gmax, 0, 2 * sizeof(unsigned), c->st);
+        if (e == hipSuccess) e = sp::launch_value_max(c->dp, c->n8[slot], c->gmax, c->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(bits, c->gmax, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->st);
         if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-        float gmax_v = 0.0f;
-        std::memcpy(&gmax_v, &bits, sizeof(float));
-        float scale = 1.0f;
-        if (gmax_v > 0.0f && std::isfinite(gmax_v)) {
-            const float q = 16384.0f / gmax_v;
-            if (std::isfinite(q)) {
-                int ex = 0;
-                std::frexp(q, &ex);   // 16384 / gmax in [2^(ex-1), 2^ex)
-                scale = std::ldexp(1.0f, ex - 1);
-            } else {
-                scale = std::ldexp(1.0f, 126);   // gradients below ~1e-34 everywhere
+        float sc[2] = {1.0f, 1.0f};
+        for (int q = 0; q < 2; ++q) {
+            float mx = 0.0f;
+            std::memcpy(&mx, &bits[q], sizeof(float));
+            if (mx > 0.0f && std::isfinite(mx)) {
+                const float r = 16384.0f / mx;
+                if (std::isfinite(r)) {
+                    int ex = 0;
+                    std::frexp(r, &ex);   // 16384 / max in [2^(ex-1), 2^ex)
+                    sc[q] = std::ldexp(1.0f, ex - 1);
+                } else {
+                    sc[q] = std::ldexp(1.0f, 126);   // values below ~1e-34 everywhere
+                }
             }
         }
-        c->gscale[slot] = scale;
-        if (e == hipSuccess) e = sp::launch_pack_half(c->dp, c->n8[slot], scale, c->rec[slot], fa, c->st);
+        c->gscale[slot] = sc[0];
+        c->vscale[slot] = sc[1];
+        if (e == hipSuccess) e = sp::launch_pack_fast(c->dp, c->n8[slot], true, sc[0], sc[1], c->rec[slot], fa, c->st);
+    } else if (c->cfg.precision == SP_PREC_F32 && !c->dp.ext_rec) {
+        e = sp::launch_pack_fast(c->dp, c->n8[slot], false, 1.0f, 1.0f, c->rec[slot], fa, c->st);
     } else {
         e = sp::launch_gradients(c->dp, c->n8[slot], c->rec[slot], fa, c->st);
     }
@@ -622,6 +631,7 @@ int sp_fields_copy(void* ctx) {
     SP_HIP(hipMemcpyAsync(c->n8[0], c->n8[1], sizeof(float) * 8 * c->nnodes, hipMemcpyDeviceToDevice, c->st));
     SP_HIP(hipStreamSynchronize(c->st));
     c->gscale[0] = c->gscale[1];
+    c->vscale[0] = c->vscale[1];
     c->have_fields[0] = true;
     return SP_OK;
 }
@@ -848,7 +858,7 @@ int mover_run(Ctx* c, double t0, double dtf, int num_fine_steps, int nsteps_inte
         SP_HIP(hipMemsetAsync(c->ctr, 0, offsetof(sp::Counters, leak_fx), c->st));
         SP_HIP(hipEventRecord(c->ev[1], c->st));
         const int grid = mover_grid(c, nrun, bpc, ppl);
-        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, fx, ppl, grid, c->st));
+        if (c->cfg.precision == SP_PREC_F32H) SP_HIP(sp::launch_mover_f32h(m, ti, fx, c->cfg.ndim, ppl, grid, c->st));
         else if (c->cfg.precision == SP_PREC_F32) SP_HIP(sp::launch_mover_f32(m, ti, fx, c->cfg.ndim, ppl, grid, c->st));
         else SP_HIP(sp::launch_mover_f64(m, ti, fx, c->cfg.ndim, grid, c->st));
         SP_HIP(hipEventRecord(c->ev[3], c->st));
@@ -1000,7 +1010,7 @@ int sp_step_once(void* ctx, int64_t n, double t0, double dtf, double* state, con
     const int fx = step_features(c->cfg);
     if (e == hipSuccess) {
         if (c->cfg.precision == SP_PREC_F32H)
-            e = sp::launch_step_once_f32h(m, ti, fx, ds, dr, df, dd, c->st);
+            e = sp::launch_step_once_f32h(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st);
         else if (c->cfg.precision == SP_PREC_F32)
             e = sp::launch_step_once_f32(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st);
         else
@@ -1049,7 +1059,7 @@ int sp_step_once_noalloc(void* ctx, int64_t n, double t0, double dtf, double* st
     const bool ti = c->cfg.time_interp != 0;
     const int fx = step_features(c->cfg);
     if (c->cfg.precision == SP_PREC_F32H)
-        SP_HIP(sp::launch_step_once_f32h(m, ti, fx, ds, dr, df, dd, c->st));
+        SP_HIP(sp::launch_step_once_f32h(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st));
     else if (c->cfg.precision == SP_PREC_F32)
         SP_HIP(sp::launch_step_once_f32(m, ti, fx, c->cfg.ndim, ds, dr, df, dd, c->st));
     else

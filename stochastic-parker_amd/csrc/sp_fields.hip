@@ -1,6 +1,8 @@
 // sp_fields.hip -- field preparation on the device: the 24 gradient
 // components of calc_fields_gradients (mhd_data_parallel.f90:504-605) and the
-// packing of each node into the record the mover gathers.
+// packing of each node into the record the mover gathers (node records for
+// the FP64 mover and the momentum-diffusion steps, the compact layouts of
+// the FP32 movers).
 //
 // Built with -ffp-contract=off: the FP32 differences, the FP64 scaling and the
 // FP32 store round exactly as in the Fortran, so the gradients are bit-exact
@@ -53,78 +55,153 @@ __global__ void __launch_bounds__(256) gradients_kernel(DevParams prm, double id
     }
 }
 
-__global__ void __launch_bounds__(256) gradient_max_kernel(DevParams prm, double idxh, double idyh,
-                                                           double idzh, const float* __restrict__ n8,
-                                                           unsigned* maxbits) {
-    __shared__ unsigned red[256];
-    const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
-    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    float mx = 0.0f;
-    if (node < total) {
-        float out[32];
-        node_components(prm, idxh, idyh, idzh, n8, node, out);
+// The FP32-mover slots of one node (pair2_src / node3_src order) from its
+// 32 farray components.
+template <int NDIM>
+__device__ __forceinline__ void fast_values(const float* out, float* v) {
 #pragma unroll
-        for (int s = R3_DVXDX; s <= R3_DBDZ; ++s) mx = fmaxf(mx, fabsf(out[rec3_src(s)]));
+    for (int s = 0; s < (NDIM == 2 ? Pair2Shape::N : 19); ++s) {
+        const int src = NDIM == 2 ? pair2_src(s) : node3_src(s);
+        if (src >= 0) v[s] = out[src];
+        else v[s] = NDIM == 2 ? out[8] + out[12] : (out[8] + out[12]) + out[16];
     }
-    red[threadIdx.x] = __float_as_uint(mx);   // non-negative floats order as integers
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + st]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(maxbits, red[0]);
 }
 
-__global__ void __launch_bounds__(256) pack_half_kernel(DevParams prm, double idxh, double idyh,
+// maxbits[0] = max |gradient slot| (divV included), maxbits[1] = max |V|
+template <int NDIM>
+__global__ void __launch_bounds__(256) value_max_kernel(DevParams prm, double idxh, double idyh,
                                                         double idzh, const float* __restrict__ n8,
-                                                        float scale, float* __restrict__ rec,
+                                                        unsigned* maxbits) {
+    constexpr int NF = NDIM == 2 ? Pair2Shape::NF : 6;
+    constexpr int N = NDIM == 2 ? Pair2Shape::N : 19;
+    __shared__ unsigned red[2][256];
+    const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float mg = 0.0f, mv = 0.0f;
+    if (node < total) {
+        float out[32], v[N];
+        node_components(prm, idxh, idyh, idzh, n8, node, out);
+        fast_values<NDIM>(out, v);
+#pragma unroll
+        for (int s = NF; s < N; ++s) mg = fmaxf(mg, fabsf(v[s]));
+#pragma unroll
+        for (int s = 0; s < NDIM; ++s) mv = fmaxf(mv, fabsf(v[s]));
+    }
+    red[0][threadIdx.x] = __float_as_uint(mg);   // non-negative floats order as integers
+    red[1][threadIdx.x] = __float_as_uint(mv);
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            red[0][threadIdx.x] = max(red[0][threadIdx.x], red[0][threadIdx.x + st]);
+            red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(maxbits, red[0][0]);
+        atomicMax(maxbits + 1, red[1][0]);
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_halves(float a, float b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)a) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
+}
+
+// One thread per node, the record layouts of the FP32 movers (Pair2Shape,
+// kRec3F, kRec3H).  2D: the x-pair record of node (i, j) from the node and
+// its +x neighbour (zeros past the last node of the row, which no cell
+// reads), node-major.  3D: the node's own record, bricked.
+template <int NDIM, bool HALF>
+__global__ void __launch_bounds__(256) pack_fast_kernel(DevParams prm, double idxh, double idyh,
+                                                        double idzh, const float* __restrict__ n8,
+                                                        float gscale, float vscale,
+                                                        float* __restrict__ rec,
                                                         float* __restrict__ fa32) {
+    using S = Pair2Shape;
+    constexpr int W = NDIM == 2 ? (HALF ? S::W16 : S::W32) : (HALF ? kRec3H : kRec3F);
+    constexpr int N = NDIM == 2 ? S::N : 19;
     const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
     const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= total) return;
-    float out[32];
-    node_components(prm, idxh, idyh, idzh, n8, node, out);
-    uint32_t wd[kRec3DH];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) wd[s] = __float_as_uint(out[rec3_src(s)]);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int s0 = R3_DVXDX + 2 * q, s1 = s0 + 1;
-        const _Float16 h0 = (_Float16)(out[rec3_src(s0)] * scale);
-        const _Float16 h1 = s1 <= R3_DBDZ ? (_Float16)(out[rec3_src(s1)] * scale) : (_Float16)0.0f;
-        wd[6 + q] = (uint32_t)__builtin_bit_cast(uint16_t, h0) |
-                    ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
-    }
-    wd[14] = wd[15] = 0u;
     const int i = (int)(node % prm.nxg);
     const int j = (int)((node / prm.nxg) % prm.nyg);
     const int k = (int)(node / ((long long)prm.nxg * prm.nyg));
-    uint4* r4 = reinterpret_cast<uint4*>(rec + brick_record(i, j, k, prm.bnx, prm.bny) * kRec3DH);
-#pragma unroll
-    for (int u = 0; u < kRec3DH / 4; ++u) r4[u] = make_uint4(wd[4 * u], wd[4 * u + 1], wd[4 * u + 2], wd[4 * u + 3]);
+    float out[32], v[2][N];
+    node_components(prm, idxh, idyh, idzh, n8, node, out);
     if (fa32) {
 #pragma unroll
         for (int c = 0; c < 32; ++c) fa32[node * 32 + c] = out[c];
     }
+    fast_values<NDIM>(out, v[0]);
+    uint32_t wd[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) wd[w] = 0u;
+    if constexpr (NDIM == 2) {
+        if (i + 1 < prm.nxg) {
+            node_components(prm, idxh, idyh, idzh, n8, node + 1, out);
+            fast_values<NDIM>(out, v[1]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < N; ++s) v[1][s] = 0.0f;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (!HALF) {
+#pragma unroll
+                for (int s = 0; s < N; ++s) wd[h * N + s] = __float_as_uint(v[h][s]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < S::NF; ++s) wd[h * S::NF + s] = __float_as_uint(v[h][s]);
+#pragma unroll
+                for (int q = 0; q < S::NGW; ++q) {
+                    const int s0 = S::NF + 2 * q, s1 = s0 + 1;
+                    wd[2 * S::NF + h * S::NGW + q] =
+                        pack_halves(v[h][s0] * gscale, s1 < N ? v[h][s1] * gscale : 0.0f);
+                }
+            }
+        }
+    } else if (!HALF) {
+#pragma unroll
+        for (int s = 0; s < N; ++s) wd[s] = __float_as_uint(v[0][s]);
+    } else {
+        // B as floats; halves: V (velocity scale), then divV and the gradients
+#pragma unroll
+        for (int s = 0; s < 3; ++s) wd[s] = __float_as_uint(v[0][3 + s]);
+        float hv[16];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) hv[s] = v[0][s] * vscale;
+#pragma unroll
+        for (int s = 6; s < N; ++s) hv[s - 3] = v[0][s] * gscale;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wd[3 + q] = pack_halves(hv[2 * q], hv[2 * q + 1]);
+    }
+    const long long r = NDIM == 3 ? brick_record(i, j, k, prm.bnx, prm.bny) : node;
+    uint4* r4 = reinterpret_cast<uint4*>(rec + r * W);
+#pragma unroll
+    for (int u = 0; u < W / 4; ++u) r4[u] = make_uint4(wd[4 * u], wd[4 * u + 1], wd[4 * u + 2], wd[4 * u + 3]);
 }
 
 }  // namespace
 
-hipError_t launch_gradient_max(const DevParams& prm, const float* nodes8, unsigned* maxbits,
-                               hipStream_t s) {
+hipError_t launch_value_max(const DevParams& prm, const float* nodes8, unsigned* maxbits,
+                            hipStream_t s) {
     const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
     const long long grid = (total + 255) / 256;
-    hipLaunchKernelGGL(gradient_max_kernel, dim3((unsigned)grid), dim3(256), 0, s, prm, 0.5 / prm.dx,
-                       0.5 / prm.dy, 0.5 / prm.dz, nodes8, maxbits);
+    auto k = prm.ndim == 2 ? value_max_kernel<2> : value_max_kernel<3>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(256), 0, s, prm, 0.5 / prm.dx, 0.5 / prm.dy,
+                       0.5 / prm.dz, nodes8, maxbits);
     return hipGetLastError();
 }
 
-hipError_t launch_pack_half(const DevParams& prm, const float* nodes8, float scale, float* rec,
-                            float* farray32, hipStream_t s) {
+hipError_t launch_pack_fast(const DevParams& prm, const float* nodes8, bool half, float gscale,
+                            float vscale, float* rec, float* farray32, hipStream_t s) {
     const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
     const long long grid = (total + 255) / 256;
-    hipLaunchKernelGGL(pack_half_kernel, dim3((unsigned)grid), dim3(256), 0, s, prm, 0.5 / prm.dx,
-                       0.5 / prm.dy, 0.5 / prm.dz, nodes8, scale, rec, farray32);
+    auto k = prm.ndim == 2 ? (half ? pack_fast_kernel<2, true> : pack_fast_kernel<2, false>)
+                           : (half ? pack_fast_kernel<3, true> : pack_fast_kernel<3, false>);
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(256), 0, s, prm, 0.5 / prm.dx, 0.5 / prm.dy,
+                       0.5 / prm.dz, nodes8, gscale, vscale, rec, farray32);
     return hipGetLastError();
 }
 

@@ -16,10 +16,6 @@ namespace sp {
 // holds 21 components plus three pads (96 B = six dwordx4 loads).
 constexpr int kRec2D = 16;
 constexpr int kRec3D = 24;
-// 3D record with FP16 gradients (SP_PREC_F32H), in 4-byte words: V and B as
-// six floats, then the 15 gradient slots of Rec3 (R3_DVXDX ..) as halves, two
-// per word, then padding (64 B = four dwordx4 loads).
-constexpr int kRec3DH = 16;
 // Records for steps with momentum diffusion (particle_module.f90:2918-2979):
 // the 2D / 3D slots above, then rho and the off-diagonal velocity gradients
 // of the flow-shear tensor.  2D: 18 components + 2 pads (80 B); 3D: 28
@@ -36,6 +32,47 @@ enum Rec3 { R3_VX, R3_VY, R3_VZ, R3_BX, R3_BY, R3_BZ, R3_DVXDX, R3_DVYDY, R3_DVZ
             R3_DBXDX, R3_DBXDY, R3_DBXDZ, R3_DBYDX, R3_DBYDY, R3_DBYDZ,
             R3_DBZDX, R3_DBZDY, R3_DBZDZ, R3_DBDX, R3_DBDY, R3_DBDZ,
             R3_RHO, R3_DVXDY, R3_DVXDZ, R3_DVYDX, R3_DVYDZ, R3_DVZDX, R3_DVZDY };
+
+// Records of the FP32 movers (steps without momentum diffusion).  The two
+// or three velocity gradients are stored as their sum divV, the only form
+// the step reads.  The texture path's cost follows the number of load
+// instructions rather than their width, so the layouts minimise dwordx4
+// loads per cell.  FP16 values are stored times a power-of-two scale of the
+// frame (one for the gradients, one for the velocity).
+//
+// 2D: x-pair records.  The record of node (i, j) holds that node and node
+// i + 1 of the same row, so the two x-corners of a cell arrive in one run of
+// loads (the 2D field set stays cache resident at twice the size).  Per node
+// NF = 5 field values (V, B) and NG = 9 gradients (divV, the B-component
+// gradients, grad |B|), in the order of pair2_src:
+//   FP32:  [node i: 14 floats][node i + 1: 14 floats]          (112 B, 7 loads)
+//   FP16:  [node i: 5 floats][node i + 1: 5 floats]
+//          [node i: 9 halves + 1 pad][node i + 1: same]         (80 B, 5 loads)
+// 3D: bricked node records (the 3D gather is sensitive to its cache
+// footprint, so nodes are not duplicated), in the order of node3_src:
+//   FP32:  V, B, divV and 12 gradients, 19 floats + 1 pad      (80 B, 5 loads)
+//   FP16:  B as 3 floats, then V, divV and the 12 gradients as
+//          16 halves, + 1 pad word                              (48 B, 3 loads)
+struct Pair2Shape {
+    static constexpr int NF = 5;
+    static constexpr int NG = 9;
+    static constexpr int N = NF + NG;
+    static constexpr int NGW = (NG + 1) / 2;   // words of halves per node
+    static constexpr int W32 = 28;
+    static constexpr int W16 = 20;
+};
+constexpr int kRec3F = 20;   // 3D FP32 record (floats)
+constexpr int kRec3H = 12;   // 3D FP16 record (words)
+// farray component of each slot; -1 marks divV (dvx/dx + dvy/dy [+ dvz/dz])
+__host__ __device__ inline int pair2_src(int s) {
+    const int m[14] = {0, 1, 4, 5, 6, -1, 20, 21, 23, 24, 26, 27, 29, 30};
+    return m[s];
+}
+// 3D slots: VX VY VZ BX BY BZ DIVV, then dBx/dx .. d|B|/dz (farray 20 .. 31)
+__host__ __device__ inline int node3_src(int s) {
+    const int m[19] = {0, 1, 2, 4, 5, 6, -1, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31};
+    return m[s];
+}
 
 // farray component (0-based) stored in each record slot
 // (slots past 14 / 20 belong to the momentum-diffusion records)
@@ -186,6 +223,7 @@ struct MoverParams {
     float f_kpara0, f_kret, f_pindex, f_gm2, f_vdp_pre, f_d1p0, f_d2p02;
     float f_half_dx, f_half_dy, f_half_dz, f_dt_min, f_dt_max;
     float f_gs1, f_gs2;           // FP16 records: 1 / gradient scale of frames 1 and 2
+    float f_vs1, f_vs2;           // FP16 3D records: 1 / velocity scale of frames 1 and 2
     double inv_p0;
     // acceleration region (template-enabled)
     double acc_region[6];
@@ -204,17 +242,18 @@ int set_error(int code, const std::string& msg);
 // Launchers (defined in the .hip translation units)
 hipError_t launch_gradients(const DevParams& prm, const float* nodes8, float* rec, float* farray32,
                             hipStream_t s);
-// FP16-gradient 3D records: max |gradient| over the 15 stored slots (as float
-// bits, into *maxbits), then the packing with the power-of-two scale.
-hipError_t launch_gradient_max(const DevParams& prm, const float* nodes8, unsigned* maxbits,
-                               hipStream_t s);
-hipError_t launch_pack_half(const DevParams& prm, const float* nodes8, float scale, float* rec,
-                            float* farray32, hipStream_t s);
+// Records of the FP32 movers.  FP16 records: max |gradient| and max |V| of
+// the frame (as float bits, into maxbits[0] and maxbits[1]) set its
+// power-of-two scales, then the packing applies them.
+hipError_t launch_value_max(const DevParams& prm, const float* nodes8, unsigned* maxbits,
+                            hipStream_t s);
+hipError_t launch_pack_fast(const DevParams& prm, const float* nodes8, bool half, float gscale,
+                            float vscale, float* rec, float* farray32, hipStream_t s);
 // ppl: particles per lane of the FP32 movers (1 or 2); the FP64 mover takes one
-int mover_blocks_per_cu_f32h(bool ti, int fx, int ppl);
-hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ppl, int grid,
+int mover_blocks_per_cu_f32h(int ndim, bool ti, int fx, int ppl);
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
                              hipStream_t s);
-hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* state,
+hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                  const double* rands, const int8_t* fixed, double* deltas,
                                  hipStream_t s);
 int mover_blocks_per_cu_f64(int ndim, bool ti, int fx);

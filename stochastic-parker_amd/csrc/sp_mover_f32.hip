@@ -6,7 +6,9 @@
 namespace sp {
 
 // FP32 gather and coefficients; one or two particles per lane, each with its
-// own copy of the per-particle diffusion parameters (KF).
+// own copy of the per-particle diffusion parameters (KF).  Steps without
+// momentum diffusion gather from x-pair records (FP32, or FP16 gradients
+// when HALF), those with it from the node records that carry its slots.
 template <bool HALF>
 struct PolicyF32Base {
     static constexpr int kMaxPPL = 2;
@@ -45,30 +47,31 @@ struct PolicyF32Base {
     }
 };
 
+// FP32 arithmetic on FP32 records.
 struct PolicyF32 : PolicyF32Base<false> {
     static constexpr bool kOnly3D = false;
     static constexpr bool kDpp = true;
 };
 
-// FP32 arithmetic on 3D records with FP16 gradients.
+// FP32 arithmetic on x-pair records with FP16 gradients.
 struct PolicyF32H : PolicyF32Base<true> {
-    static constexpr bool kOnly3D = true;
+    static constexpr bool kOnly3D = false;
     static constexpr bool kDpp = false;   // no momentum-diffusion slots in these records
 };
 
-int mover_blocks_per_cu_f32h(bool ti, int fx, int ppl) {
-    return mover_occupancy<PolicyF32H>(3, ti, fx, ppl);
+int mover_blocks_per_cu_f32h(int ndim, bool ti, int fx, int ppl) {
+    return mover_occupancy<PolicyF32H>(ndim, ti, fx, ppl);
 }
 
-hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ppl, int grid,
+hipError_t launch_mover_f32h(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
                              hipStream_t s) {
-    return launch_mover<PolicyF32H>(m, ti, fx, 3, ppl, grid, s);
+    return launch_mover<PolicyF32H>(m, ti, fx, ndim, ppl, grid, s);
 }
 
-hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, double* state,
+hipError_t launch_step_once_f32h(const MoverParams& m, bool ti, int fx, int ndim, double* state,
                                  const double* rands, const int8_t* fixed, double* deltas,
                                  hipStream_t s) {
-    return launch_step_once<PolicyF32H>(m, ti, fx, 3, state, rands, fixed, deltas, s);
+    return launch_step_once<PolicyF32H>(m, ti, fx, ndim, state, rands, fixed, deltas, s);
 }
 
 int mover_blocks_per_cu_f32(int ndim, bool ti, int fx, int ppl) {

@@ -125,63 +125,148 @@ __device__ __forceinline__ void gather(const MoverParams& m, const Interp& I, fl
 }
 
 
-// Gather from 3D records with FP16 gradients (kRec3DH): V, B as floats, the 15
-// gradient slots as halves scaled by the frame's power of two; the scale is
-// undone after the sum over corners (scaling by a power of two is exact).
-template <bool TINTERP>
-__device__ __forceinline__ void gather_half(const MoverParams& m, const Interp& I, float rt,
-                                            float* __restrict__ f) {
-    constexpr int NC = 21;
-    float g[TINTERP ? NC : 1];
+// FP16 value `t` of a record's halves starting at word w0.
+__device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
+    const uint32_t word = wd[w0 + t / 2];
+    const uint16_t bits = (uint16_t)((t & 1) ? (word >> 16) : (word & 0xFFFFu));
+    return (float)__builtin_bit_cast(_Float16, bits);
+}
+
+// 2D gather from x-pair records (Pair2Shape): the record of node (ix, iy + j)
+// holds corners (0, j) and (1, j).  The sums run over the corners in the
+// order of gather() above; FP16 gradients are summed as floats and the
+// frame's scale is undone after the sum (scaling by a power of two is exact).
+// Output in Rec2 slot order, divV in R2_DVXDX and zero in R2_DVYDY (the step
+// only adds the two).
+template <bool TINTERP, bool HALF>
+__device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp& I, float rt,
+                                             float* __restrict__ f) {
+    using S = Pair2Shape;
+    constexpr int W = HALF ? S::W16 : S::W32;
+    float a[S::N], g[TINTERP ? S::N : 1];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        f[c] = 0.0f;
+    for (int c = 0; c < S::N; ++c) {
+        a[c] = 0.0f;
         if (TINTERP) g[c] = 0.0f;
     }
 #pragma unroll
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
-        float* acc = frame == 0 ? f : g;
-        // all eight corners' loads go out before any is used
-        uint4 raw[8][4];
+        float* acc = frame == 0 ? a : g;
+        uint4 raw[2][W / 4];   // both rows' loads go out before any is used
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const long long node = (long long)I.ix + (long long)(I.iy + j) * m.nxg;
+            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+#pragma unroll
+            for (int u = 0; u < W / 4; ++u) raw[j][u] = r4[u];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            uint32_t wd[W];
+#pragma unroll
+            for (int u = 0; u < W / 4; ++u) {
+                wd[4 * u] = raw[j][u].x; wd[4 * u + 1] = raw[j][u].y;
+                wd[4 * u + 2] = raw[j][u].z; wd[4 * u + 3] = raw[j][u].w;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float wq = I.w[2 * j + h];
+#pragma unroll
+                for (int c = 0; c < S::N; ++c) {
+                    float v;
+                    if (!HALF) v = __uint_as_float(wd[h * S::N + c]);
+                    else if (c < S::NF) v = __uint_as_float(wd[h * S::NF + c]);
+                    else v = half_at(wd, 2 * S::NF + h * S::NGW, c - S::NF);
+                    acc[c] += v * wq;
+                }
+            }
+        }
+        if (HALF) {
+            const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
+#pragma unroll
+            for (int c = S::NF; c < S::N; ++c) acc[c] *= gs;
+        }
+    }
+    if (TINTERP) {
+#pragma unroll
+        for (int c = 0; c < S::N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[c] = a[c];   // VX .. BZ
+    f[R2_DVXDX] = a[5];
+    f[R2_DVYDY] = 0.0f;
+#pragma unroll
+    for (int c = R2_DBXDX; c <= R2_DBDY; ++c) f[c] = a[c - 1];
+}
+
+// 3D gather from the FP32 movers' bricked node records (kRec3F, kRec3H), in
+// the corner order of gather() above.  FP16 values (V with the velocity
+// scale, divV and the gradients with the gradient scale) are summed as
+// floats and scaled after the sum.  Output in Rec3 slot order, divV in
+// R3_DVXDX and zeros in R3_DVYDY, R3_DVZDZ.
+template <bool TINTERP, bool HALF>
+__device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp& I, float rt,
+                                             float* __restrict__ f) {
+    constexpr int W = HALF ? kRec3H : kRec3F;
+    constexpr int N = 19;
+    float a[N], g[TINTERP ? N : 1];
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+        a[c] = 0.0f;
+        if (TINTERP) g[c] = 0.0f;
+    }
+#pragma unroll
+    for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
+        const float* rec = frame == 0 ? m.rec1 : m.rec2;
+        float* acc = frame == 0 ? a : g;
+        uint4 raw[8][W / 4];   // all eight corners' loads go out before any is used
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
             const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
-            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * kRec3DH);
+            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) raw[q][u] = r4[u];
+            for (int u = 0; u < W / 4; ++u) raw[q][u] = r4[u];
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            uint32_t wd[14];
+            uint32_t wd[W];
 #pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                const uint4 t = raw[q][u];
-                wd[4 * u] = t.x; wd[4 * u + 1] = t.y; wd[4 * u + 2] = t.z; wd[4 * u + 3] = t.w;
-            }
-            {
-                const uint4 t = raw[q][3];
-                wd[12] = t.x; wd[13] = t.y;
+            for (int u = 0; u < W / 4; ++u) {
+                wd[4 * u] = raw[q][u].x; wd[4 * u + 1] = raw[q][u].y;
+                wd[4 * u + 2] = raw[q][u].z; wd[4 * u + 3] = raw[q][u].w;
             }
             const float wq = I.w[q];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc[c] += __uint_as_float(wd[c]) * wq;
-#pragma unroll
-            for (int c = 6; c < NC; ++c) {
-                const uint32_t word = wd[6 + (c - 6) / 2];
-                const uint16_t bits = (uint16_t)(((c - 6) & 1) ? (word >> 16) : (word & 0xFFFFu));
-                acc[c] += (float)__builtin_bit_cast(_Float16, bits) * wq;
+            for (int c = 0; c < N; ++c) {
+                float v;
+                if (!HALF) v = __uint_as_float(wd[c]);
+                else if (c >= 3 && c < 6) v = __uint_as_float(wd[c - 3]);   // B
+                else v = half_at(wd, 3, c < 3 ? c : c - 3);               // V, divV, gradients
+                acc[c] += v * wq;
             }
         }
-        const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
+        if (HALF) {
+            const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
+            const float vs = frame == 0 ? m.f_vs1 : m.f_vs2;
 #pragma unroll
-        for (int c = 6; c < NC; ++c) acc[c] *= gs;
+            for (int c = 0; c < 3; ++c) acc[c] *= vs;
+#pragma unroll
+            for (int c = 6; c < N; ++c) acc[c] *= gs;
+        }
     }
     if (TINTERP) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) f[c] = f[c] * (1.0f - rt) + g[c] * rt;
+        for (int c = 0; c < N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) f[c] = a[c];   // VX .. BZ
+    f[R3_DVXDX] = a[6];
+    f[R3_DVYDY] = 0.0f;
+    f[R3_DVZDZ] = 0.0f;
+#pragma unroll
+    for (int c = R3_DBXDX; c <= R3_DBDZ; ++c) f[c] = a[c - 2];
 }
 
 // Momentum diffusion in FP32 (f64::dpp_terms, particle_module.f90:2918-2979):
@@ -460,8 +545,9 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF
     float f[RecShape<NDIM, DPP>::NC];
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
-    if constexpr (HALF && NDIM == 3) gather_half<TINTERP>(m, I, rt, f);
-    else gather<NDIM, TINTERP, DPP>(m, I, rt, f);
+    if constexpr (DPP) gather<NDIM, TINTERP, DPP>(m, I, rt, f);
+    else if constexpr (NDIM == 2) gather_pair2<TINTERP, HALF>(m, I, rt, f);
+    else gather_node3<TINTERP, HALF>(m, I, rt, f);
     kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
 }
 

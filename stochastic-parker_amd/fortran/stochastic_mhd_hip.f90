@@ -7,7 +7,7 @@
 !<
 !< GPU-only flags: -gp/--gpu_precision (f64 = reference arithmetic, default;
 !< f32 = FP32 gather/coefficients with FP64 positions and time; f32h = f32 on
-!< 3D records with FP16 gradients),
+!< records with FP16 gradients),
 !< -rs/--rng_seed (Philox key) and -ss/--sort_steps (spatial re-sort of the
 !< particles every that many steps inside an interval; 0 = once per interval).
 !*******************************************************************************

@@ -324,16 +324,12 @@ def test_spatial_sort_is_a_stable_permutation():
     pz.close()
 
 
-def test_f32h_is_3d_only():
-    with pytest.raises(Exception) as ei:
-        ParkerPusher(**dict(cfg2d(16), precision="f32h"))
-    assert "3D" in str(ei.value)
-
-
-def test_f32h_steps_close_to_f64():
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_f32h_steps_close_to_f64(ndim):
     """FP16-stored gradients change only the drift and kappa-divergence terms,
-    by about the FP16 rounding (2^-11 relative) of the gradients."""
-    c = cfg3d(12, mag_dependency=0)
+    by about the FP16 rounding (2^-11 relative) of the gradients; the FP32
+    x-pair records (divV stored as one sum) stay at FP32 rounding."""
+    c = cfg2d(16, mag_dependency=0) if ndim == 2 else cfg3d(12, mag_dependency=0)
     nodes = nodes_for(c)
     n = 3000
     a = random_states(c, n, seed=21)
@@ -348,17 +344,21 @@ def test_f32h_steps_close_to_f64():
         pz.close()
     d64, d32, d16 = out["f64"][1], out["f32"][1], out["f32h"][1]
     for d in (d32, d16):
-        scale = np.abs(d64[:, :3]).max(axis=1, keepdims=True)
-        rel = np.abs(d[:, :3] - d64[:, :3]) / scale
+        scale = np.abs(d64[:, :ndim]).max(axis=1, keepdims=True)
+        rel = np.abs(d[:, :ndim] - d64[:, :ndim]) / scale
         assert np.median(rel) < 2e-4 and rel.max() < 2e-2
     # the FP16 records move the result by more than FP32 rounding, but not much
-    e32 = np.abs(d32[:, :3] - d64[:, :3]).mean()
-    e16 = np.abs(d16[:, :3] - d64[:, :3]).mean()
+    e32 = np.abs(d32[:, :ndim] - d64[:, :ndim]).mean()
+    e16 = np.abs(d16[:, :ndim] - d64[:, :ndim]).mean()
     assert e16 > e32 and e16 < 1e3 * e32 + 1e-12
+    # momentum: dp = -(p/3) divV dt from the interpolated divV
+    rel_p = np.abs(d32[:, 3] - d64[:, 3]) / np.maximum(np.abs(d64[:, 3]), 1e-300)
+    assert np.median(rel_p) < 1e-5
 
 
-def test_f32h_mover_distribution_close_to_f64():
-    c = cfg3d(12, pbcx=1, dt_max_rel=1e-2)
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_f32h_mover_distribution_close_to_f64(ndim):
+    c = cfg2d(32, pbcx=1, dt_max_rel=1e-2) if ndim == 2 else cfg3d(12, pbcx=1, dt_max_rel=1e-2)
     nodes = nodes_for(c)
     res = {}
     for prec in ("f64", "f32h"):
@@ -379,11 +379,13 @@ def test_f32h_mover_distribution_close_to_f64():
     assert np.abs(h64 / h64.sum() - h16 / h16.sum()).sum() < 0.01
 
 
+@pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("precision", ["f32", "f32h"])
-def test_3d_fast_paths_with_time_interpolation_close_to_f64(precision):
+def test_fast_paths_with_time_interpolation_close_to_f64(precision, ndim):
     """Two field frames blended in time: the FP32 and FP16-gradient gathers
     read both frames (each with its own gradient scale) as the FP64 path does."""
-    c = cfg3d(12, pbcx=1, dt_max_rel=1e-2, time_interp=1)
+    c = (cfg2d(32, pbcx=1, dt_max_rel=1e-2, time_interp=1) if ndim == 2
+         else cfg3d(12, pbcx=1, dt_max_rel=1e-2, time_interp=1))
     n1, n2 = nodes_for(c, seed=1), nodes_for(c, seed=2)
     res = {}
     for prec in ("f64", precision):
