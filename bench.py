@@ -123,11 +123,11 @@ class Workload:
         log(f"generated {self.desc} in {time.perf_counter() - t0:.1f} s")
 
     def config(self, precision, n):
-        """sp_config of a run on this workload.  In 3D only the FP16-gradient
-        records run fastest sorted once per interval; the FP64 and FP32-record
-        paths re-sort every 200 steps (results do not depend on it)."""
+        """sp_config of a run on this workload.  In 3D the FP32 movers run
+        fastest sorted once per interval; the FP64 path re-sorts every 200
+        steps (results do not depend on it)."""
         c = dict(self.cfg, precision=precision, nptl_max=n)
-        if self.kind == "3d" and precision != "f32h":
+        if self.kind == "3d" and precision == "f64":
             c["sort_steps"] = 200
         return c
 
