@@ -17,13 +17,15 @@ struct PolicyF32Base {
     __device__ __forceinline__ static void set_var(MoverParams&, Var& v, const KappaVar& kv) {
         v = f32::kf_of(kv);
     }
-    template <int NDIM, bool TI, int FX>
+    template <int NDIM, bool TI, int FX, int PPL>
+    using Cache = f32::GatherCache<NDIM, TI, HALF, (FX & FX_DPP) != 0, PPL>;
+    template <int NDIM, bool TI, int FX, class GC>
     __device__ __forceinline__ static void step(const MoverParams& m, const Var& kf, State& s,
                                                 bool fixed, uint32_t tagi, uint32_t tags,
-                                                uint32_t rstep, double d[4]) {
+                                                uint32_t rstep, double d[4], GC& gc) {
         float r[4];
         f32::step_rands(m, tagi, tags, rstep, r);
-        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, r, d);
+        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, r, d, gc);
     }
     // PPL particles: the Philox blocks, the gathers and the steps of all
     // slots in one block, for the scheduler to interleave
@@ -34,16 +36,18 @@ struct PolicyF32Base {
         float r[PPL][4];
 #pragma unroll
         for (int k = 0; k < PPL; ++k) f32::step_rands(m, sl[k].tagi, sl[k].tags, sl[k].rstep, r[k]);
+        Cache<NDIM, TI, FX, PPL> none;   // no records kept across steps at PPL > 1
 #pragma unroll
         for (int k = 0; k < PPL; ++k)
-            f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf[k], s[k], fixed[k], r[k], d[k]);
+            f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf[k], s[k], fixed[k], r[k], d[k], none);
     }
     template <int NDIM, bool TI, int FX>
     __device__ __forceinline__ static void step_given(const MoverParams& m, const Var& kf,
                                                       State& s, bool fixed, const double* r,
                                                       double d[4]) {
         const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
-        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, rf, d);
+        Cache<NDIM, TI, FX, 2> none;   // one step: nothing to keep
+        f32::gather_kappa_push<NDIM, TI, FX, HALF>(m, kf, s, fixed, rf, d, none);
     }
 };
 

@@ -20,10 +20,15 @@ struct PolicyF64 {
         m.p0_2mpi = kv.p0_2mpi;
         m.perp_mag = kv.perp_mag; m.perp_mom = kv.perp_mom;
     }
-    template <int NDIM, bool TI, int FX>
+    // no records are kept across steps
+    template <int NDIM, bool TI, int FX, int PPL>
+    struct Cache {
+        __device__ __forceinline__ void invalidate() {}
+    };
+    template <int NDIM, bool TI, int FX, class GC>
     __device__ __forceinline__ static void step(const MoverParams& m, const Var&, State& s,
                                                 bool fixed, uint32_t tagi, uint32_t tags,
-                                                uint32_t rstep, double d[4]) {
+                                                uint32_t rstep, double d[4], GC&) {
         double r[4];
         f64::step_rands(m, tagi, tags, rstep, r);
         f64::gather_kappa_push<NDIM, TI, FX>(m, s, fixed, r, d);

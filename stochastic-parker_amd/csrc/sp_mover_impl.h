@@ -125,6 +125,8 @@ mover_kernel(MoverParams m0) {
     int part = (int)(xcc_id() % (unsigned)kParts), parts_done = 0;
     Slot sl[PPL];
     typename Pol::Var var[PPL];
+    typename Pol::template Cache<NDIM, TINTERP, FX, PPL> gc;   // slot 0's last records
+    gc.invalidate();
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
         sl[k].idx = -1;
@@ -165,6 +167,7 @@ mover_kernel(MoverParams m0) {
                     pb + (long long)(base + (unsigned long long)__popcll(need_mask & ((1ull << lane) - 1ull)));
                 if (j < pe) {
                     q.idx = j;
+                    if (k == 0) gc.invalidate();
                     q.s.x = px[j];
                     q.s.y = py[j];
                     q.s.z = pz[j];
@@ -315,7 +318,7 @@ mover_kernel(MoverParams m0) {
         if (any_push) {
             if constexpr (PPL == 1) {
                 Pol::template step<NDIM, TINTERP, FX>(m, var[0], sl[0].s, fixed[0], sl[0].tagi,
-                                                      sl[0].tags, sl[0].rstep, sl[0].dl);
+                                                      sl[0].tags, sl[0].rstep, sl[0].dl, gc);
             } else {
                 // every slot steps (one straight-line block); a slot without a
                 // step to take keeps its state

@@ -5,8 +5,9 @@
 // in FP32; position, momentum and elapsed time stay FP64 accumulators.
 // Arithmetic is regrouped for the hardware: reciprocals and square roots use
 // the single-instruction v_rcp_f32 / v_rsq_f32 / v_sqrt_f32, powers are
-// exp2(y * log2(x)) on v_log_f32 / v_exp_f32, and contraction into FMA is
-// allowed.
+// exp2(y * log2(x)) on v_log_f32 / v_exp_f32, and a multiply and an add of
+// one expression contract into an FMA (never across statements, so every
+// kernel that runs this step rounds it alike).
 #pragma once
 
 #include "sp_device.h"
@@ -132,15 +133,34 @@ __device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
     return (float)__builtin_bit_cast(_Float16, bits);
 }
 
+// Records of the cell a lane's particle last gathered from, kept in
+// registers across its steps: fields do not change within a mover launch, so
+// a particle that stays in its cell reuses them and only lanes that moved
+// issue loads (the texture path's cost follows the active lanes of a load).
+// Used by the one-frame 2D step of one particle per lane; empty elsewhere.
+template <int NDIM, bool TINTERP, bool HALF, bool DPP, int PPL>
+struct GatherCache {
+    static constexpr bool kOn = false;
+    __device__ __forceinline__ void invalidate() {}
+};
+template <bool HALF>
+struct GatherCache<2, false, HALF, false, 1> {
+    static constexpr bool kOn = true;
+    static constexpr int kW = HALF ? Pair2Shape::W16 : Pair2Shape::W32;
+    uint4 raw[2][kW / 4];
+    long long key;   // node index of the cached row-0 pair record; -1: none
+    __device__ __forceinline__ void invalidate() { key = -1; }
+};
+
 // 2D gather from x-pair records (Pair2Shape): the record of node (ix, iy + j)
 // holds corners (0, j) and (1, j).  The sums run over the corners in the
 // order of gather() above; FP16 gradients are summed as floats and the
 // frame's scale is undone after the sum (scaling by a power of two is exact).
 // Output in Rec2 slot order, divV in R2_DVXDX and zero in R2_DVYDY (the step
 // only adds the two).
-template <bool TINTERP, bool HALF>
+template <bool TINTERP, bool HALF, class GC>
 __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp& I, float rt,
-                                             float* __restrict__ f) {
+                                             float* __restrict__ f, GC& gc) {
     using S = Pair2Shape;
     constexpr int W = HALF ? S::W16 : S::W32;
     float a[S::N], g[TINTERP ? S::N : 1];
@@ -153,13 +173,29 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
+        const long long node0 = (long long)I.ix + (long long)I.iy * m.nxg;
         uint4 raw[2][W / 4];   // both rows' loads go out before any is used
+        if constexpr (GC::kOn) {
+            if (gc.key != node0) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const long long node = (long long)I.ix + (long long)(I.iy + j) * m.nxg;
-            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+                for (int j = 0; j < 2; ++j) {
+                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + (node0 + j * (long long)m.nxg) * W);
 #pragma unroll
-            for (int u = 0; u < W / 4; ++u) raw[j][u] = r4[u];
+                    for (int u = 0; u < W / 4; ++u) gc.raw[j][u] = r4[u];
+                }
+                gc.key = node0;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int u = 0; u < W / 4; ++u) raw[j][u] = gc.raw[j][u];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + (node0 + j * (long long)m.nxg) * W);
+#pragma unroll
+                for (int u = 0; u < W / 4; ++u) raw[j][u] = r4[u];
+            }
         }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -537,16 +573,17 @@ __device__ __forceinline__ void step_rands(const MoverParams& m, uint32_t tag_i,
     r[3] = (float)u.d * sc + of;
 }
 
-template <int NDIM, bool TINTERP, int FX, bool HALF = false>
+template <int NDIM, bool TINTERP, int FX, bool HALF, class GC>
 __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF& kf, State& s,
-                                                  bool fixed_dt, const float r[4], double d[4]) {
+                                                  bool fixed_dt, const float r[4], double d[4],
+                                                  GC& gc) {
     constexpr bool DPP = (FX & FX_DPP) != 0;
     static_assert(!(HALF && DPP), "FP16-gradient records carry no momentum-diffusion slots");
     float f[RecShape<NDIM, DPP>::NC];
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (DPP) gather<NDIM, TINTERP, DPP>(m, I, rt, f);
-    else if constexpr (NDIM == 2) gather_pair2<TINTERP, HALF>(m, I, rt, f);
+    else if constexpr (NDIM == 2) gather_pair2<TINTERP, HALF>(m, I, rt, f, gc);
     else gather_node3<TINTERP, HALF>(m, I, rt, f);
     kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
 }
