@@ -137,7 +137,8 @@ __device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
 // registers across its steps: fields do not change within a mover launch, so
 // a particle that stays in its cell reuses them and only lanes that moved
 // issue loads (the texture path's cost follows the active lanes of a load).
-// Used by the one-frame 2D step of one particle per lane; empty elsewhere.
+// Used by the one-frame step of one particle per lane, 2D and 3D FP16 (3D
+// FP32 records would take 160 registers); empty elsewhere.
 template <int NDIM, bool TINTERP, bool HALF, bool DPP, int PPL>
 struct GatherCache {
     static constexpr bool kOn = false;
@@ -149,6 +150,14 @@ struct GatherCache<2, false, HALF, false, 1> {
     static constexpr int kW = HALF ? Pair2Shape::W16 : Pair2Shape::W32;
     uint4 raw[2][kW / 4];
     long long key;   // node index of the cached row-0 pair record; -1: none
+    __device__ __forceinline__ void invalidate() { key = -1; }
+};
+
+template <>
+struct GatherCache<3, false, true, false, 1> {
+    static constexpr bool kOn = true;
+    uint4 raw[8][kRec3H / 4];
+    long long key;   // node index of the cached cell's corner 0; -1: none
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
 
@@ -241,9 +250,9 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
 // scale, divV and the gradients with the gradient scale) are summed as
 // floats and scaled after the sum.  Output in Rec3 slot order, divV in
 // R3_DVXDX and zeros in R3_DVYDY, R3_DVZDZ.
-template <bool TINTERP, bool HALF>
+template <bool TINTERP, bool HALF, class GC>
 __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp& I, float rt,
-                                             float* __restrict__ f) {
+                                             float* __restrict__ f, GC& gc) {
     constexpr int W = HALF ? kRec3H : kRec3F;
     constexpr int N = 19;
     float a[N], g[TINTERP ? N : 1];
@@ -257,13 +266,32 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
         uint4 raw[8][W / 4];   // all eight corners' loads go out before any is used
+        if constexpr (GC::kOn) {
+            const long long cell = (long long)I.ix + (long long)m.nxg * (I.iy + (long long)m.nyg * I.iz);
+            if (gc.key != cell) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-            const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
-            const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+                for (int q = 0; q < 8; ++q) {
+                    const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
+                    const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
+                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
 #pragma unroll
-            for (int u = 0; u < W / 4; ++u) raw[q][u] = r4[u];
+                    for (int u = 0; u < W / 4; ++u) gc.raw[q][u] = r4[u];
+                }
+                gc.key = cell;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int u = 0; u < W / 4; ++u) raw[q][u] = gc.raw[q][u];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
+                const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+#pragma unroll
+                for (int u = 0; u < W / 4; ++u) raw[q][u] = r4[u];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -584,7 +612,7 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (DPP) gather<NDIM, TINTERP, DPP>(m, I, rt, f);
     else if constexpr (NDIM == 2) gather_pair2<TINTERP, HALF>(m, I, rt, f, gc);
-    else gather_node3<TINTERP, HALF>(m, I, rt, f);
+    else gather_node3<TINTERP, HALF>(m, I, rt, f, gc);
     kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
 }
 
