@@ -42,9 +42,12 @@ namespace {
     } while (0)
 
 constexpr double kInvFx = 1.0 / sp::kFxScale;
-// Resident 256-lane mover blocks per CU: 2D records (64 B) of a die's window
-// fit its L2 with two blocks, 3D records (96 B, 8 corners) with one.
-constexpr int kMoverBlocksPerCU2D = 2;
+// Resident 256-lane mover blocks per CU (capped by the kernel's occupancy):
+// 2D FP32 movers three (their one-frame kernels are built for three waves per
+// SIMD), the 2D FP64 mover two, 3D one (its window of records then stays in
+// the die's L2).
+constexpr int kMoverBlocksPerCU2D = 3;
+constexpr int kMoverBlocksPerCU2D64 = 2;
 constexpr int kMoverBlocksPerCU3D = 1;
 // Particles per lane of the FP32 movers unless tuned or overridden.
 constexpr int kMoverPPLDefault = 1;
@@ -280,10 +283,11 @@ int mover_grid(Ctx* c, long long n, int bpc = 0, int ppl = 1) {
     const int occ = c->mover_bpc[prec][ppl - 1];
     // Resident 256-lane blocks per CU.  Particles in flight are, on each XCD,
     // a window of consecutive (spatially sorted) indices; a smaller window
-    // keeps its field records in that XCD's L2.  Measured on MI355X: two
-    // blocks per CU on the 2048^2 snapshot, one on the 512^3 snapshot, for
-    // both precisions; SP_MOVER_BLOCKS_PER_CU overrides.
-    const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D : kMoverBlocksPerCU2D;
+    // keeps its field records in that XCD's L2.  Measured on MI355X: three
+    // blocks per CU for the FP32 movers on the 2048^2 snapshot (two for
+    // FP64), one on the 512^3 snapshot; SP_MOVER_BLOCKS_PER_CU overrides.
+    const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D
+                               : (prec == SP_PREC_F64 ? kMoverBlocksPerCU2D64 : kMoverBlocksPerCU2D);
     int per_cu = std::min(std::max(1, occ), per_cu_default);
     if (c->tuned_bpc > 0) per_cu = std::min(std::max(1, occ), c->tuned_bpc);
     if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {

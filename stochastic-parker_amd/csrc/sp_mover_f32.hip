@@ -12,6 +12,7 @@ namespace sp {
 template <bool HALF>
 struct PolicyF32Base {
     static constexpr int kMaxPPL = 2;
+    static constexpr bool kThreeWave2D = true;
     using Var = f32::KF;
     __device__ __forceinline__ static Var var_of(const MoverParams& m) { return f32::kf_of(m); }
     __device__ __forceinline__ static void set_var(MoverParams&, Var& v, const KappaVar& kv) {

@@ -98,16 +98,25 @@ struct Slot {
 // the compiler; two 3D particles need the one-wave budget.
 template <int NDIM, int PPL>
 constexpr bool kTwoWavePair = NDIM == 2 && PPL == 2;
+// The one-frame 2D step of the FP32 movers with one particle per lane runs
+// three waves per SIMD (141-152 VGPRs, the constants left to the compiler):
+// once its gathers come from the register cache, the step is bound by issue
+// and latency rather than the texture path, and a third wave hides more of
+// the latency than pinned constants save (-9 % kernel time against two
+// pinned waves).  Two-frame steps would spill at that budget.
+template <class Pol, int NDIM, bool TINTERP, int PPL>
+constexpr bool kThreeWave = Pol::kThreeWave2D && NDIM == 2 && !TINTERP && PPL == 1;
 
 template <class Pol, int NDIM, bool TINTERP, int FX, int PPL>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(kTwoWavePair<NDIM, PPL> ? 2 : 1, 2)))
+__attribute__((amdgpu_waves_per_eu(kThreeWave<Pol, NDIM, TINTERP, PPL> ? 3 : (kTwoWavePair<NDIM, PPL> ? 2 : 1),
+                                   kThreeWave<Pol, NDIM, TINTERP, PPL> ? 3 : 2)))
 mover_kernel(MoverParams m0) {
     static_assert(PPL >= 1 && PPL <= Pol::kMaxPPL, "particles per lane of this policy");
     MoverParams m = m0;
     // within the two-wave register budget the constants fit in VGPRs for 2D
     // and 3D alike (2D: +5 %, 3D: +1 %)
-    if constexpr (!kTwoWavePair<NDIM, PPL>) pin_constants(m);
+    if constexpr (!kTwoWavePair<NDIM, PPL> && !kThreeWave<Pol, NDIM, TINTERP, PPL>) pin_constants(m);
     const int lane = threadIdx.x & (kWave - 1);
     double* const px = m.P.d(PD_X);
     double* const py = m.P.d(PD_Y);
