@@ -42,12 +42,13 @@ namespace {
     } while (0)
 
 constexpr double kInvFx = 1.0 / sp::kFxScale;
-// Resident 256-lane mover blocks per CU (capped by the kernel's occupancy):
+// Resident 256-lane mover blocks per CU, capped by the kernel's occupancy:
 // 2D FP32 movers three (their one-frame kernels are built for three waves per
-// SIMD), the 2D FP64 mover two, 3D one (its window of records then stays in
-// the die's L2).
+// SIMD), the 2D FP64 mover two; 3D FP16 records two (their register cache
+// leaves the die's L2 a smaller window to hold), other 3D movers one.
 constexpr int kMoverBlocksPerCU2D = 3;
 constexpr int kMoverBlocksPerCU2D64 = 2;
+constexpr int kMoverBlocksPerCU3DH = 2;
 constexpr int kMoverBlocksPerCU3D = 1;
 // Particles per lane of the FP32 movers unless tuned or overridden.
 constexpr int kMoverPPLDefault = 1;
@@ -285,9 +286,11 @@ int mover_grid(Ctx* c, long long n, int bpc = 0, int ppl = 1) {
     // a window of consecutive (spatially sorted) indices; a smaller window
     // keeps its field records in that XCD's L2.  Measured on MI355X: three
     // blocks per CU for the FP32 movers on the 2048^2 snapshot (two for
-    // FP64), one on the 512^3 snapshot; SP_MOVER_BLOCKS_PER_CU overrides.
-    const int per_cu_default = c->cfg.ndim == 3 ? kMoverBlocksPerCU3D
-                               : (prec == SP_PREC_F64 ? kMoverBlocksPerCU2D64 : kMoverBlocksPerCU2D);
+    // FP64), two for the FP16-record mover on the 512^3 snapshot (-30 %
+    // kernel time against one); SP_MOVER_BLOCKS_PER_CU overrides.
+    const int per_cu_default =
+        c->cfg.ndim == 3 ? (prec == SP_PREC_F32H ? kMoverBlocksPerCU3DH : kMoverBlocksPerCU3D)
+                         : (prec == SP_PREC_F64 ? kMoverBlocksPerCU2D64 : kMoverBlocksPerCU2D);
     int per_cu = std::min(std::max(1, occ), per_cu_default);
     if (c->tuned_bpc > 0) per_cu = std::min(std::max(1, occ), c->tuned_bpc);
     if (const char* e = std::getenv("SP_MOVER_BLOCKS_PER_CU")) {

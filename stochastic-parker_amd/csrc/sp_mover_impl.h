@@ -102,8 +102,8 @@ constexpr bool kTwoWavePair = NDIM == 2 && PPL == 2;
 // three waves per SIMD (141-152 VGPRs, the constants left to the compiler):
 // once its gathers come from the register cache, the step is bound by issue
 // and latency rather than the texture path, and a third wave hides more of
-// the latency than pinned constants save (-9 % kernel time against two
-// pinned waves).  Two-frame steps would spill at that budget.
+// the latency than pinning every constant saves (-9 % kernel time against
+// two pinned waves).  Two-frame steps would spill at that budget.
 template <class Pol, int NDIM, bool TINTERP, int PPL>
 constexpr bool kThreeWave = Pol::kThreeWave2D && NDIM == 2 && !TINTERP && PPL == 1;
 
@@ -117,6 +117,16 @@ mover_kernel(MoverParams m0) {
     // within the two-wave register budget the constants fit in VGPRs for 2D
     // and 3D alike (2D: +5 %, 3D: +1 %)
     if constexpr (!kTwoWavePair<NDIM, PPL> && !kThreeWave<Pol, NDIM, TINTERP, PPL>) pin_constants(m);
+    // at three waves only the FP64 constants of every iteration fit (interval
+    // times, the interpolation origin and scale, the slack bounds): -6 to -9 %
+    // kernel time; the acceleration-gated NLGC step has no room for them
+    constexpr bool kPinHot = kThreeWave<Pol, NDIM, TINTERP, PPL> &&
+                             !((FX & FX_ACC) != 0 && (FX & FX_NLGC) != 0);
+    if constexpr (kPinHot) {
+        pin_vgpr(m.t0); pin_vgpr(m.dt_fine); pin_vgpr(m.dt_limit);
+        pin_vgpr(m.xmin); pin_vgpr(m.ymin); pin_vgpr(m.idx); pin_vgpr(m.idy);
+        pin_vgpr(m.slo[0]); pin_vgpr(m.slo[1]); pin_vgpr(m.shi[0]); pin_vgpr(m.shi[1]);
+    }
     const int lane = threadIdx.x & (kWave - 1);
     double* const px = m.P.d(PD_X);
     double* const py = m.P.d(PD_Y);
