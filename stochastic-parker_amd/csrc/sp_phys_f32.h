@@ -137,8 +137,9 @@ __device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
 // registers across its steps: fields do not change within a mover launch, so
 // a particle that stays in its cell reuses them and only lanes that moved
 // issue loads (the texture path's cost follows the active lanes of a load).
-// Used by the one-frame step of one particle per lane, 2D and 3D FP16 (3D
-// FP32 records would take 160 registers); empty elsewhere.
+// Used by the one-frame step of one particle per lane (3D FP32 records take
+// 160 registers, most of them accumulation registers at one wave per SIMD,
+// and still run 32 % faster); empty elsewhere.
 template <int NDIM, bool TINTERP, bool HALF, bool DPP, int PPL>
 struct GatherCache {
     static constexpr bool kOn = false;
@@ -153,10 +154,10 @@ struct GatherCache<2, false, HALF, false, 1> {
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
 
-template <>
-struct GatherCache<3, false, true, false, 1> {
+template <bool HALF>
+struct GatherCache<3, false, HALF, false, 1> {
     static constexpr bool kOn = true;
-    uint4 raw[8][kRec3H / 4];
+    uint4 raw[8][(HALF ? kRec3H : kRec3F) / 4];
     long long key;   // node index of the cached cell's corner 0; -1: none
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
