@@ -502,7 +502,7 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
     c->nnodes = (long long)c->dp.nxg * c->dp.nyg * c->dp.nzg;
     if (c->dp.ext_rec) c->recw = cfg->ndim == 2 ? sp::kRec2DX : sp::kRec3DX;
     else if (cfg->precision == SP_PREC_F64) c->recw = cfg->ndim == 2 ? sp::kRec2D : sp::kRec3D;
-    else if (cfg->precision == SP_PREC_F32) c->recw = cfg->ndim == 2 ? sp::Pair2Shape::W32 : sp::kRec3F;
+    else if (cfg->precision == SP_PREC_F32) c->recw = cfg->ndim == 2 ? sp::kCell2D : sp::kRec3F;
     else c->recw = cfg->ndim == 2 ? sp::Pair2Shape::W16 : sp::kRec3H;
     c->nrec = cfg->ndim == 3 ? (long long)c->dp.bnx * c->dp.bny * c->dp.bnz * 512 : c->nnodes;
     auto bail = [&](int code) {

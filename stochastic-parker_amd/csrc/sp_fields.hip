@@ -108,10 +108,11 @@ __device__ __forceinline__ uint32_t pack_halves(float a, float b) {
            ((uint32_t)__builtin_bit_cast(uint16_t, (_Float16)b) << 16);
 }
 
-// One thread per node, the record layouts of the FP32 movers (Pair2Shape,
-// kRec3F, kRec3H).  2D: the x-pair record of node (i, j) from the node and
-// its +x neighbour (zeros past the last node of the row, which no cell
-// reads), node-major.  3D: the node's own record, bricked.
+// One thread per node, the record layouts of the FP32 movers (kCell2D,
+// Pair2Shape, kRec3F, kRec3H).  2D FP32: the coefficient record of the cell
+// whose corner 0 is the node.  2D FP16 gradients: the x-pair record of node
+// (i, j) from the node and its +x neighbour (zeros past the last node of the
+// row, which no cell reads), node-major.  3D: the node's own record, bricked.
 template <int NDIM, bool HALF>
 __global__ void __launch_bounds__(256) pack_fast_kernel(DevParams prm, double idxh, double idyh,
                                                         double idzh, const float* __restrict__ n8,
@@ -119,7 +120,7 @@ __global__ void __launch_bounds__(256) pack_fast_kernel(DevParams prm, double id
                                                         float* __restrict__ rec,
                                                         float* __restrict__ fa32) {
     using S = Pair2Shape;
-    constexpr int W = NDIM == 2 ? (HALF ? S::W16 : S::W32) : (HALF ? kRec3H : kRec3F);
+    constexpr int W = NDIM == 2 ? (HALF ? S::W16 : kCell2D) : (HALF ? kRec3H : kRec3F);
     constexpr int N = NDIM == 2 ? S::N : 19;
     const long long total = (long long)prm.nxg * prm.nyg * prm.nzg;
     const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,7 +138,27 @@ __global__ void __launch_bounds__(256) pack_fast_kernel(DevParams prm, double id
     uint32_t wd[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) wd[w] = 0u;
-    if constexpr (NDIM == 2) {
+    if constexpr (NDIM == 2 && !HALF) {
+        // bilinear coefficients of cell (i, j); zeros for the last row and
+        // column of nodes, which start no cell
+        if (i + 1 < prm.nxg && j + 1 < prm.nyg) {
+            float c[3][N];
+            const long long nb[3] = {node + 1, node + prm.nxg, node + prm.nxg + 1};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                node_components(prm, idxh, idyh, idzh, n8, nb[q], out);
+                fast_values<NDIM>(out, c[q]);
+            }
+#pragma unroll
+            for (int s = 0; s < N; ++s) {
+                const float v00 = v[0][s], v10 = c[0][s], v01 = c[1][s], v11 = c[2][s];
+                wd[s] = __float_as_uint(v00);
+                wd[N + s] = __float_as_uint(v10 - v00);
+                wd[2 * N + s] = __float_as_uint(v01 - v00);
+                wd[3 * N + s] = __float_as_uint((v11 - v10) - (v01 - v00));
+            }
+        }
+    } else if constexpr (NDIM == 2) {
         if (i + 1 < prm.nxg) {
             node_components(prm, idxh, idyh, idzh, n8, node + 1, out);
             fast_values<NDIM>(out, v[1]);
@@ -147,18 +168,13 @@ __global__ void __launch_bounds__(256) pack_fast_kernel(DevParams prm, double id
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (!HALF) {
 #pragma unroll
-                for (int s = 0; s < N; ++s) wd[h * N + s] = __float_as_uint(v[h][s]);
-            } else {
+            for (int s = 0; s < S::NF; ++s) wd[h * S::NF + s] = __float_as_uint(v[h][s]);
 #pragma unroll
-                for (int s = 0; s < S::NF; ++s) wd[h * S::NF + s] = __float_as_uint(v[h][s]);
-#pragma unroll
-                for (int q = 0; q < S::NGW; ++q) {
-                    const int s0 = S::NF + 2 * q, s1 = s0 + 1;
-                    wd[2 * S::NF + h * S::NGW + q] =
-                        pack_halves(v[h][s0] * gscale, s1 < N ? v[h][s1] * gscale : 0.0f);
-                }
+            for (int q = 0; q < S::NGW; ++q) {
+                const int s0 = S::NF + 2 * q, s1 = s0 + 1;
+                wd[2 * S::NF + h * S::NGW + q] =
+                    pack_halves(v[h][s0] * gscale, s1 < N ? v[h][s1] * gscale : 0.0f);
             }
         }
     } else if (!HALF) {

@@ -61,6 +61,12 @@ struct Pair2Shape {
     static constexpr int W32 = 28;
     static constexpr int W16 = 20;
 };
+// 2D FP32: cell records of bilinear coefficients.  The record of cell (i, j)
+// holds, per slot s of pair2_src, c0 = v00, c1 = v10 - v00, c2 = v01 - v00,
+// c3 = (v11 - v10) - (v01 - v00) (v_ab the value at node (i + a, j + b)), as
+// [c0 x 14][c1 x 14][c2 x 14][c3 x 14] (224 B, 14 loads), so the step
+// interpolates with three FMAs per value and no corner weights.
+constexpr int kCell2D = 56;
 constexpr int kRec3F = 20;   // 3D FP32 record (floats)
 constexpr int kRec3H = 12;   // 3D FP16 record (words)
 // farray component of each slot; -1 marks divV (dvx/dx + dvy/dy [+ dvz/dz])

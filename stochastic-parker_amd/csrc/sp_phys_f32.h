@@ -27,7 +27,8 @@ __device__ __forceinline__ float fpow(float x, float y) {
 
 struct Interp {
     int ix, iy, iz;
-    float w[8];
+    float rx, ry, rz;   // position inside the cell, [0, 1)
+    float w[8];         // corner weights (unused by the coefficient records)
 };
 
 // The diffusion parameters a batched run varies per particle (KappaVar's
@@ -61,6 +62,9 @@ __device__ __forceinline__ Interp interp_params(const MoverParams& m, double x, 
         rz = (float)(pz - fz);
         I.iz = clamp_cell(fz, m.nzg - 4) + 2;
     }
+    I.rx = rx;
+    I.ry = ry;
+    I.rz = rz;
     const float rx1 = 1.0f - rx, ry1 = 1.0f - ry, rz1 = 1.0f - rz;
     I.w[0] = rx1 * ry1 * rz1;
     I.w[1] = rx * ry1 * rz1;
@@ -148,9 +152,10 @@ struct GatherCache {
 template <bool HALF>
 struct GatherCache<2, false, HALF, false, 1> {
     static constexpr bool kOn = true;
-    static constexpr int kW = HALF ? Pair2Shape::W16 : Pair2Shape::W32;
-    uint4 raw[2][kW / 4];
-    long long key;   // node index of the cached row-0 pair record; -1: none
+    // FP32: the cell's coefficient record; FP16 gradients: its two pair records
+    static constexpr int kN = HALF ? 2 * Pair2Shape::W16 / 4 : kCell2D / 4;
+    uint4 raw[kN];
+    long long key;   // node index of the cell's corner 0; -1: none
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
 
@@ -191,14 +196,14 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
                 for (int j = 0; j < 2; ++j) {
                     const uint4* r4 = reinterpret_cast<const uint4*>(rec + (node0 + j * (long long)m.nxg) * W);
 #pragma unroll
-                    for (int u = 0; u < W / 4; ++u) gc.raw[j][u] = r4[u];
+                    for (int u = 0; u < W / 4; ++u) gc.raw[j * (W / 4) + u] = r4[u];
                 }
                 gc.key = node0;
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int u = 0; u < W / 4; ++u) raw[j][u] = gc.raw[j][u];
+                for (int u = 0; u < W / 4; ++u) raw[j][u] = gc.raw[j * (W / 4) + u];
         } else {
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -237,6 +242,61 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
     if (TINTERP) {
 #pragma unroll
         for (int c = 0; c < S::N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[c] = a[c];   // VX .. BZ
+    f[R2_DVXDX] = a[5];
+    f[R2_DVYDY] = 0.0f;
+#pragma unroll
+    for (int c = R2_DBXDX; c <= R2_DBDY; ++c) f[c] = a[c - 1];
+}
+
+// 2D gather from FP32 cell records (kCell2D): f = c0 + rx c1 + ry (c2 + rx c3)
+// for each value, three FMAs and no corner weights; the bilinear interpolant
+// of the four corners, rounded differently from their weighted sum.  Output in
+// Rec2 slot order, divV in R2_DVXDX and zero in R2_DVYDY.
+template <bool TINTERP, class GC>
+__device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, float ry, int ix,
+                                             int iy, float rt, float* __restrict__ f, GC& gc) {
+    constexpr int N = Pair2Shape::N;
+    constexpr int NU = kCell2D / 4;
+    const long long cell = (long long)ix + (long long)iy * m.nxg;
+    float a[N], g[TINTERP ? N : 1];
+#pragma unroll
+    for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
+        const float* rec = frame == 0 ? m.rec1 : m.rec2;
+        float* acc = frame == 0 ? a : g;
+        uint4 raw[NU];
+        if constexpr (GC::kOn) {
+            if (gc.key != cell) {
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + cell * kCell2D);
+#pragma unroll
+                for (int u = 0; u < NU; ++u) gc.raw[u] = r4[u];
+                gc.key = cell;
+            }
+#pragma unroll
+            for (int u = 0; u < NU; ++u) raw[u] = gc.raw[u];
+        } else {
+            const uint4* r4 = reinterpret_cast<const uint4*>(rec + cell * kCell2D);
+#pragma unroll
+            for (int u = 0; u < NU; ++u) raw[u] = r4[u];
+        }
+        uint32_t wd[kCell2D];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            wd[4 * u] = raw[u].x; wd[4 * u + 1] = raw[u].y;
+            wd[4 * u + 2] = raw[u].z; wd[4 * u + 3] = raw[u].w;
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            const float c0 = __uint_as_float(wd[c]), c1 = __uint_as_float(wd[N + c]);
+            const float c2 = __uint_as_float(wd[2 * N + c]), c3 = __uint_as_float(wd[3 * N + c]);
+            acc[c] = __builtin_fmaf(ry, __builtin_fmaf(rx, c3, c2), __builtin_fmaf(rx, c1, c0));
+        }
+    }
+    if (TINTERP) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
     }
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c] = a[c];   // VX .. BZ
@@ -612,7 +672,8 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (DPP) gather<NDIM, TINTERP, DPP>(m, I, rt, f);
-    else if constexpr (NDIM == 2) gather_pair2<TINTERP, HALF>(m, I, rt, f, gc);
+    else if constexpr (NDIM == 2 && HALF) gather_pair2<TINTERP, HALF>(m, I, rt, f, gc);
+    else if constexpr (NDIM == 2) gather_cell2<TINTERP>(m, I.rx, I.ry, I.ix, I.iy, rt, f, gc);
     else gather_node3<TINTERP, HALF>(m, I, rt, f, gc);
     kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
 }
