@@ -428,7 +428,10 @@ def surrogate_bench(a):
         "accuracy": {"l1_mean": float(err.mean()), "l1_p90": float(np.quantile(err, 0.9)),
                      "l1_max": float(err.max()), "pusher_8192_l1_mean": float(err_sde.mean()),
                      "accepted_queries": float(q_ok.mean()),
-                     "accepted_query_l1_max_mean": float(q_err[q_ok].mean()) if q_ok.any() else None},
+                     "accepted_query_l1_max_mean": float(q_err[q_ok].mean()) if q_ok.any() else None,
+                     "accepted_query_l1_max_max": float(q_err[q_ok].max()) if q_ok.any() else None,
+                     "accepted_within_tol": float((q_err[q_ok] <= 0.05).mean()) if q_ok.any() else None,
+                     "calibration_quantile": 0.95},
         "speed": {"pusher_ms_per_query": ms_sde, "surrogate_us_per_query": 1e6 * t_sweep / 1e5,
                   "speedup": ms_sde * 1e-3 * 1e5 / t_sweep,
                   "host_arrays_queries_per_s": 1e5 / t_host,

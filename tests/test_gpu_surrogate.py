@@ -77,7 +77,10 @@ def test_fallback_runs_the_pusher(trained):
     np.testing.assert_array_equal(f, sde.run(**q))
     # in the domain the calibrated estimate decides: a tolerance below every
     # estimate sends all queries to the pusher, a looser one keeps most on the
-    # network, and the accepted answers meet the tolerance
+    # network, and the accepted answers meet the tolerance (the estimate is
+    # calibrated to cover 95 % of held-out rows; at this test's scale, 4096
+    # particles per training run, the network's error is near 0.05, so the
+    # tolerance checked is 0.1)
     q2 = S.sample_queries(64, 5)
     _, used0 = S.SpectrumService(sur, sde, tol=0.0).answer(**q2)
     assert used0.all()
@@ -88,4 +91,4 @@ def test_fallback_runs_the_pusher(trained):
     print(f"error estimate c0 {sur.c0:.4f} c1 {sur.c1:.3f}; pusher fallback for {used2.mean():.2f}; "
           f"accepted max-over-time L1: {np.round(np.sort(err[~used2])[-5:], 4)}")
     assert used2.mean() < 0.5
-    assert np.mean(err[~used2] <= tol) >= 0.85
+    assert np.mean(err[~used2] <= tol) >= 0.95
