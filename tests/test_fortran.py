@@ -1,7 +1,7 @@
 """The Fortran host side: bind(C) layout, command-line scope checks (CPU), and
-an end-to-end run of the driver over files in the reference's formats
-(mhd_config.dat, mhd_data_NNNN, config/conf.dat) checked against the
-Python host path over the same library (GPU)."""
+end-to-end runs of the driver over files in the reference's formats
+(mhd_config.dat, mhd_data_NNNN, config/conf.dat) checked against the Python
+host path over the same library and, for FP64, against the CPU oracle (GPU)."""
 import os
 import struct
 import subprocess
@@ -173,6 +173,59 @@ def test_driver_end_to_end_matches_host_api(tmp_path, precision, inject):
     for face in ("x", "y"):
         assert e[f"fescaped1_{face}"].shape == esc[0][face].shape
         np.testing.assert_array_equal(e[f"fescaped1_{face}"], esc[0][face])
+
+
+@pytest.mark.gpu
+def test_driver_f64_outputs_match_oracle(tmp_path):
+    """The driver's files against the CPU oracle directly (no host path in
+    between): quick.dat frames 0-2 and fdists_0002.h5 (fglobal, flocal1) of
+    two intervals with splitting, FP64 arithmetic.  The oracle replays the
+    reference's sequence (particle_module.f90:1846-1974, :5430-5480,
+    diagnostics.f90:116-170, :178-284) with the same Philox increments."""
+    import oracle
+
+    _need(DRIVER)
+    n, npart, dtf = 64, 3000, 0.05
+    nodes = write_case(tmp_path, n, dtf)
+    args = [DRIVER, "-dm", "mhd/", "-np", str(npart), "-nm", "20000", "-ts", "0", "-te", "2",
+            "-st", "1", "-df", "1", "-sf", "1", "-sr", "1.3", "-ps", "1.05", "-nl", ".false.",
+            "-ug", "1", "-nd", "2", "-ld", ".true.", "-dd", "data/", "-dt", "1e-7", "-gp", "f64"]
+    print("driver:", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    quick = (tmp_path / "data" / "quick.dat").read_text().splitlines()
+    assert len(quick) == 4
+
+    cfg = workloads.config_2d(n, pbcy=1, dt_max_rel=1e-2, gamma_turb=1.6666667)
+    prm = oracle.params(cfg)
+    fa = oracle.gradients(prm, nodes)
+    box = (0.0, 0.0, 0.0, 1.0, 1.0, 1.0)
+    dist = dict(npp_global=64, nmu_global=1, local_dist=1,
+                local=[dict(enabled=1, npbins=8, nmu=1, rx=4, ry=4, rz=1, pmin=1e-2, pmax=1e4),
+                       None, None, None])
+    ref, _ = oracle.inject(prm, np.zeros(0, oracle.PARTICLE_DTYPE), 20000, npart, 1e-7, 1, 5.0,
+                           0.0, dtf, box, 7.0)
+    nsplit, leak, negp = 0, 0.0, 0.0
+    rows = [(0, oracle.quick_check(ref, nsplit, leak, negp))]
+    for tf in (1, 2):
+        ref, acc = oracle.mover(prm, fa, ref, (tf - 1) * dtf, dtf, num_fine_steps=1, nthreads=8)
+        leak += acc.leak
+        negp += acc.leak_negp
+        ref, ns = oracle.split(prm, ref, 20000, 1.3, 1.05)
+        nsplit += ns
+        rows.append((tf, oracle.quick_check(ref, nsplit, leak, negp)))
+    assert nsplit > 0
+    keys = ("nptl_current", "nptl_split", "ntot", "leak", "leak_negp", "pdt_min", "pdt_max",
+            "pdt_avg")
+    for line, (tf, q) in zip(quick[1:], rows):
+        # I6.6 frame, then 8 E13.6 columns: 6 significant digits
+        assert int(line[:6]) == tf
+        got = [float(line[6 + 13 * i: 6 + 13 * (i + 1)]) for i in range(8)]
+        np.testing.assert_allclose(got, [q[k] for k in keys], rtol=5e-6, atol=1e-300)
+    fg, locs = oracle.distributions(prm, dist, ref)
+    h = read_h5(str(tmp_path / "data" / "fdists_0002.h5"))
+    np.testing.assert_array_equal(h["fglobal"][:, 0], fg)
+    np.testing.assert_array_equal(h["flocal1"].ravel(), locs[0])
 
 
 @pytest.mark.gpu
