@@ -1,7 +1,9 @@
 """Particles per mover lane (1 or 2, FP32 movers): a lane carrying two
 particles steps each from its own Philox counter, so every particle's
 trajectory, step count and the leak sums are bit-identical to one particle per
-lane, whatever the features, the batched variants or the paused launches."""
+lane, whatever the features, the batched variants or the paused launches.  One
+particle per lane keeps its cell's records in registers across steps and two
+do not, so these tests also check that the register cache changes no bit."""
 import os
 
 import numpy as np
@@ -50,6 +52,7 @@ CASES = {
     "2d": (lambda: cfg2d(32, pbcy=1, dt_max_rel=1e-2), "f32", False),
     "3d": (lambda: cfg3d(12, pbcx=1, dt_max_rel=1e-2), "f32", False),
     "3d_f32h": (lambda: cfg3d(12, pbcx=1, dt_max_rel=1e-2), "f32h", False),
+    "2d_f32h": (lambda: cfg2d(32, pbcy=1, dt_max_rel=1e-2), "f32h", False),
     "2d_batched": (lambda: cfg2d(32, pbcy=1, dt_max_rel=1e-2), "f32", True),
     "2d_acc_nlgc_mag": (lambda: cfg2d(32, pbcy=1, dt_max_rel=1e-2, acc_region_flag=1,
                                       acc_region=(0.1, 0.7, 0.2, 0.9, 0.0, 1.0), nlgc=1,
