@@ -40,14 +40,15 @@ enum Rec3 { R3_VX, R3_VY, R3_VZ, R3_BX, R3_BY, R3_BZ, R3_DVXDX, R3_DVYDY, R3_DVZ
 // loads per cell.  FP16 values are stored times a power-of-two scale of the
 // frame (one for the gradients, one for the velocity).
 //
-// 2D: x-pair records.  The record of node (i, j) holds that node and node
-// i + 1 of the same row, so the two x-corners of a cell arrive in one run of
-// loads (the 2D field set stays cache resident at twice the size).  Per node
-// NF = 5 field values (V, B) and NG = 9 gradients (divV, the B-component
-// gradients, grad |B|), in the order of pair2_src:
-//   FP32:  [node i: 14 floats][node i + 1: 14 floats]          (112 B, 7 loads)
-//   FP16:  [node i: 5 floats][node i + 1: 5 floats]
+// 2D values per node, in the order of pair2_src: NF = 5 field values (V, B)
+// and NG = 9 gradients (divV, the B-component gradients, grad |B|).
+// 2D FP16 gradients: x-pair records.  The record of node (i, j) holds that
+// node and node i + 1 of the same row, so the two x-corners of a cell arrive
+// in one run of loads (the 2D field set stays cache resident at twice the
+// size):
+//          [node i: 5 floats][node i + 1: 5 floats]
 //          [node i: 9 halves + 1 pad][node i + 1: same]         (80 B, 5 loads)
+// 2D FP32: cell coefficient records (kCell2D below).
 // 3D: bricked node records (the 3D gather is sensitive to its cache
 // footprint, so nodes are not duplicated), in the order of node3_src:
 //   FP32:  V, B, divV and 12 gradients, 19 floats + 1 pad      (80 B, 5 loads)
@@ -58,8 +59,7 @@ struct Pair2Shape {
     static constexpr int NG = 9;
     static constexpr int N = NF + NG;
     static constexpr int NGW = (NG + 1) / 2;   // words of halves per node
-    static constexpr int W32 = 28;
-    static constexpr int W16 = 20;
+    static constexpr int W16 = 20;             // record width in words
 };
 // 2D FP32: cell records of bilinear coefficients.  The record of cell (i, j)
 // holds, per slot s of pair2_src, c0 = v00, c1 = v10 - v00, c2 = v01 - v00,

@@ -167,17 +167,17 @@ struct GatherCache<3, false, HALF, false, 1> {
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
 
-// 2D gather from x-pair records (Pair2Shape): the record of node (ix, iy + j)
-// holds corners (0, j) and (1, j).  The sums run over the corners in the
-// order of gather() above; FP16 gradients are summed as floats and the
-// frame's scale is undone after the sum (scaling by a power of two is exact).
-// Output in Rec2 slot order, divV in R2_DVXDX and zero in R2_DVYDY (the step
-// only adds the two).
-template <bool TINTERP, bool HALF, class GC>
+// 2D gather from FP16-gradient x-pair records (Pair2Shape): the record of
+// node (ix, iy + j) holds corners (0, j) and (1, j).  The sums run over the
+// corners in the order of gather() above; FP16 gradients are summed as floats
+// and the frame's scale is undone after the sum (scaling by a power of two is
+// exact).  Output in Rec2 slot order, divV in R2_DVXDX and zero in R2_DVYDY
+// (the step only adds the two).
+template <bool TINTERP, class GC>
 __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp& I, float rt,
                                              float* __restrict__ f, GC& gc) {
     using S = Pair2Shape;
-    constexpr int W = HALF ? S::W16 : S::W32;
+    constexpr int W = S::W16;
     float a[S::N], g[TINTERP ? S::N : 1];
 #pragma unroll
     for (int c = 0; c < S::N; ++c) {
@@ -225,19 +225,15 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
                 const float wq = I.w[2 * j + h];
 #pragma unroll
                 for (int c = 0; c < S::N; ++c) {
-                    float v;
-                    if (!HALF) v = __uint_as_float(wd[h * S::N + c]);
-                    else if (c < S::NF) v = __uint_as_float(wd[h * S::NF + c]);
-                    else v = half_at(wd, 2 * S::NF + h * S::NGW, c - S::NF);
+                    const float v = c < S::NF ? __uint_as_float(wd[h * S::NF + c])
+                                              : half_at(wd, 2 * S::NF + h * S::NGW, c - S::NF);
                     acc[c] += v * wq;
                 }
             }
         }
-        if (HALF) {
-            const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
+        const float gs = frame == 0 ? m.f_gs1 : m.f_gs2;
 #pragma unroll
-            for (int c = S::NF; c < S::N; ++c) acc[c] *= gs;
-        }
+        for (int c = S::NF; c < S::N; ++c) acc[c] *= gs;
     }
     if (TINTERP) {
 #pragma unroll
@@ -672,7 +668,7 @@ __device__ __forceinline__ void gather_kappa_push(const MoverParams& m, const KF
     const Interp I = interp_params<NDIM>(m, s.x, s.y, s.z);
     const float rt = TINTERP ? (float)((s.t - m.t0) / m.dtf) : 0.0f;
     if constexpr (DPP) gather<NDIM, TINTERP, DPP>(m, I, rt, f);
-    else if constexpr (NDIM == 2 && HALF) gather_pair2<TINTERP, HALF>(m, I, rt, f, gc);
+    else if constexpr (NDIM == 2 && HALF) gather_pair2<TINTERP>(m, I, rt, f, gc);
     else if constexpr (NDIM == 2) gather_cell2<TINTERP>(m, I.rx, I.ry, I.ix, I.iy, rt, f, gc);
     else gather_node3<TINTERP, HALF>(m, I, rt, f, gc);
     kappa_push<NDIM, FX>(m, kf, s, f, fixed_dt, r, d);
