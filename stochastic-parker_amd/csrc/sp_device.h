@@ -30,6 +30,17 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k) {
 // (both halves at once) instead of a v_mul_lo_u32 / v_mul_hi_u32 pair.
 __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                            uint32_t k0, uint32_t k1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // A scalar zero the compiler cannot see through makes the round keys
+    // depend on the call: they are then formed by scalar adds where they are
+    // used instead of being hoisted out of the mover's loop as twenty
+    // constants, which overflow the scalar registers and come back one
+    // v_readlane per use.
+    uint32_t z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    k0 += z;
+    k1 += z;
+#endif
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r > 0) {
