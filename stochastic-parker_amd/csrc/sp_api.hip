@@ -212,10 +212,13 @@ int validate(const sp_config* c) {
     return SP_OK;
 }
 
-// Step features of a configuration (template argument of the mover).
+// Step features of a configuration (template argument of the mover).  The
+// lean mark goes with exactly the configurations whose flags (mover_params)
+// hold neither MF_MAG_DEP nor MF_DRIFT_2D.
 int step_features(const sp_config& c) {
+    const bool lean = c.mag_dependency != 1 && !(c.ndim == 2 && c.check_drift_2d);
     return (c.acc_region_flag == 1 ? sp::FX_ACC : 0) | (c.dpp_wave || c.dpp_shear ? sp::FX_DPP : 0) |
-           (c.nlgc ? sp::FX_NLGC : 0);
+           (c.nlgc ? sp::FX_NLGC : 0) | (lean ? sp::FX_LEAN : 0);
 }
 
 sp::DevParams make_params(const sp_config& c) {

@@ -94,8 +94,11 @@ __host__ __device__ inline int rec3_src(int s) {
 
 // Step features chosen at compile time (template argument of the mover):
 // the acceleration-region gate, the momentum-diffusion terms and the NLGC
-// perpendicular diffusion coefficient.
-enum StepFeatures { FX_ACC = 1, FX_DPP = 2, FX_NLGC = 4 };
+// perpendicular diffusion coefficient.  FX_LEAN marks a configuration
+// without kappa's magnetic-field dependence and without the 2D out-of-plane
+// drift; the FP32 movers then build the step without the terms those flags
+// zero (the FP64 mover ignores it).
+enum StepFeatures { FX_ACC = 1, FX_DPP = 2, FX_NLGC = 4, FX_LEAN = 8 };
 
 // 3D node records are stored in bricks of 8 x 8 x 8 nodes (bricks x-fastest,
 // nodes x-fastest inside a brick), 32-56 KB each: the 8 corners of a cell share

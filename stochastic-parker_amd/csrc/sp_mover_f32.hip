@@ -7,12 +7,15 @@ namespace sp {
 
 // FP32 gather and coefficients; one or two particles per lane, each with its
 // own copy of the per-particle diffusion parameters (KF).  Steps without
-// momentum diffusion gather from x-pair records (FP32, or FP16 gradients
-// when HALF), those with it from the node records that carry its slots.
+// momentum diffusion gather from compact records (2D: FP32 cell coefficient
+// records, or FP16-gradient x-pair records when HALF; 3D: bricked node
+// records, FP16 values when HALF), those with it from the node records that
+// carry its slots.  Configurations marked FX_LEAN get their own kernels.
 template <bool HALF>
 struct PolicyF32Base {
     static constexpr int kMaxPPL = 2;
     static constexpr bool kThreeWave2D = true;
+    static constexpr bool kLean = true;
     using Var = f32::KF;
     __device__ __forceinline__ static Var var_of(const MoverParams& m) { return f32::kf_of(m); }
     __device__ __forceinline__ static void set_var(MoverParams&, Var& v, const KappaVar& kv) {

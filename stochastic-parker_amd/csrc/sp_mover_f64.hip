@@ -13,6 +13,7 @@ struct PolicyF64 {
     static constexpr bool kDpp = true;
     static constexpr int kMaxPPL = 1;
     static constexpr bool kThreeWave2D = false;
+    static constexpr bool kLean = false;
     struct Var {};
     __device__ __forceinline__ static Var var_of(const MoverParams&) { return Var{}; }
     __device__ __forceinline__ static void set_var(MoverParams& m, Var&, const KappaVar& kv) {

@@ -99,11 +99,10 @@ struct Slot {
 template <int NDIM, int PPL>
 constexpr bool kTwoWavePair = NDIM == 2 && PPL == 2;
 // The one-frame 2D step of the FP32 movers with one particle per lane runs
-// three waves per SIMD (141-152 VGPRs, the constants left to the compiler):
-// once its gathers come from the register cache, the step is bound by issue
-// and latency rather than the texture path, and a third wave hides more of
-// the latency than pinning every constant saves (-9 % kernel time against
-// two pinned waves).  Two-frame steps would spill at that budget.
+// three waves per SIMD (129-168 VGPRs, the constants left to the compiler):
+// with most gathers served by the register cache, a third wave hides more of
+// the step's latency than pinning every constant saves (-9 % kernel time
+// against two pinned waves).  Two-frame steps would spill at that budget.
 template <class Pol, int NDIM, bool TINTERP, int PPL>
 constexpr bool kThreeWave = Pol::kThreeWave2D && NDIM == 2 && !TINTERP && PPL == 1;
 
@@ -236,6 +235,23 @@ mover_kernel(MoverParams m0) {
             do_push[k] = false;
             fixed[k] = false;
             if (q.idx < 0) continue;
+            // ---- the common case: an inner-loop step of a particle inside the
+            // slack bounds with a non-negative momentum, for which the action
+            // choice, the check and the transitions below reduce to "push"
+            // with nothing changed.  Tested with the same comparisons the
+            // check makes, so NaN positions take this path as they pass there.
+            {
+                bool inside = !(q.s.p < 0.0) && !(q.s.x < m.slo[0]) && !(q.s.x > m.shi[0]) &&
+                              !(q.s.y < m.slo[1]) && !(q.s.y > m.shi[1]);
+                if (NDIM == 3) inside = inside && !(q.s.z < m.slo[2]) && !(q.s.z > m.shi[2]);
+                const bool running = q.pending == PEND_NONE && (q.s.t - m.t0) < q.dtarget &&
+                                     q.cf == SP_COUNT_FLAG_INBOX &&
+                                     !(m.step_budget != 0u && q.ksteps >= m.step_budget);
+                if (running && inside) {
+                    do_push[k] = true;
+                    continue;
+                }
+            }
             // ---- choose this slot's action ----
             int act = ACT_NONE;
             bool need_bc = false;
@@ -417,27 +433,36 @@ auto dispatch_acc(bool ti, int fx, F&& f) {
     return with_acc ? f(Variant<N, false, BASE | FX_ACC>{}) : f(Variant<N, false, BASE>{});
 }
 
-template <bool DPP, int N, class F>
-auto dispatch_fx(bool ti, int fx, F&& f) {
+template <bool DPP, int N, int LEAN, class F>
+auto dispatch_fx_base(bool ti, int fx, F&& f) {
     if constexpr (DPP) {
         if (fx & FX_DPP) {
-            if (fx & FX_NLGC) return dispatch_acc<DPP, N, FX_DPP | FX_NLGC>(ti, fx, f);
-            return dispatch_acc<DPP, N, FX_DPP>(ti, fx, f);
+            if (fx & FX_NLGC) return dispatch_acc<DPP, N, LEAN | FX_DPP | FX_NLGC>(ti, fx, f);
+            return dispatch_acc<DPP, N, LEAN | FX_DPP>(ti, fx, f);
         }
     }
-    if (fx & FX_NLGC) return dispatch_acc<DPP, N, FX_NLGC>(ti, fx, f);
-    return dispatch_acc<DPP, N, 0>(ti, fx, f);
+    if (fx & FX_NLGC) return dispatch_acc<DPP, N, LEAN | FX_NLGC>(ti, fx, f);
+    return dispatch_acc<DPP, N, LEAN>(ti, fx, f);
+}
+
+template <bool DPP, bool LEAN, int N, class F>
+auto dispatch_fx(bool ti, int fx, F&& f) {
+    if constexpr (LEAN) {
+        if (fx & FX_LEAN) return dispatch_fx_base<DPP, N, FX_LEAN>(ti, fx, f);
+    }
+    return dispatch_fx_base<DPP, N, 0>(ti, fx, f);
 }
 
 // Calls f(Variant<...>{}) for the runtime (ndim, time_interp, step features)
 // combination; a policy with kOnly3D builds (and serves) 3D variants only, one
-// without kDpp no momentum-diffusion variants.
-template <bool ONLY3D, bool DPP, class F>
+// without kDpp no momentum-diffusion variants, one without kLean ignores
+// FX_LEAN.
+template <class Pol, class F>
 auto dispatch_variant(int ndim, bool ti, int fx, F&& f) {
-    if constexpr (!ONLY3D) {
-        if (ndim == 2) return dispatch_fx<DPP, 2>(ti, fx, f);
+    if constexpr (!Pol::kOnly3D) {
+        if (ndim == 2) return dispatch_fx<Pol::kDpp, Pol::kLean, 2>(ti, fx, f);
     }
-    return dispatch_fx<DPP, 3>(ti, fx, f);
+    return dispatch_fx<Pol::kDpp, Pol::kLean, 3>(ti, fx, f);
 }
 
 // Calls f(v, std::integral_constant<int, PPL>{}) for the runtime particles
@@ -453,7 +478,7 @@ auto dispatch_ppl(int ppl, V v, F&& f) {
 
 template <class Pol>
 int mover_occupancy(int ndim, bool ti, int fx, int ppl) {
-    return dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
+    return dispatch_variant<Pol>(ndim, ti, fx, [&](auto v) {
         return dispatch_ppl<Pol>(ppl, v, [](auto vv, auto pc) {
             using V = decltype(vv);
             int nb = 0;
@@ -467,7 +492,7 @@ int mover_occupancy(int ndim, bool ti, int fx, int ppl) {
 template <class Pol>
 hipError_t launch_mover(const MoverParams& m, bool ti, int fx, int ndim, int ppl, int grid,
                         hipStream_t st) {
-    dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
+    dispatch_variant<Pol>(ndim, ti, fx, [&](auto v) {
         return dispatch_ppl<Pol>(ppl, v, [&](auto vv, auto pc) {
             using V = decltype(vv);
             hipLaunchKernelGGL((mover_kernel<Pol, V::ndim, V::ti, V::fx, decltype(pc)::value>),
@@ -484,7 +509,7 @@ hipError_t launch_step_once(const MoverParams& m, bool ti, int fx, int ndim, dou
                             hipStream_t st) {
     if (m.n <= 0) return hipSuccess;
     const int grid = (int)((m.n + 255) / 256);
-    dispatch_variant<Pol::kOnly3D, Pol::kDpp>(ndim, ti, fx, [&](auto v) {
+    dispatch_variant<Pol>(ndim, ti, fx, [&](auto v) {
         using V = decltype(v);
         hipLaunchKernelGGL((step_once_kernel<Pol, V::ndim, V::ti, V::fx>), dim3(grid), dim3(256),
                            0, st, m, state, rands, fixed, deltas);

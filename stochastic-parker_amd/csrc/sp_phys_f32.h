@@ -445,10 +445,13 @@ __device__ __forceinline__ void dpp_terms(const MoverParams& m, const KF& kf, fl
 // configuration's flags and the per-particle choices (the dt rule's branches,
 // the acceleration gate, the floor) are selects rather than branches, so
 // the step of several particles per lane is one straight-line block.
+// With FX_LEAN (no magnetic-field dependence of kappa, no 2D out-of-plane
+// drift) the terms those flags zero are left out of the arithmetic.
 template <int NDIM, int FX>
 __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, State& s,
                                            const float* f, bool fixed_dt, const float r[4],
                                            double d[4]) {
+    constexpr bool GEN = (FX & FX_LEAN) == 0;
     constexpr bool ACC = (FX & FX_ACC) != 0;
     constexpr bool DPP = (FX & FX_DPP) != 0;
     constexpr bool NLGC = (FX & FX_NLGC) != 0;
@@ -462,7 +465,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, S
     const float ib = bzero ? 0.0f : ibk;         // pusher's 1/B (0 when B = 0)
     const float ib2 = ibk * ibk, ib3 = ibk * ib2;
     const float pf = (float)s.p;
-    const bool mag = (m.flags & MF_MAG_DEP) != 0;
+    const bool mag = GEN && (m.flags & MF_MAG_DEP) != 0;
 
     const bool momdep = (m.flags & MF_MOMENTUM_DEP) != 0;
     const float pnorm = (float)(s.p * m.inv_p0);
@@ -516,16 +519,21 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, S
             dkyy_dy = kperp * ey + ay * by * by + kb3 * by * (dby_dy * b - by * db_dy);
             dkxy_dx = ax * bx * by + kb2 * (dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
             dkxy_dy = ay * bx * by + kb2 * (dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
-        } else {
+        } else if constexpr (GEN) {
             dkxx_dx = kperp * dkdx + kb2 * dkdx * bx * bx + kb3 * bx * (dbx_dx * b - bx * db_dx);
             dkyy_dy = kperp * dkdy + kb2 * dkdy * by * by + kb3 * by * (dby_dy * b - by * db_dy);
             dkxy_dx = kb2 * (dkdx * bx * by + dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
             dkxy_dy = kb2 * (dkdy * bx * by + dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+        } else {
+            dkxx_dx = kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kb3 * by * (dby_dy * b - by * db_dy);
+            dkxy_dx = kb2 * (dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = kb2 * (dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
         }
         const float ibp2 = ib * ib, ibp3 = ib * ibp2;
         const float vdx = vdp * (dbz_dy * ibp2 - 2.0f * bz * db_dy * ibp3);
         const float vdy = vdp * (-(dbz_dx * ibp2) + 2.0f * bz * db_dx * ibp3);
-        const float vdz = (m.flags & MF_DRIFT_2D)
+        const float vdz = (GEN && (m.flags & MF_DRIFT_2D))
             ? vdp * ((dby_dx - dbx_dy) * ibp2 - 2.0f * (by * db_dx - bx * db_dy) * ibp3) : 0.f;
         dx_dt = f[R2_VX] + vdx + dkxx_dx + dkxy_dy;
         dy_dt = f[R2_VY] + vdy + dkxy_dx + dkyy_dy;
@@ -555,7 +563,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, S
             dkxz_dz = az * bx * bz + kb2 * (dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
             dkyz_dy = ay * by * bz + kb2 * (dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
             dkyz_dz = az * by * bz + kb2 * (dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
-        } else {
+        } else if constexpr (GEN) {
             const float dkdx = mag ? db_dx * kf.gm2 : 0.f;
             const float dkdy = mag ? db_dy * kf.gm2 : 0.f;
             const float dkdz = mag ? db_dz * kf.gm2 : 0.f;
@@ -568,6 +576,16 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, S
             dkxz_dz = kb2 * (dkdz * bx * bz + dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
             dkyz_dy = kb2 * (dkdy * by * bz + dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
             dkyz_dz = kb2 * (dkdz * by * bz + dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
+        } else {
+            dkxx_dx = kb3 * bx * (dbx_dx * b - bx * db_dx);
+            dkyy_dy = kb3 * by * (dby_dy * b - by * db_dy);
+            dkzz_dz = kb3 * bz * (dbz_dz * b - bz * db_dz);
+            dkxy_dx = kb2 * (dbx_dx * by + bx * dby_dx) - kb3 * bx * by * db_dx;
+            dkxy_dy = kb2 * (dbx_dy * by + bx * dby_dy) - kb3 * bx * by * db_dy;
+            dkxz_dx = kb2 * (dbx_dx * bz + bx * dbz_dx) - kb3 * bx * bz * db_dx;
+            dkxz_dz = kb2 * (dbx_dz * bz + bx * dbz_dz) - kb3 * bx * bz * db_dz;
+            dkyz_dy = kb2 * (dby_dy * bz + by * dbz_dy) - kb3 * by * bz * db_dy;
+            dkyz_dz = kb2 * (dby_dz * bz + by * dbz_dz) - kb3 * by * bz * db_dz;
         }
         bxn = bx * ib; byn = by * ib; bzn = bz * ib;
         const float bxy2 = bxn * bxn + byn * byn;
@@ -618,7 +636,7 @@ __device__ __forceinline__ void kappa_push(const MoverParams& m, const KF& kf, S
     }
     s.x += (double)deltax;
     s.y += (double)deltay;
-    s.z += (double)deltaz;
+    if (NDIM == 3 || GEN) s.z += (double)deltaz;   // 2D without the drift: deltaz = 0
     s.t += s.dt;
     double dp = DPP ? (double)(dp_dt * dtf32 + r[3] * fsqrt(2.0f * dpp) * sdt) : (double)(dp_dt * dtf32);
     if (ACC) {
