@@ -24,13 +24,15 @@ HBM before timing starts.
 
 Printed JSON line (one):
   value          particle-steps/s over the K timed intervals (FP32 gather path)
-  roofline       mover kernel against its binding roof, the VALU issue rate:
-                 VALU instructions per particle-step (committed rocprofv3
-                 counter run of the same configuration) x particle-steps per
-                 second of mover kernel time (HIP events on the library's
-                 stream) against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz;
-                 beside it the HBM view (algorithmic gather bytes and the
-                 counter-measured HBM bytes per step against 8 TB/s)
+  roofline       mover kernel against its binding roof, the VALU issue rate
+                 or the texture-data (TD) path, whichever is closer: VALU
+                 instructions (or TD busy cycles) per particle-step from the
+                 committed rocprofv3 counter run of the same configuration x
+                 particle-steps per second of mover kernel time (HIP events on
+                 the library's stream) against 256 CUs x 4 SIMDs x 32 lanes x
+                 2.4 GHz (or 256 TD units x 2.4 GHz); beside it the HBM view
+                 (algorithmic gather bytes and the counter-measured HBM bytes
+                 per step against 8 TB/s)
   c2_f32h        the same 2D workload on records with FP16-stored gradients:
                  rate, roofline, f(p) / spatial-map L1 against FP64
   c3             the 3D 512^3 snapshot (1e8 particles) on the FP16 and FP32
@@ -240,8 +242,13 @@ def cpu_baseline(wl, n, nthreads):
     return steps / dt, steps, dt
 
 
-# MI355X non-packed VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
-VALU_LANE_INSTR_PEAK = 256 * 4 * 16 * 2.4e9
+# MI355X VALU issue rate: the published FP32 vector rate (157.3 TFLOP/s, an
+# FMA being one lane-instruction and two flops) = 256 CUs x 4 SIMDs x 32 lanes
+# x 2.4 GHz.  Half of it (16 lanes per SIMD per clock) is not a ceiling: the
+# FP16-gradient 2D mover issues 4.3e13 lane-instructions/s (counter run).
+VALU_LANE_INSTR_PEAK = 256 * 4 * 32 * 2.4e9
+# texture-data (TD) unit cycles: one unit per CU, 256 CUs x 2.4 GHz
+TD_CYCLES_PEAK = 256 * 2.4e9
 COUNTERS = os.path.join(ROOT, "profiles", "mover_counters.json")
 
 
@@ -263,12 +270,14 @@ def counter_run(wl, n, precision):
 
 
 def roofline(wl, n, precision, steps, kernel_ms, launches):
-    """The mover against the VALU issue roof, its arithmetic ceiling (the
-    gathers are served by L2 / MALL, so the algorithmic bytes overstate what
-    reaches HBM), with the texture-path busy fractions and the HBM figures
-    beside it.  Instruction counts and busy fractions come from the committed
-    counter run of this exact configuration; without one the line falls back
-    to the HBM roof."""
+    """The mover against its binding roof: the VALU issue rate (its
+    arithmetic ceiling) or the texture-data path that returns the gathered
+    records to the registers, whichever this run comes closer to; the other
+    and the HBM figures beside it (the gathers are served by L2 / MALL, so the
+    algorithmic bytes overstate what reaches HBM).  Instructions and TD busy
+    cycles per particle-step come from the committed counter run of this exact
+    configuration and are scaled by this run's rate; without one the line
+    falls back to the HBM roof."""
     rate = steps / (kernel_ms * 1e-3)
     alg = bytes_per_step(wl.ndim, precision)
     t = counter_run(wl, n, precision)
@@ -286,10 +295,18 @@ def roofline(wl, n, precision, steps, kernel_ms, launches):
                "valu_instr_per_step": t["valu_instr_per_step"], "launches": launches,
                "launch_ms": kernel_ms / max(launches, 1), "source": t.get("source"), "hbm": hbm}
         if t.get("td_busy") is not None:
-            # the unit the counter run found saturated: the texture data path
-            # returning the gathered node records to the vector registers
+            # the texture data path returning the gathered records to the
+            # vector registers: busy fractions of the counter run
             out["vmem_path"] = {"ta_busy": t.get("ta_busy"), "td_busy": t["td_busy"],
                                 "vmem_instr_per_step": t.get("vmem_instr_per_step")}
+        if t.get("td_cycles_per_step"):
+            td_rate = t["td_cycles_per_step"] * rate
+            if td_rate / TD_CYCLES_PEAK > out["frac"]:
+                valu = {k: out[k] for k in ("achieved", "peak", "unit", "frac")}
+                out.update({"bound": "texture_data", "achieved": td_rate / 1e9,
+                            "peak": TD_CYCLES_PEAK / 1e9, "unit": "G TD busy-cycles/s",
+                            "frac": td_rate / TD_CYCLES_PEAK,
+                            "td_cycles_per_step": t["td_cycles_per_step"], "valu_issue": valu})
         return out
     return {"bound": "hbm", "achieved": hbm["algorithmic_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": hbm["algorithmic_frac"], "traffic": None, "kernel": kern, "launches": launches,
