@@ -137,13 +137,14 @@ __device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
     return (float)__builtin_bit_cast(_Float16, bits);
 }
 
-// Records of the cell a lane's particle last gathered from, kept in
-// registers across its steps: fields do not change within a mover launch, so
-// a particle that stays in its cell reuses them and only lanes that moved
-// issue loads (the texture path's cost follows the active lanes of a load).
-// Used by the one-frame step of one particle per lane (3D FP32 records take
-// 160 registers, most of them accumulation registers at one wave per SIMD,
-// and still run 32 % faster); empty elsewhere.
+// Records a lane's particle last gathered, kept in registers across its
+// steps: fields do not change within a mover launch, so records that are the
+// same at the next step are not loaded again and only the lanes that need new
+// ones issue loads (the texture path's cost follows the active lanes of a
+// load).  2D: the records of the last cell; 3D: the record of each corner
+// parity class.  Used by the one-frame step of one particle per lane (3D FP32
+// records take 160 registers, most of them accumulation registers at one wave
+// per SIMD); empty elsewhere.
 template <int NDIM, bool TINTERP, bool HALF, bool DPP, int PPL>
 struct GatherCache {
     static constexpr bool kOn = false;
@@ -159,12 +160,18 @@ struct GatherCache<2, false, HALF, false, 1> {
     __device__ __forceinline__ void invalidate() { key = -1; }
 };
 
+// 3D: the record of each corner parity class (gather_node3) with the node
+// coordinates of the classes along each axis.  A step into a neighbouring
+// cell keeps the classes whose node the two cells share (four of eight for
+// a move along one axis) and loads only the others.
 template <bool HALF>
 struct GatherCache<3, false, HALF, false, 1> {
     static constexpr bool kOn = true;
     uint4 raw[8][(HALF ? kRec3H : kRec3F) / 4];
-    long long key;   // node index of the cached cell's corner 0; -1: none
-    __device__ __forceinline__ void invalidate() { key = -1; }
+    int nx[2], ny[2], nz[2];   // -1: none
+    __device__ __forceinline__ void invalidate() {
+        nx[0] = nx[1] = ny[0] = ny[1] = nz[0] = nz[1] = -1;
+    }
 };
 
 // 2D gather from FP16-gradient x-pair records (Pair2Shape): the record of
@@ -302,16 +309,29 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
     for (int c = R2_DBXDX; c <= R2_DBDY; ++c) f[c] = a[c - 1];
 }
 
-// 3D gather from the FP32 movers' bricked node records (kRec3F, kRec3H), in
-// the corner order of gather() above.  FP16 values (V with the velocity
-// scale, divV and the gradients with the gradient scale) are summed as
-// floats and scaled after the sum.  Output in Rec3 slot order, divV in
-// R3_DVXDX and zeros in R3_DVYDY, R3_DVZDZ.
+// 3D gather from the FP32 movers' bricked node records (kRec3F, kRec3H).  The
+// eight corners are visited by parity class: class q = (a, b, c) is the
+// corner whose node coordinates have parities a, b, c, i.e. the node
+// (ix + (a ^ ix&1), iy + (b ^ iy&1), iz + (c ^ iz&1)).  Every FP32 3D gather
+// sums in class order, so the cached and uncached kernels round alike.  FP16
+// values (V with the velocity scale, divV and the gradients with the gradient
+// scale) are summed as floats and scaled after the sum.  Output in Rec3 slot
+// order, divV in R3_DVXDX and zeros in R3_DVYDY, R3_DVZDZ.
 template <bool TINTERP, bool HALF, class GC>
 __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp& I, float rt,
                                              float* __restrict__ f, GC& gc) {
     constexpr int W = HALF ? kRec3H : kRec3F;
     constexpr int N = 19;
+    const int px = I.ix & 1, py = I.iy & 1, pz = I.iz & 1;
+    // node coordinate of each parity class along each axis
+    const int nx[2] = {I.ix + px, I.ix + (px ^ 1)};
+    const int ny[2] = {I.iy + py, I.iy + (py ^ 1)};
+    const int nz[2] = {I.iz + pz, I.iz + (pz ^ 1)};
+    // weight factor of each parity class along each axis
+    const float rx1 = 1.0f - I.rx, ry1 = 1.0f - I.ry, rz1 = 1.0f - I.rz;
+    const float wx[2] = {px ? I.rx : rx1, px ? rx1 : I.rx};
+    const float wy[2] = {py ? I.ry : ry1, py ? ry1 : I.ry};
+    const float wz[2] = {pz ? I.rz : rz1, pz ? rz1 : I.rz};
     float a[N], g[TINTERP ? N : 1];
 #pragma unroll
     for (int c = 0; c < N; ++c) {
@@ -322,19 +342,32 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
-        uint4 raw[8][W / 4];   // all eight corners' loads go out before any is used
+        uint4 raw[8][W / 4];   // all eight classes' loads go out before any is used
         if constexpr (GC::kOn) {
-            const long long cell = (long long)I.ix + (long long)m.nxg * (I.iy + (long long)m.nyg * I.iz);
-            if (gc.key != cell) {
+            // a class keeps its node when the cell moved along no axis on
+            // which that class's node coordinate changed
+            bool keep_x[2], keep_y[2], keep_z[2];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-                    const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
+            for (int q = 0; q < 2; ++q) {
+                keep_x[q] = nx[q] == gc.nx[q];
+                keep_y[q] = ny[q] == gc.ny[q];
+                keep_z[q] = nz[q] == gc.nz[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int qa = q & 1, qb = (q >> 1) & 1, qc = (q >> 2) & 1;
+                if (!(keep_x[qa] && keep_y[qb] && keep_z[qc])) {
+                    const long long node = brick_record(nx[qa], ny[qb], nz[qc], m.bnx, m.bny);
                     const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
 #pragma unroll
                     for (int u = 0; u < W / 4; ++u) gc.raw[q][u] = r4[u];
                 }
-                gc.key = cell;
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                gc.nx[q] = nx[q];
+                gc.ny[q] = ny[q];
+                gc.nz[q] = nz[q];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q)
@@ -343,8 +376,8 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int i = q & 1, j = (q >> 1) & 1, k = (q >> 2) & 1;
-                const long long node = brick_record(I.ix + i, I.iy + j, I.iz + k, m.bnx, m.bny);
+                const long long node =
+                    brick_record(nx[q & 1], ny[(q >> 1) & 1], nz[(q >> 2) & 1], m.bnx, m.bny);
                 const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
 #pragma unroll
                 for (int u = 0; u < W / 4; ++u) raw[q][u] = r4[u];
@@ -358,7 +391,7 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
                 wd[4 * u] = raw[q][u].x; wd[4 * u + 1] = raw[q][u].y;
                 wd[4 * u + 2] = raw[q][u].z; wd[4 * u + 3] = raw[q][u].w;
             }
-            const float wq = I.w[q];
+            const float wq = wx[q & 1] * wy[(q >> 1) & 1] * wz[(q >> 2) & 1];
 #pragma unroll
             for (int c = 0; c < N; ++c) {
                 float v;
