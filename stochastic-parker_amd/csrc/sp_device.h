@@ -123,6 +123,48 @@ __device__ __forceinline__ int clamp_cell(double fl, int ncell_max) {
     return (int)c;
 }
 
+// One particle's entry in every particle array.  Copies read the whole row
+// before writing any of it, so the 18 loads go out together (source and
+// destination may alias as far as the compiler knows, and interleaved loads
+// and stores would each wait for the one before).
+struct PtlRow {
+    double x, y, z, p, v, mu, w, t, dt, rdt;
+    int8_t split_times, count_flag;
+    int32_t origin, nsteps_tracked, nsteps_pushed, tag_injected, tag_splitted;
+    uint32_t rng_step;
+};
+
+__device__ __forceinline__ PtlRow load_row(const Ptl& S, long long i) {
+    PtlRow r;
+    r.x = S.x[i]; r.y = S.y[i]; r.z = S.z[i]; r.p = S.p[i]; r.v = S.v[i];
+    r.mu = S.mu[i]; r.w = S.w[i]; r.t = S.t[i]; r.dt = S.dt[i]; r.rdt = S.rdt[i];
+    r.split_times = S.split_times[i];
+    r.count_flag = S.count_flag[i];
+    r.origin = S.origin[i];
+    r.nsteps_tracked = S.nsteps_tracked[i];
+    r.nsteps_pushed = S.nsteps_pushed[i];
+    r.tag_injected = S.tag_injected[i];
+    r.tag_splitted = S.tag_splitted[i];
+    r.rng_step = S.rng_step[i];
+    return r;
+}
+
+// RDT: also the pause record (rdt), which only the spatial sort carries
+template <bool RDT>
+__device__ __forceinline__ void store_row(const Ptl& D, long long o, const PtlRow& r) {
+    D.x[o] = r.x; D.y[o] = r.y; D.z[o] = r.z; D.p[o] = r.p; D.v[o] = r.v;
+    D.mu[o] = r.mu; D.w[o] = r.w; D.t[o] = r.t; D.dt[o] = r.dt;
+    if (RDT) D.rdt[o] = r.rdt;
+    D.split_times[o] = r.split_times;
+    D.count_flag[o] = r.count_flag;
+    D.origin[o] = r.origin;
+    D.nsteps_tracked[o] = r.nsteps_tracked;
+    D.nsteps_pushed[o] = r.nsteps_pushed;
+    D.tag_injected[o] = r.tag_injected;
+    D.tag_splitted[o] = r.tag_splitted;
+    D.rng_step[o] = r.rng_step;
+}
+
 // Per-particle dynamic state carried through the steps of one interval.
 struct State {
     double x, y, z, p, t, dt;

@@ -464,23 +464,7 @@ __global__ void __launch_bounds__(1024) tile_scan_kernel(const unsigned* tile_cn
 }
 
 __device__ __forceinline__ void copy_particle(const Ptl& S, long long i, const Ptl& D, long long o) {
-    D.x[o] = S.x[i];
-    D.y[o] = S.y[i];
-    D.z[o] = S.z[i];
-    D.p[o] = S.p[i];
-    D.v[o] = S.v[i];
-    D.mu[o] = S.mu[i];
-    D.w[o] = S.w[i];
-    D.t[o] = S.t[i];
-    D.dt[o] = S.dt[i];
-    D.split_times[o] = S.split_times[i];
-    D.count_flag[o] = S.count_flag[i];
-    D.origin[o] = S.origin[i];
-    D.nsteps_tracked[o] = S.nsteps_tracked[i];
-    D.nsteps_pushed[o] = S.nsteps_pushed[i];
-    D.tag_injected[o] = S.tag_injected[i];
-    D.tag_splitted[o] = S.tag_splitted[i];
-    D.rng_step[o] = S.rng_step[i];
+    store_row<false>(D, o, load_row(S, i));
 }
 
 __global__ void __launch_bounds__(kBlock) compact_scatter_kernel(Ptl S, Ptl D, long long n,

@@ -126,24 +126,7 @@ __global__ void __launch_bounds__(kBlock) permute_kernel(Ptl S, Ptl D, long long
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const long long i = perm ? (long long)perm[j] : j;
-    D.x[j] = S.x[i];
-    D.y[j] = S.y[i];
-    D.z[j] = S.z[i];
-    D.p[j] = S.p[i];
-    D.v[j] = S.v[i];
-    D.mu[j] = S.mu[i];
-    D.w[j] = S.w[i];
-    D.t[j] = S.t[i];
-    D.dt[j] = S.dt[i];
-    D.rdt[j] = S.rdt[i];
-    D.split_times[j] = S.split_times[i];
-    D.count_flag[j] = S.count_flag[i];
-    D.origin[j] = S.origin[i];
-    D.nsteps_tracked[j] = S.nsteps_tracked[i];
-    D.nsteps_pushed[j] = S.nsteps_pushed[i];
-    D.tag_injected[j] = S.tag_injected[i];
-    D.tag_splitted[j] = S.tag_splitted[i];
-    D.rng_step[j] = S.rng_step[i];
+    store_row<true>(D, j, load_row(S, i));
 }
 
 }  // namespace
