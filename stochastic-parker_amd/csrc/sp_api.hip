@@ -289,7 +289,7 @@ int mover_grid(Ctx* c, long long n, int bpc = 0, int ppl = 1) {
     // a window of consecutive (spatially sorted) indices; a smaller window
     // keeps its field records in that XCD's L2.  Measured on MI355X: three
     // blocks per CU for the FP32 movers on the 2048^2 snapshot (two for
-    // FP64), two for the FP16-record mover on the 512^3 snapshot (-30 %
+    // FP64), two for the FP16-record mover on the 512^3 snapshot (-38 %
     // kernel time against one); SP_MOVER_BLOCKS_PER_CU overrides.
     const int per_cu_default =
         c->cfg.ndim == 3 ? (prec == SP_PREC_F32H ? kMoverBlocksPerCU3DH : kMoverBlocksPerCU3D)
