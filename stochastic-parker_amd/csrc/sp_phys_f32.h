@@ -332,6 +332,20 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
     const float wx[2] = {px ? I.rx : rx1, px ? rx1 : I.rx};
     const float wy[2] = {py ? I.ry : ry1, py ? ry1 : I.ry};
     const float wz[2] = {pz ? I.rz : rz1, pz ? rz1 : I.rz};
+    // brick_record in 32-bit arithmetic (the host refuses grids of 2^32 or
+    // more bricked nodes), its per-axis parts formed once per class
+    unsigned bx[2], by[2], bz[2], lo_x[2], lo_y[2], lo_z[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        bx[q] = (unsigned)nx[q] >> 3; lo_x[q] = (unsigned)nx[q] & 7u;
+        by[q] = (unsigned)ny[q] >> 3; lo_y[q] = ((unsigned)ny[q] & 7u) << 3;
+        bz[q] = (unsigned)nz[q] >> 3; lo_z[q] = ((unsigned)nz[q] & 7u) << 6;
+    }
+    auto record = [&](int q) -> unsigned {
+        const int qa = q & 1, qb = (q >> 1) & 1, qc = (q >> 2) & 1;
+        const unsigned b = __umul24(__umul24(bz[qc], (unsigned)m.bny) + by[qb], (unsigned)m.bnx) + bx[qa];
+        return (b << 9) | lo_z[qc] | lo_y[qb] | lo_x[qa];
+    };
     float a[N], g[TINTERP ? N : 1];
 #pragma unroll
     for (int c = 0; c < N; ++c) {
@@ -357,8 +371,7 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
             for (int q = 0; q < 8; ++q) {
                 const int qa = q & 1, qb = (q >> 1) & 1, qc = (q >> 2) & 1;
                 if (!(keep_x[qa] && keep_y[qb] && keep_z[qc])) {
-                    const long long node = brick_record(nx[qa], ny[qb], nz[qc], m.bnx, m.bny);
-                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)record(q) * W);
 #pragma unroll
                     for (int u = 0; u < W / 4; ++u) gc.raw[q][u] = r4[u];
                 }
@@ -376,9 +389,7 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const long long node =
-                    brick_record(nx[q & 1], ny[(q >> 1) & 1], nz[(q >> 2) & 1], m.bnx, m.bny);
-                const uint4* r4 = reinterpret_cast<const uint4*>(rec + node * W);
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)record(q) * W);
 #pragma unroll
                 for (int u = 0; u < W / 4; ++u) raw[q][u] = r4[u];
             }
