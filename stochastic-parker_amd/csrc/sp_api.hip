@@ -512,10 +512,10 @@ int sp_ctx_create(const sp_config* cfg, void** out_ctx) {
         destroy(c);
         return code;
     };
-    // the FP32 movers index 3D node records in 32 bits
-    if (cfg->ndim == 3 && !c->dp.ext_rec && cfg->precision != SP_PREC_F64 && c->nrec > 0xFFFFFFFFLL)
+    // the FP32 movers index node records in 32 bits
+    if (!c->dp.ext_rec && cfg->precision != SP_PREC_F64 && c->nrec >= 0xFFFFFFFFLL)
         return bail(fail(SP_ERR_UNSUPPORTED,
-                         "3D grid has more than 2^32 bricked nodes: use SP_PREC_F64"));
+                         "grid has 2^32 or more (bricked) nodes: use SP_PREC_F64"));
 #define SP_HIP_C(expr)                                                                           \
     do {                                                                                         \
         hipError_t e_ = (expr);                                                                  \

@@ -156,8 +156,8 @@ struct GatherCache<2, false, HALF, false, 1> {
     // FP32: the cell's coefficient record; FP16 gradients: its two pair records
     static constexpr int kN = HALF ? 2 * Pair2Shape::W16 / 4 : kCell2D / 4;
     uint4 raw[kN];
-    long long key;   // node index of the cell's corner 0; -1: none
-    __device__ __forceinline__ void invalidate() { key = -1; }
+    unsigned key;    // node index of the cell's corner 0; ~0u: none
+    __device__ __forceinline__ void invalidate() { key = ~0u; }
 };
 
 // 3D: the record of each corner parity class (gather_node3) with the node
@@ -195,13 +195,14 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
-        const long long node0 = (long long)I.ix + (long long)I.iy * m.nxg;
+        // node indices in 32 bits (the host refuses 2D grids of 2^32 nodes)
+        const unsigned node0 = (unsigned)I.ix + (unsigned)I.iy * (unsigned)m.nxg;
         uint4 raw[2][W / 4];   // both rows' loads go out before any is used
         if constexpr (GC::kOn) {
             if (gc.key != node0) {
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + (node0 + j * (long long)m.nxg) * W);
+                    const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)(node0 + j * (unsigned)m.nxg) * W);
 #pragma unroll
                     for (int u = 0; u < W / 4; ++u) gc.raw[j * (W / 4) + u] = r4[u];
                 }
@@ -214,7 +215,7 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
         } else {
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const uint4* r4 = reinterpret_cast<const uint4*>(rec + (node0 + j * (long long)m.nxg) * W);
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)(node0 + j * (unsigned)m.nxg) * W);
 #pragma unroll
                 for (int u = 0; u < W / 4; ++u) raw[j][u] = r4[u];
             }
@@ -263,7 +264,8 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
                                              int iy, float rt, float* __restrict__ f, GC& gc) {
     constexpr int N = Pair2Shape::N;
     constexpr int NU = kCell2D / 4;
-    const long long cell = (long long)ix + (long long)iy * m.nxg;
+    // node index in 32 bits (the host refuses 2D grids of 2^32 nodes)
+    const unsigned cell = (unsigned)ix + (unsigned)iy * (unsigned)m.nxg;
     float a[N], g[TINTERP ? N : 1];
 #pragma unroll
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
@@ -272,7 +274,7 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
         uint4 raw[NU];
         if constexpr (GC::kOn) {
             if (gc.key != cell) {
-                const uint4* r4 = reinterpret_cast<const uint4*>(rec + cell * kCell2D);
+                const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)cell * kCell2D);
 #pragma unroll
                 for (int u = 0; u < NU; ++u) gc.raw[u] = r4[u];
                 gc.key = cell;
@@ -280,7 +282,7 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
 #pragma unroll
             for (int u = 0; u < NU; ++u) raw[u] = gc.raw[u];
         } else {
-            const uint4* r4 = reinterpret_cast<const uint4*>(rec + cell * kCell2D);
+            const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)cell * kCell2D);
 #pragma unroll
             for (int u = 0; u < NU; ++u) raw[u] = r4[u];
         }
