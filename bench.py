@@ -42,6 +42,10 @@ Printed JSON line (one):
                  a bounded sample of the same workload, on this host
   accuracy       f(p) and spatial-map L1 of the FP32 path against the FP64
                  path on a particle sample of the same workload
+--time-interp runs the workload on two field frames with time interpolation
+between them (the step of real MHD runs), without the CPU baseline and the
+extra entries; its roofline falls back to the HBM view (the counter runs are
+of one-frame steps).
 """
 from __future__ import annotations
 
@@ -94,9 +98,11 @@ def log(msg):
 class Workload:
     """Snapshot, configuration, interval and injection of one workload."""
 
-    def __init__(self, kind, grid, nsteps, nptl):
+    def __init__(self, kind, grid, nsteps, nptl, ti=False):
         self.kind, self.grid, self.nsteps = kind, grid, nsteps
         self.variants = None
+        self.ti = ti
+        self.nodes2 = None
         t0 = time.perf_counter()
         if kind == "3d":
             self.ndim = 3
@@ -121,6 +127,13 @@ class Workload:
                              f"{self.desc}")
             else:
                 self.dtf = workloads.interval_2d(grid, nsteps)
+        if ti:
+            # a second frame of the same kind (another seed): the mover
+            # interpolates between the two in time
+            seed = workloads.FIELD_SEED + 1
+            self.nodes2 = (workloads.fields_3d(grid, seed=seed) if kind == "3d"
+                           else workloads.fields_2d(grid, grid, seed=seed))
+            self.desc += ", two frames with time interpolation"
         self.nptl = int(self.variants["nptl"].sum()) if self.variants is not None else int(nptl)
         log(f"generated {self.desc} in {time.perf_counter() - t0:.1f} s")
 
@@ -128,7 +141,7 @@ class Workload:
         """sp_config of a run on this workload.  In 3D the FP32 movers run
         fastest sorted once per interval; the FP64 path re-sorts every 200
         steps (results do not depend on it)."""
-        c = dict(self.cfg, precision=precision, nptl_max=n)
+        c = dict(self.cfg, precision=precision, nptl_max=n, time_interp=1 if self.ti else 0)
         if self.kind == "3d" and precision == "f64":
             c["sort_steps"] = 200
         return c
@@ -141,6 +154,9 @@ class Workload:
         """Upload fields, set diagnostics and inject n particles (all variants)."""
         pz.upload_fields(self.nodes)
         pz.calc_fields_gradients()
+        if self.ti:
+            pz.upload_fields(self.nodes2, slot=1)
+            pz.calc_fields_gradients(slot=1)
         pz.init_particle_distributions(**self.dist)
         if self.variants is not None:
             v = self.variants
@@ -255,7 +271,9 @@ COUNTERS = os.path.join(ROOT, "profiles", "mover_counters.json")
 def counter_run(wl, n, precision):
     """The committed rocprofv3 counter figures of the mover on exactly this
     configuration (workload, grid, particles, steps per interval, precision),
-    or None."""
+    or None (the counter runs are of one-frame steps)."""
+    if wl.ti:
+        return None
     try:
         with open(COUNTERS) as f:
             t = json.load(f).get(f"{wl.kind}_{precision}")
@@ -477,6 +495,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-interp", action="store_true",
+                    help="two field frames with time interpolation between them (the step "
+                         "of real MHD runs); no CPU baseline or extra entries")
     ap.add_argument("--no-extras", "--no-3d", dest="no_extras", action="store_true",
                     help="skip the extra entries of the 2D run's line (FP16-record 2D path, "
                          "3D snapshot)")
@@ -491,8 +512,10 @@ def main():
 
     d = DEFAULTS[a.workload]
     a.precision = a.precision or d["precision"]
+    if a.time_interp:
+        a.no_cpu = a.no_extras = True
     wl = Workload(a.workload, a.grid or d["grid"], a.nsteps or d["nsteps"],
-                  d["nptl"] if a.nptl < 0 else a.nptl)
+                  d["nptl"] if a.nptl < 0 else a.nptl, ti=a.time_interp)
     n = wl.nptl
     with ParkerPusher(**wl.config(a.precision, n)) as pz:
         wl.setup(pz, n)

@@ -150,12 +150,13 @@ struct GatherCache {
     static constexpr bool kOn = false;
     __device__ __forceinline__ void invalidate() {}
 };
-template <bool HALF>
-struct GatherCache<2, false, HALF, false, 1> {
+template <bool TINTERP, bool HALF>
+struct GatherCache<2, TINTERP, HALF, false, 1> {
     static constexpr bool kOn = true;
-    // FP32: the cell's coefficient record; FP16 gradients: its two pair records
+    // FP32: the cell's coefficient record; FP16 gradients: its two pair
+    // records; one set per field frame
     static constexpr int kN = HALF ? 2 * Pair2Shape::W16 / 4 : kCell2D / 4;
-    uint4 raw[kN];
+    uint4 raw[TINTERP ? 2 : 1][kN];
     unsigned key;    // node index of the cell's corner 0; ~0u: none
     __device__ __forceinline__ void invalidate() { key = ~0u; }
 };
@@ -191,27 +192,28 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
         a[c] = 0.0f;
         if (TINTERP) g[c] = 0.0f;
     }
+    // node indices in 32 bits (the host refuses 2D grids of 2^32 nodes)
+    const unsigned node0 = (unsigned)I.ix + (unsigned)I.iy * (unsigned)m.nxg;
+    bool miss = false;
+    if constexpr (GC::kOn) miss = gc.key != node0;
 #pragma unroll
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
-        // node indices in 32 bits (the host refuses 2D grids of 2^32 nodes)
-        const unsigned node0 = (unsigned)I.ix + (unsigned)I.iy * (unsigned)m.nxg;
         uint4 raw[2][W / 4];   // both rows' loads go out before any is used
         if constexpr (GC::kOn) {
-            if (gc.key != node0) {
+            if (miss) {
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)(node0 + j * (unsigned)m.nxg) * W);
 #pragma unroll
-                    for (int u = 0; u < W / 4; ++u) gc.raw[j * (W / 4) + u] = r4[u];
+                    for (int u = 0; u < W / 4; ++u) gc.raw[frame][j * (W / 4) + u] = r4[u];
                 }
-                gc.key = node0;
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int u = 0; u < W / 4; ++u) raw[j][u] = gc.raw[j * (W / 4) + u];
+                for (int u = 0; u < W / 4; ++u) raw[j][u] = gc.raw[frame][j * (W / 4) + u];
         } else {
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -243,6 +245,7 @@ __device__ __forceinline__ void gather_pair2(const MoverParams& m, const Interp&
 #pragma unroll
         for (int c = S::NF; c < S::N; ++c) acc[c] *= gs;
     }
+    if constexpr (GC::kOn) gc.key = node0;
     if (TINTERP) {
 #pragma unroll
         for (int c = 0; c < S::N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
@@ -266,6 +269,8 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
     constexpr int NU = kCell2D / 4;
     // node index in 32 bits (the host refuses 2D grids of 2^32 nodes)
     const unsigned cell = (unsigned)ix + (unsigned)iy * (unsigned)m.nxg;
+    bool miss = false;
+    if constexpr (GC::kOn) miss = gc.key != cell;
     float a[N], g[TINTERP ? N : 1];
 #pragma unroll
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
@@ -273,14 +278,13 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
         float* acc = frame == 0 ? a : g;
         uint4 raw[NU];
         if constexpr (GC::kOn) {
-            if (gc.key != cell) {
+            if (miss) {
                 const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)cell * kCell2D);
 #pragma unroll
-                for (int u = 0; u < NU; ++u) gc.raw[u] = r4[u];
-                gc.key = cell;
+                for (int u = 0; u < NU; ++u) gc.raw[frame][u] = r4[u];
             }
 #pragma unroll
-            for (int u = 0; u < NU; ++u) raw[u] = gc.raw[u];
+            for (int u = 0; u < NU; ++u) raw[u] = gc.raw[frame][u];
         } else {
             const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)cell * kCell2D);
 #pragma unroll
@@ -299,6 +303,7 @@ __device__ __forceinline__ void gather_cell2(const MoverParams& m, float rx, flo
             acc[c] = __builtin_fmaf(ry, __builtin_fmaf(rx, c3, c2), __builtin_fmaf(rx, c1, c0));
         }
     }
+    if constexpr (GC::kOn) gc.key = cell;
     if (TINTERP) {
 #pragma unroll
         for (int c = 0; c < N; ++c) a[c] = a[c] * (1.0f - rt) + g[c] * rt;
