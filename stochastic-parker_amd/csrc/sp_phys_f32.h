@@ -165,10 +165,10 @@ struct GatherCache<2, TINTERP, HALF, false, 1> {
 // coordinates of the classes along each axis.  A step into a neighbouring
 // cell keeps the classes whose node the two cells share (four of eight for
 // a move along one axis) and loads only the others.
-template <bool HALF>
-struct GatherCache<3, false, HALF, false, 1> {
+template <bool TINTERP, bool HALF>
+struct GatherCache<3, TINTERP, HALF, false, 1> {
     static constexpr bool kOn = true;
-    uint4 raw[8][(HALF ? kRec3H : kRec3F) / 4];
+    uint4 raw[TINTERP ? 2 : 1][8][(HALF ? kRec3H : kRec3F) / 4];   // per field frame
     int nx[2], ny[2], nz[2];   // -1: none
     __device__ __forceinline__ void invalidate() {
         nx[0] = nx[1] = ny[0] = ny[1] = nz[0] = nz[1] = -1;
@@ -359,40 +359,36 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
         a[c] = 0.0f;
         if (TINTERP) g[c] = 0.0f;
     }
+    // a class keeps its node when the cell moved along no axis on which that
+    // class's node coordinate changed
+    bool keep_x[2] = {false, false}, keep_y[2] = {false, false}, keep_z[2] = {false, false};
+    if constexpr (GC::kOn) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            keep_x[q] = nx[q] == gc.nx[q];
+            keep_y[q] = ny[q] == gc.ny[q];
+            keep_z[q] = nz[q] == gc.nz[q];
+        }
+    }
 #pragma unroll
     for (int frame = 0; frame < (TINTERP ? 2 : 1); ++frame) {
         const float* rec = frame == 0 ? m.rec1 : m.rec2;
         float* acc = frame == 0 ? a : g;
         uint4 raw[8][W / 4];   // all eight classes' loads go out before any is used
         if constexpr (GC::kOn) {
-            // a class keeps its node when the cell moved along no axis on
-            // which that class's node coordinate changed
-            bool keep_x[2], keep_y[2], keep_z[2];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                keep_x[q] = nx[q] == gc.nx[q];
-                keep_y[q] = ny[q] == gc.ny[q];
-                keep_z[q] = nz[q] == gc.nz[q];
-            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int qa = q & 1, qb = (q >> 1) & 1, qc = (q >> 2) & 1;
                 if (!(keep_x[qa] && keep_y[qb] && keep_z[qc])) {
                     const uint4* r4 = reinterpret_cast<const uint4*>(rec + (size_t)record(q) * W);
 #pragma unroll
-                    for (int u = 0; u < W / 4; ++u) gc.raw[q][u] = r4[u];
+                    for (int u = 0; u < W / 4; ++u) gc.raw[frame][q][u] = r4[u];
                 }
-            }
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                gc.nx[q] = nx[q];
-                gc.ny[q] = ny[q];
-                gc.nz[q] = nz[q];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q)
 #pragma unroll
-                for (int u = 0; u < W / 4; ++u) raw[q][u] = gc.raw[q][u];
+                for (int u = 0; u < W / 4; ++u) raw[q][u] = gc.raw[frame][q][u];
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -426,6 +422,14 @@ __device__ __forceinline__ void gather_node3(const MoverParams& m, const Interp&
             for (int c = 0; c < 3; ++c) acc[c] *= vs;
 #pragma unroll
             for (int c = 6; c < N; ++c) acc[c] *= gs;
+        }
+    }
+    if constexpr (GC::kOn) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            gc.nx[q] = nx[q];
+            gc.ny[q] = ny[q];
+            gc.nz[q] = nz[q];
         }
     }
     if (TINTERP) {
