@@ -142,9 +142,9 @@ __device__ __forceinline__ float half_at(const uint32_t* wd, int w0, int t) {
 // same at the next step are not loaded again and only the lanes that need new
 // ones issue loads (the texture path's cost follows the active lanes of a
 // load).  2D: the records of the last cell; 3D: the record of each corner
-// parity class.  Used by the one-frame step of one particle per lane (3D FP32
-// records take 160 registers, most of them accumulation registers at one wave
-// per SIMD); empty elsewhere.
+// parity class; one set per field frame.  Used by the steps of one particle
+// per lane (3D FP32 records take 160 registers per frame, most of them
+// accumulation registers at one wave per SIMD); empty elsewhere.
 template <int NDIM, bool TINTERP, bool HALF, bool DPP, int PPL>
 struct GatherCache {
     static constexpr bool kOn = false;
