@@ -44,8 +44,8 @@ Printed JSON line (one):
                  path on a particle sample of the same workload
 --time-interp runs the workload on two field frames with time interpolation
 between them (the step of real MHD runs), without the CPU baseline and the
-extra entries; its roofline falls back to the HBM view (the counter runs are
-of one-frame steps).
+extra entries; its roofline takes the counter run of the two-frame
+configuration, and its algorithmic bytes count both frames' records.
 """
 from __future__ import annotations
 
@@ -67,15 +67,16 @@ HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30
 
 
-def bytes_per_step(ndim, precision):
+def bytes_per_step(ndim, precision, frames=1):
     """Algorithmic gather bytes of one particle-step: the stored bytes of the
-    node values the step consumes x the corners of its interpolation cell.
+    node values the step consumes x the corners of its interpolation cell x
+    the field frames it interpolates between (two under time interpolation).
     FP32 records hold V, B, divV and the B and |B| gradients: 2D 14 FP32
     values per corner (224 B), 3D 19 (608 B).  FP16 records: 2D V and B as 5
     FP32 and 9 FP16 gradients (152 B); 3D B as 3 FP32 and V, divV and the 12
     gradients as 16 FP16 (352 B).  The FP64 path reads 15 / 21 FP32 node
     values (240 / 672 B)."""
-    corners = 4 if ndim == 2 else 8
+    corners = (4 if ndim == 2 else 8) * frames
     if precision == "f64":
         return corners * (15 if ndim == 2 else 21) * 4
     if precision == "f32h":
@@ -270,18 +271,17 @@ COUNTERS = os.path.join(ROOT, "profiles", "mover_counters.json")
 
 def counter_run(wl, n, precision):
     """The committed rocprofv3 counter figures of the mover on exactly this
-    configuration (workload, grid, particles, steps per interval, precision),
-    or None (the counter runs are of one-frame steps)."""
-    if wl.ti:
-        return None
+    configuration (workload, grid, particles, steps per interval, precision,
+    one or two field frames), or None."""
     try:
         with open(COUNTERS) as f:
-            t = json.load(f).get(f"{wl.kind}_{precision}")
+            t = json.load(f).get(f"{wl.kind}_{'ti_' if wl.ti else ''}{precision}")
     except (OSError, ValueError):
         return None
     if not t:
         return None
-    want = dict(grid=wl.grid, nptl=n, nsteps=wl.nsteps, precision=precision)
+    want = dict(grid=wl.grid, nptl=n, nsteps=wl.nsteps, precision=precision,
+                time_interp=bool(wl.ti))
     if any(t.get(k) != v for k, v in want.items()):
         return None
     return t
@@ -297,7 +297,7 @@ def roofline(wl, n, precision, steps, kernel_ms, launches):
     configuration and are scaled by this run's rate; without one the line
     falls back to the HBM roof."""
     rate = steps / (kernel_ms * 1e-3)
-    alg = bytes_per_step(wl.ndim, precision)
+    alg = bytes_per_step(wl.ndim, precision, 2 if wl.ti else 1)
     t = counter_run(wl, n, precision)
     hbm = {"algorithmic_bytes_per_step": alg, "algorithmic_GBs": alg * rate / 1e9,
            "algorithmic_frac": alg * rate / 1e9 / HBM_PEAK_GBS, "peak_GBs": HBM_PEAK_GBS,
