@@ -70,8 +70,11 @@ def test_mover_dpp_matches_oracle(ndim):
     _mover_case(c, dpp_nodes(c), 1500, 0.2)
 
 
-def test_mover_dpp_weak_scattering_with_time_interpolation():
-    c = dict(cfg2d(32, pbcy=1, dt_max_rel=1e-2, time_interp=1), weak_scattering=1, **DPP)
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_mover_dpp_weak_scattering_with_time_interpolation(ndim):
+    c = (cfg2d(32, pbcy=1, dt_max_rel=1e-2, time_interp=1) if ndim == 2
+         else cfg3d(12, pbcx=1, dt_max_rel=1e-2, time_interp=1))
+    c = dict(c, weak_scattering=1, **DPP)
     _mover_case(c, dpp_nodes(c, seed=1), 1500, 0.2, nodes2=dpp_nodes(c, seed=2))
 
 

@@ -184,8 +184,10 @@ def test_mover_f64_multi_interval_with_split():
     _mover_case(c, nodes_for(c), 1500, 0.2, intervals=3, split=(1.2, 1.05))
 
 
-def test_mover_f64_time_interpolation():
-    c = cfg2d(32, pbcy=1, time_interp=1, dt_max_rel=1e-2)
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_mover_f64_time_interpolation(ndim):
+    c = (cfg2d(32, pbcy=1, time_interp=1, dt_max_rel=1e-2) if ndim == 2
+         else cfg3d(12, pbcx=1, time_interp=1, dt_max_rel=1e-2))
     n1 = nodes_for(c, seed=1)
     n2 = nodes_for(c, seed=2)
     _mover_case(c, n1, 1500, 0.2, nodes2=n2)
